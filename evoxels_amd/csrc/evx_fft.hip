@@ -1,0 +1,752 @@
+// Real 3-D FFTs and the semi-implicit spectral steps.
+//
+// Spectrum layout: (channels, nx, ny, nh) complex, nh = nz/2 + 1, kz fastest,
+// the same index order as torch.fft.rfftn's output (voxelgrid.py:203-204).
+//
+// Transforms are mixed-radix Stockham (self-sorting, decimation in time)
+// passes on lines staged in LDS.  A workgroup owns `lpw` lines laid out
+// element-major (LDS index e*lpw + line) and `tpl` threads per line; thread
+// `tid` works on line tid % lpw.  Radices 2, 3, 4, 5 and 8 have closed-form
+// butterflies; any other prime factor up to 31 uses a generic O(R^2) DFT.
+//
+// Sweeps of one IMEX step (each reads and writes HBM once):
+//   z : RHS lines (pairs packed as one complex line) -> half spectra
+//   y : forward C2C along y
+//   x : forward C2C along x, times prefac(k)/N, inverse C2C along x
+//   y : inverse C2C along y
+//   z : C2R of line pairs, u_out = u + update
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "evx_common.hpp"
+
+namespace evx {
+
+int stencil_rhs_ch_periodic(int32_t dtype, const void* u, void* out, const Dims& d, const double div_dx[3],
+                            const double div_dx2[3], double eps, double D, hipStream_t s);
+int stencil_rhs_rd_periodic(int32_t dtype, const void* u, void* out, const Dims& d, const double div_dx2[3],
+                            double D, hipStream_t s);
+
+namespace {
+
+constexpr int MAXPASS = 24;
+constexpr int EMAX = 32;           // max complex values a thread holds in one pass
+constexpr size_t LDS_BUDGET = 64 * 1024;
+
+template <typename T>
+struct Axis {
+  int n;
+  int npass;
+  int tpl;
+  int radix[MAXPASS];
+  const cpx<T>* tw;  // n roots of unity exp(-2 pi i m / n)
+};
+
+// ------------------------------------------------------------ butterflies --
+template <typename T, bool INV>
+__device__ __forceinline__ cpx<T> mul_mi(cpx<T> z) {  // times -i (forward) or +i (inverse)
+  return INV ? cpx<T>{-z.y, z.x} : cpx<T>{z.y, -z.x};
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft2(cpx<T>* x) {
+  const cpx<T> a = x[0], b = x[1];
+  x[0] = cadd(a, b);
+  x[1] = csub(a, b);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft4(cpx<T>& a0, cpx<T>& a1, cpx<T>& a2, cpx<T>& a3) {
+  const cpx<T> s0 = cadd(a0, a2), d0 = csub(a0, a2), s1 = cadd(a1, a3), d1 = mul_mi<T, INV>(csub(a1, a3));
+  a0 = cadd(s0, s1);
+  a2 = csub(s0, s1);
+  a1 = cadd(d0, d1);
+  a3 = csub(d0, d1);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft3(cpx<T>* x) {
+  const T h = T(0.86602540378443864676);  // sin(2 pi / 3)
+  const cpx<T> a = cadd(x[1], x[2]), b = csub(x[1], x[2]);
+  const cpx<T> y0 = cadd(x[0], a);
+  const cpx<T> t = {x[0].x - T(0.5) * a.x, x[0].y - T(0.5) * a.y};
+  const cpx<T> v = mul_mi<T, INV>(cscale(b, h));
+  x[0] = y0;
+  x[1] = cadd(t, v);
+  x[2] = csub(t, v);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft5(cpx<T>* x) {
+  const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);  // cos(2pi/5), cos(4pi/5)
+  const T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);   // sin(2pi/5), sin(4pi/5)
+  const cpx<T> a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
+  const cpx<T> a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
+  const cpx<T> x0 = x[0];
+  const cpx<T> t1 = {x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y};
+  const cpx<T> t2 = {x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y};
+  const cpx<T> u1 = mul_mi<T, INV>(cpx<T>{s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y});
+  const cpx<T> u2 = mul_mi<T, INV>(cpx<T>{s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y});
+  x[0] = cadd(x0, cadd(a1, a2));
+  x[1] = cadd(t1, u1);
+  x[4] = csub(t1, u1);
+  x[2] = cadd(t2, u2);
+  x[3] = csub(t2, u2);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft8(cpx<T>* x) {
+  const T h = T(0.70710678118654752440);  // sqrt(2)/2
+  cpx<T> a0 = cadd(x[0], x[4]), b0 = csub(x[0], x[4]);
+  cpx<T> a1 = cadd(x[1], x[5]), b1 = csub(x[1], x[5]);
+  cpx<T> a2 = cadd(x[2], x[6]), b2 = csub(x[2], x[6]);
+  cpx<T> a3 = cadd(x[3], x[7]), b3 = csub(x[3], x[7]);
+  // b_r *= w8^r (forward w8 = e^{-i pi/4}; inverse uses the conjugate)
+  b1 = INV ? cpx<T>{(b1.x - b1.y) * h, (b1.y + b1.x) * h} : cpx<T>{(b1.x + b1.y) * h, (b1.y - b1.x) * h};
+  b2 = mul_mi<T, INV>(b2);
+  b3 = INV ? cpx<T>{(-b3.x - b3.y) * h, (b3.x - b3.y) * h} : cpx<T>{(b3.y - b3.x) * h, (-b3.x - b3.y) * h};
+  dft4<T, INV>(a0, a1, a2, a3);
+  dft4<T, INV>(b0, b1, b2, b3);
+  x[0] = a0; x[2] = a1; x[4] = a2; x[6] = a3;
+  x[1] = b0; x[3] = b1; x[5] = b2; x[7] = b3;
+}
+
+template <typename T, bool INV, int R>
+__device__ __forceinline__ void dft_fixed(cpx<T>* x) {
+  if constexpr (R == 2) dft2<T, INV>(x);
+  else if constexpr (R == 3) dft3<T, INV>(x);
+  else if constexpr (R == 4) dft4<T, INV>(x[0], x[1], x[2], x[3]);
+  else if constexpr (R == 5) dft5<T, INV>(x);
+  else dft8<T, INV>(x);
+}
+
+// One Stockham pass of radix R over every line of the workgroup.
+// p = product of the radices already applied.  Butterfly i (0 <= i < n/R)
+// reads elements i + r*n/R, twiddles by w_{pR}^{r*(i mod p)}, and writes
+// (i - i mod p)*R + i mod p + r*p.
+template <typename T, bool INV, int R>
+__device__ __forceinline__ void pass_fixed(cpx<T>* buf, int lpw, int line, int tl, int tpl, int n, int p,
+                                           const cpx<T>* tw, bool active) {
+  constexpr int SMAX = EMAX / R;
+  const int m = n / R;
+  const int stride = n / (p * R);
+  cpx<T> y[SMAX][R];
+#pragma unroll
+  for (int s = 0; s < SMAX; ++s) {
+    const int i = tl + s * tpl;
+    if (active && i < m) {
+      const int k = i % p;
+      cpx<T> x[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = buf[(i + r * m) * lpw + line];
+      if (p > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const cpx<T> w = tw[r * k * stride];
+          x[r] = INV ? cmulc(x[r], w) : cmul(x[r], w);
+        }
+      }
+      dft_fixed<T, INV, R>(x);
+#pragma unroll
+      for (int r = 0; r < R; ++r) y[s][r] = x[r];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < SMAX; ++s) {
+    const int i = tl + s * tpl;
+    if (active && i < m) {
+      const int k = i % p;
+      const int j = (i - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(j + r * p) * lpw + line] = y[s][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Generic odd-prime radix: y_q = sum_r (x_r w^{rk}) w_R^{rq}, O(R^2).
+template <typename T, bool INV>
+__device__ __noinline__ void pass_generic(cpx<T>* buf, int lpw, int line, int tl, int tpl, int n, int p, int R,
+                                          const cpx<T>* tw, bool active) {
+  const int m = n / R;
+  const int stride = n / (p * R);
+  const int nr = n / R;
+  cpx<T> y[EMAX];
+  int cnt = 0;
+  for (int i = tl; active && i < m; i += tpl) {
+    const int k = i % p;
+    for (int q = 0; q < R; ++q) {
+      cpx<T> acc = {T(0), T(0)};
+      for (int r = 0; r < R; ++r) {
+        cpx<T> xr = buf[(i + r * m) * lpw + line];
+        const cpx<T> w = tw[r * k * stride];
+        xr = INV ? cmulc(xr, w) : cmul(xr, w);
+        const cpx<T> wq = tw[((r * q) % R) * nr];
+        acc = cadd(acc, INV ? cmulc(xr, wq) : cmul(xr, wq));
+      }
+      y[cnt++] = acc;
+    }
+  }
+  __syncthreads();
+  cnt = 0;
+  for (int i = tl; active && i < m; i += tpl) {
+    const int k = i % p;
+    const int j = (i - k) * R + k;
+    for (int q = 0; q < R; ++q) buf[(j + q * p) * lpw + line] = y[cnt++];
+  }
+  __syncthreads();
+}
+
+template <typename T, bool INV>
+__device__ void fft_lines(cpx<T>* buf, int lpw, int line, int tl, const Axis<T>& a, const cpx<T>* tw,
+                          bool active) {
+  int p = 1;
+  for (int s = 0; s < a.npass; ++s) {
+    const int R = a.radix[s];
+    switch (R) {
+      case 8: pass_fixed<T, INV, 8>(buf, lpw, line, tl, a.tpl, a.n, p, tw, active); break;
+      case 4: pass_fixed<T, INV, 4>(buf, lpw, line, tl, a.tpl, a.n, p, tw, active); break;
+      case 2: pass_fixed<T, INV, 2>(buf, lpw, line, tl, a.tpl, a.n, p, tw, active); break;
+      case 3: pass_fixed<T, INV, 3>(buf, lpw, line, tl, a.tpl, a.n, p, tw, active); break;
+      case 5: pass_fixed<T, INV, 5>(buf, lpw, line, tl, a.tpl, a.n, p, tw, active); break;
+      default: pass_generic<T, INV>(buf, lpw, line, tl, a.tpl, a.n, p, R, tw, active); break;
+    }
+    p *= R;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_twiddles(cpx<T>* tw, const Axis<T>& a) {
+  for (int i = threadIdx.x; i < a.n; i += blockDim.x) tw[i] = a.tw[i];
+}
+
+// --------------------------------------------------------------- z sweeps --
+// Real lines of length n along z; line pairs (2q, 2q+1) form one complex line.
+template <typename T>
+__global__ void z_r2c_kernel(const T* __restrict__ x, cpx<T>* __restrict__ spec, int64_t nlines, int nh,
+                             Axis<T> a, int lpw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* buf = tw + a.n;
+  const int n = a.n, tid = threadIdx.x;
+  load_twiddles(tw, a);
+  const int64_t l0 = (int64_t)blockIdx.x * lpw * 2;
+  const int nl = (int)min((int64_t)2 * lpw, nlines - l0);
+  const int np = (nl + 1) / 2;
+  const T* src = x + l0 * n;
+  for (int q = tid; q < 2 * np * n; q += blockDim.x) {
+    const int l = q / n, e = q - (q / n) * n;
+    const T v = l < nl ? src[q] : T(0);
+    T* z = reinterpret_cast<T*>(&buf[e * lpw + (l >> 1)]);
+    z[l & 1] = v;
+  }
+  __syncthreads();
+  const int line = tid % lpw, tl = tid / lpw;
+  fft_lines<T, false>(buf, lpw, line, tl, a, tw, line < np && tl < a.tpl);
+  cpx<T>* dst = spec + l0 * nh;
+  for (int q = tid; q < nl * nh; q += blockDim.x) {
+    const int l = q / nh, k = q - (q / nh) * nh;
+    const cpx<T> Z = buf[k * lpw + (l >> 1)];
+    const cpx<T> Zc = buf[((n - k) % n) * lpw + (l >> 1)];
+    cpx<T> o;
+    if ((l & 1) == 0) o = {(Z.x + Zc.x) * T(0.5), (Z.y - Zc.y) * T(0.5)};
+    else o = {(Z.y + Zc.y) * T(0.5), (Zc.x - Z.x) * T(0.5)};
+    dst[q] = o;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> herm_load(const cpx<T>* line, int k, int n, int nh) {
+  cpx<T> v = k < nh ? line[k] : cpx<T>{line[n - k].x, -line[n - k].y};
+  if (k == 0 || 2 * k == n) v.y = T(0);  // C2R ignores Im of the DC and Nyquist bins
+  return v;
+}
+
+// out = base + scale * irfft(spec) per line (base may be NULL; out may alias base).
+template <typename T>
+__global__ void z_c2r_kernel(const cpx<T>* __restrict__ spec, T* out, const T* base, T scale, int64_t nlines,
+                             int nh, Axis<T> a, int lpw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* buf = tw + a.n;
+  const int n = a.n, tid = threadIdx.x;
+  load_twiddles(tw, a);
+  const int64_t l0 = (int64_t)blockIdx.x * lpw * 2;
+  const int nl = (int)min((int64_t)2 * lpw, nlines - l0);
+  const int np = (nl + 1) / 2;
+  for (int q = tid; q < np * n; q += blockDim.x) {
+    const int pr = q / n, k = q - (q / n) * n;
+    const cpx<T>* la = spec + (l0 + 2 * pr) * nh;
+    const cpx<T> A = herm_load(la, k, n, nh);
+    const cpx<T> B = (2 * pr + 1 < nl) ? herm_load(la + nh, k, n, nh) : cpx<T>{T(0), T(0)};
+    buf[k * lpw + pr] = {A.x - B.y, A.y + B.x};
+  }
+  __syncthreads();
+  const int line = tid % lpw, tl = tid / lpw;
+  fft_lines<T, true>(buf, lpw, line, tl, a, tw, line < np && tl < a.tpl);
+  T* o = out + l0 * n;
+  const T* b = base ? base + l0 * n : nullptr;
+  for (int q = tid; q < nl * n; q += blockDim.x) {
+    const int l = q / n, e = q - (q / n) * n;
+    const T* z = reinterpret_cast<const T*>(&buf[e * lpw + (l >> 1)]);
+    const T v = scale * z[l & 1];
+    o[q] = b ? b[q] + v : v;
+  }
+}
+
+// ---------------------------------------------------------- strided sweeps --
+// Lines along y or x of the half spectrum.  Flat line index L:
+//   outer = L / nh, k = L % nh, start = (outer / G) * GM + (outer % G) * OM + k
+// y: G = 1, GM = ny*nh            (outer = c*nx + x)
+// x: G = ny, GM = nx*ny*nh, OM = nh (outer = c*ny + y)
+struct LineMap {
+  int64_t nh, G, GM, OM, es;
+};
+
+__device__ __forceinline__ int64_t line_start(const LineMap& m, int64_t L) {
+  const int64_t outer = L / m.nh, k = L - (L / m.nh) * m.nh;
+  return (outer / m.G) * m.GM + (outer % m.G) * m.OM + k;
+}
+
+// float32 IMEX multiplier of the reference: k^2 from fftfreq (float32), the
+// symbol sc*k^2 (RD) or sc*(k^2)^2 (CH), prefac = reciprocal(1 - dt*sym)*dt.
+struct Prefac {
+  int ch;          // 1: Cahn-Hilliard k^4 symbol, 0: k^2 symbol
+  float sc;        // float32 scalar of the symbol (e.g. -2*eps*D*A)
+  float dt;        // float32(dt)
+  float twopi;     // float32(2*pi)
+  float inv_nd[3]; // float32(1/(n*d)) per axis
+  int nx, ny, nz;
+  double scale;    // 1/(nx*ny*nz) of the inverse transform
+};
+
+__device__ __forceinline__ float prefac_f32(int jx, int jy, int jz, const Prefac& p) {
+#pragma clang fp contract(off)
+  const int sx = jx < (p.nx + 1) / 2 ? jx : jx - p.nx;
+  const int sy = jy < (p.ny + 1) / 2 ? jy : jy - p.ny;
+  const float kx = p.twopi * (static_cast<float>(sx) * p.inv_nd[0]);
+  const float ky = p.twopi * (static_cast<float>(sy) * p.inv_nd[1]);
+  const float kz = p.twopi * (static_cast<float>(jz) * p.inv_nd[2]);
+  const float k2 = (kx * kx + ky * ky) + kz * kz;
+  const float sym = p.ch ? p.sc * (k2 * k2) : p.sc * k2;
+  const float den = 1.0f - p.dt * sym;
+  return (1.0f / den) * p.dt;
+}
+
+// MODE 0: forward, 1: inverse, 2: forward * prefac * inverse (x axis only)
+template <typename T, int MODE>
+__global__ void strided_kernel(cpx<T>* spec, int64_t nlines, LineMap lm, Axis<T> a, int lpw, Prefac pf) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* buf = tw + a.n;
+  const int n = a.n, tid = threadIdx.x;
+  load_twiddles(tw, a);
+  const int64_t L0 = (int64_t)blockIdx.x * lpw;
+  const int nl = (int)min((int64_t)lpw, nlines - L0);
+  // blockDim.x is a multiple of lpw, so each thread always serves line tid % lpw.
+  const int line = tid % lpw, tl = tid / lpw, jstep = blockDim.x / lpw;
+  const int64_t start = line < nl ? line_start(lm, L0 + line) : 0;
+  if (line < nl)
+    for (int j = tl; j < n; j += jstep) buf[j * lpw + line] = spec[start + j * lm.es];
+  __syncthreads();
+  const bool active = line < nl && tl < a.tpl;
+  if (MODE == 1) {
+    fft_lines<T, true>(buf, lpw, line, tl, a, tw, active);
+  } else {
+    fft_lines<T, false>(buf, lpw, line, tl, a, tw, active);
+  }
+  if (MODE == 2) {
+    const T sc = static_cast<T>(pf.scale);
+    if (line < nl) {
+      const int64_t L = L0 + line;
+      const int64_t outer = L / lm.nh;
+      const int jz = (int)(L - outer * lm.nh);
+      const int jy = (int)(outer % pf.ny);
+      for (int j = tl; j < n; j += jstep) {
+        const T f = static_cast<T>(prefac_f32(j, jy, jz, pf)) * sc;
+        buf[j * lpw + line] = cscale(buf[j * lpw + line], f);
+      }
+    }
+    __syncthreads();
+    fft_lines<T, true>(buf, lpw, line, tl, a, tw, active);
+  }
+  if (line < nl)
+    for (int j = tl; j < n; j += jstep) spec[start + j * lm.es] = buf[j * lpw + line];
+}
+
+// ------------------------------------------------------------------- plan --
+std::vector<int> factorize(int n) {
+  std::vector<int> r;
+  while (n % 8 == 0) { r.push_back(8); n /= 8; }
+  while (n % 4 == 0) { r.push_back(4); n /= 4; }
+  while (n % 2 == 0) { r.push_back(2); n /= 2; }
+  for (int p = 3; p <= n; p += 2)
+    while (n % p == 0) { r.push_back(p); n /= p; }
+  return r;
+}
+
+// Threads per line such that no pass needs more than EMAX values per thread.
+int min_tpl(int n, const std::vector<int>& rad) {
+  int tpl = 1;
+  for (;;) {
+    bool ok = true;
+    for (int R : rad) {
+      const int m = n / R;
+      if (((m + tpl - 1) / tpl) * R > EMAX) { ok = false; break; }
+    }
+    if (ok) return tpl;
+    ++tpl;
+  }
+}
+
+// Largest threads-per-line <= tmax for which every pass is balanced (each
+// thread gets the same butterfly count); tpl_min if none is.
+int pick_tpl(int n, const std::vector<int>& rad, int tpl_min, int tmax) {
+  if (rad.empty()) return 1;
+  for (int t = tmax; t >= tpl_min; --t) {
+    bool ok = true;
+    for (int R : rad)
+      if ((n / R) % t) { ok = false; break; }
+    if (ok) return t;
+  }
+  return tpl_min;
+}
+
+struct AxisHost {
+  int n = 1;
+  std::vector<int> rad;
+  int tpl_min = 1;
+  void* tw = nullptr;  // device
+};
+
+}  // namespace
+}  // namespace evx
+
+struct evx_fft_plan {
+  int dtype;
+  evx::Dims d;
+  int64_t nh;
+  evx::AxisHost ax[3];  // x, y, z (z: complex length nz)
+};
+
+namespace evx {
+namespace {
+
+template <typename T>
+Axis<T> axis_dev(const AxisHost& h, int tpl) {
+  Axis<T> a;
+  a.n = h.n;
+  a.npass = (int)h.rad.size();
+  for (int i = 0; i < a.npass; ++i) a.radix[i] = h.rad[i];
+  a.tpl = tpl;
+  a.tw = static_cast<const cpx<T>*>(h.tw);
+  return a;
+}
+
+template <typename T>
+int make_twiddles(AxisHost& h) {
+  std::vector<cpx<T>> t(h.n);
+  const long double tau = 6.283185307179586476925286766559005768L;
+  for (int m = 0; m < h.n; ++m) {
+    const long double ang = tau * (long double)m / (long double)h.n;
+    t[m].x = static_cast<T>(cosl(ang));
+    t[m].y = static_cast<T>(-sinl(ang));
+  }
+  if (hipMalloc(&h.tw, sizeof(cpx<T>) * h.n) != hipSuccess) {
+    h.tw = nullptr;
+    return fail(EVX_ERR_NOMEM, "hipMalloc of %d twiddles failed", h.n);
+  }
+  if (hipMemcpy(h.tw, t.data(), sizeof(cpx<T>) * h.n, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(EVX_ERR_HIP, "twiddle upload failed");
+  return EVX_OK;
+}
+
+// Launch geometry: lines per workgroup and threads per line for an axis.
+struct Geom {
+  int lpw, tpl, threads;
+  size_t lds;
+};
+
+template <typename T>
+Geom z_geom(const AxisHost& h, int64_t npairs) {
+  Geom g;
+  g.tpl = pick_tpl(h.n, h.rad, h.tpl_min, 64);
+  // Prefer ~256 threads per workgroup within the LDS budget.
+  const size_t line_bytes = sizeof(cpx<T>) * h.n;
+  int lpw = 256 / g.tpl;
+  if (lpw < 1) lpw = 1;
+  while (lpw > 1 && line_bytes * (lpw + 1) > LDS_BUDGET) --lpw;
+  if (lpw > npairs) lpw = (int)(npairs < 1 ? 1 : npairs);
+  g.lpw = lpw;
+  g.threads = lpw * g.tpl;
+  if (g.threads > 1024) g.threads = 1024;
+  g.lds = line_bytes * (lpw + 1);
+  return g;
+}
+
+template <typename T>
+Geom strided_geom(const AxisHost& h, int64_t nlines) {
+  Geom g;
+  g.tpl = pick_tpl(h.n, h.rad, h.tpl_min, 16);
+  const size_t line_bytes = sizeof(cpx<T>) * h.n;
+  int lpw = 16;  // 16 adjacent lines: 128 B (f32) / 256 B (f64) contiguous runs
+  if (lpw * g.tpl > 1024) lpw = 1024 / g.tpl;
+  if (lpw < 1) lpw = 1;
+  while (lpw * g.tpl < 256 && line_bytes * (2 * lpw + 1) <= LDS_BUDGET && 2 * lpw * g.tpl <= 1024) lpw *= 2;
+  while (lpw > 1 && line_bytes * (lpw + 1) > LDS_BUDGET) --lpw;
+  if (lpw > nlines) lpw = (int)(nlines < 1 ? 1 : nlines);
+  g.lpw = lpw;
+  g.threads = lpw * g.tpl;
+  g.lds = line_bytes * (lpw + 1);
+  return g;
+}
+
+template <typename T>
+int run_z_r2c(evx_fft_plan* p, const void* x, void* spec, hipStream_t s) {
+  const AxisHost& h = p->ax[2];
+  const int64_t nlines = p->d.c * p->d.nx * p->d.ny;
+  const Geom g = z_geom<T>(h, (nlines + 1) / 2);
+  const int64_t blocks = (nlines + 2 * g.lpw - 1) / (2 * g.lpw);
+  hipLaunchKernelGGL(z_r2c_kernel<T>, dim3((unsigned)blocks), dim3(g.threads), g.lds, s, static_cast<const T*>(x),
+                     static_cast<cpx<T>*>(spec), nlines, (int)p->nh, axis_dev<T>(h, g.tpl), g.lpw);
+  return check_launch("z_r2c_kernel");
+}
+
+template <typename T>
+int run_z_c2r(evx_fft_plan* p, const void* spec, void* out, const void* base, double scale, hipStream_t s) {
+  const AxisHost& h = p->ax[2];
+  const int64_t nlines = p->d.c * p->d.nx * p->d.ny;
+  const Geom g = z_geom<T>(h, (nlines + 1) / 2);
+  const int64_t blocks = (nlines + 2 * g.lpw - 1) / (2 * g.lpw);
+  hipLaunchKernelGGL(z_c2r_kernel<T>, dim3((unsigned)blocks), dim3(g.threads), g.lds, s,
+                     static_cast<const cpx<T>*>(spec), static_cast<T*>(out), static_cast<const T*>(base),
+                     static_cast<T>(scale), nlines, (int)p->nh, axis_dev<T>(h, g.tpl), g.lpw);
+  return check_launch("z_c2r_kernel");
+}
+
+template <typename T, int MODE>
+int run_strided(evx_fft_plan* p, int axis, void* spec, const Prefac& pf, hipStream_t s) {
+  const AxisHost& h = p->ax[axis];
+  const Dims& d = p->d;
+  LineMap lm;
+  lm.nh = p->nh;
+  int64_t nlines;
+  if (axis == 1) {
+    nlines = d.c * d.nx * p->nh;
+    lm.G = 1;
+    lm.GM = d.ny * p->nh;
+    lm.OM = 0;
+    lm.es = p->nh;
+  } else {
+    nlines = d.c * d.ny * p->nh;
+    lm.G = d.ny;
+    lm.GM = d.nx * d.ny * p->nh;
+    lm.OM = p->nh;
+    lm.es = d.ny * p->nh;
+  }
+  if (h.n == 1 && MODE != 2) return EVX_OK;  // length-1 transform is the identity
+  const Geom g = strided_geom<T>(h, nlines);
+  const int64_t blocks = (nlines + g.lpw - 1) / g.lpw;
+  hipLaunchKernelGGL((strided_kernel<T, MODE>), dim3((unsigned)blocks), dim3(g.threads), g.lds, s,
+                     static_cast<cpx<T>*>(spec), nlines, lm, axis_dev<T>(h, g.tpl), g.lpw, pf);
+  return check_launch("strided_kernel");
+}
+
+Prefac make_prefac(const evx_fft_plan* p, int ch, double sym_scalar, double dt, const double spacing[3]) {
+  Prefac pf;
+  pf.ch = ch;
+  pf.sc = static_cast<float>(sym_scalar);
+  pf.dt = static_cast<float>(dt);
+  pf.twopi = static_cast<float>(2.0 * M_PI);
+  const int64_t n[3] = {p->d.nx, p->d.ny, p->d.nz};
+  for (int a = 0; a < 3; ++a) pf.inv_nd[a] = static_cast<float>(1.0 / ((double)n[a] * spacing[a]));
+  pf.nx = (int)p->d.nx;
+  pf.ny = (int)p->d.ny;
+  pf.nz = (int)p->d.nz;
+  pf.scale = 1.0 / ((double)p->d.nx * (double)p->d.ny * (double)p->d.nz);
+  return pf;
+}
+
+size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+size_t real_bytes(const evx_fft_plan* p) {
+  return align_up((size_t)p->d.c * p->d.vol() * (p->dtype == EVX_F32 ? 4 : 8));
+}
+size_t spec_bytes(const evx_fft_plan* p) {
+  return align_up((size_t)p->d.c * p->d.nx * p->d.ny * p->nh * (p->dtype == EVX_F32 ? 8 : 16));
+}
+
+template <typename T>
+int spectral_update(evx_fft_plan* p, const void* rhs, void* spec, const void* u, void* u_out, const Prefac& pf,
+                    hipStream_t s) {
+  EVX_TRY(run_z_r2c<T>(p, rhs, spec, s));
+  EVX_TRY((run_strided<T, 0>(p, 1, spec, pf, s)));
+  EVX_TRY((run_strided<T, 2>(p, 0, spec, pf, s)));
+  EVX_TRY((run_strided<T, 1>(p, 1, spec, pf, s)));
+  return run_z_c2r<T>(p, spec, u_out, u, 1.0, s);
+}
+
+int check_plan(const evx_fft_plan* p) {
+  if (!p) return fail(EVX_ERR_INVALID, "null plan");
+  return EVX_OK;
+}
+
+int imex_common(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm) {
+  EVX_TRY(check_plan(p));
+  if (!u || !u_out || !ws || !prm) return fail(EVX_ERR_INVALID, "null argument to imex step");
+  for (int a = 0; a < 3; ++a)
+    if (!(prm->spacing[a] > 0.0)) return fail(EVX_ERR_INVALID, "spacing must be positive");
+  return EVX_OK;
+}
+
+// div_dx = 1/float(spacing), div_dx2 = 1/float(spacing)^2 in the working
+// precision, as VoxelGrid.__init__ computes them (voxelgrid.py:29-30).
+void geometry_from_spacing(int dtype, const double sp[3], double dd[3], double dd2[3]) {
+  for (int a = 0; a < 3; ++a) {
+    if (dtype == EVX_F32) {
+      const float s = static_cast<float>(sp[a]);
+      dd[a] = 1.0f / s;
+      dd2[a] = 1.0f / (s * s);
+    } else {
+      dd[a] = 1.0 / sp[a];
+      dd2[a] = 1.0 / (sp[a] * sp[a]);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace evx
+
+using namespace evx;
+
+extern "C" int evx_fft_plan_create(evx_fft_plan** out, int32_t dtype, int64_t channels, int64_t nx, int64_t ny,
+                                   int64_t nz) {
+  if (!out) return fail(EVX_ERR_INVALID, "null plan out-pointer");
+  *out = nullptr;
+  if (dtype != EVX_F32 && dtype != EVX_F64) return fail(EVX_ERR_INVALID, "bad dtype");
+  if (channels < 1 || nx < 1 || ny < 1 || nz < 1) return fail(EVX_ERR_INVALID, "shape must be positive");
+  if (nx > (1 << 20) || ny > (1 << 20) || nz > (1 << 20)) return fail(EVX_ERR_UNSUPPORTED, "axis too long");
+  evx_fft_plan* p = new evx_fft_plan();
+  p->dtype = dtype;
+  p->d = Dims{channels, nx, ny, nz};
+  p->nh = nz / 2 + 1;
+  const int64_t n[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a) {
+    AxisHost& h = p->ax[a];
+    h.n = (int)n[a];
+    h.rad = factorize(h.n);
+    if (h.n == 1) h.rad.clear();
+    for (int R : h.rad)
+      if (R > 31) {
+        evx_fft_plan_destroy(p);
+        return fail(EVX_ERR_UNSUPPORTED, "axis length %d has a prime factor %d > 31", h.n, R);
+      }
+    if ((int)h.rad.size() > MAXPASS) {
+      evx_fft_plan_destroy(p);
+      return fail(EVX_ERR_UNSUPPORTED, "axis length %d needs too many passes", h.n);
+    }
+    h.tpl_min = min_tpl(h.n, h.rad);
+    if (h.tpl_min > 1024) {
+      evx_fft_plan_destroy(p);
+      return fail(EVX_ERR_UNSUPPORTED, "axis length %d needs more than 1024 threads per line", h.n);
+    }
+    const size_t elem = dtype == EVX_F32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+    if (elem * h.n * 2 > LDS_BUDGET) {
+      evx_fft_plan_destroy(p);
+      return fail(EVX_ERR_UNSUPPORTED, "axis length %d exceeds the LDS line budget", h.n);
+    }
+    const int st = dtype == EVX_F32 ? make_twiddles<float>(h) : make_twiddles<double>(h);
+    if (st != EVX_OK) {
+      evx_fft_plan_destroy(p);
+      return st;
+    }
+  }
+  *out = p;
+  return EVX_OK;
+}
+
+extern "C" int evx_fft_plan_destroy(evx_fft_plan* p) {
+  if (!p) return EVX_OK;
+  for (auto& h : p->ax)
+    if (h.tw) (void)hipFree(h.tw);
+  delete p;
+  return EVX_OK;
+}
+
+extern "C" int evx_fft_plan_workspace_bytes(const evx_fft_plan* p, size_t* bytes) {
+  EVX_TRY(check_plan(p));
+  if (!bytes) return fail(EVX_ERR_INVALID, "null out-pointer");
+  *bytes = real_bytes(p) + spec_bytes(p);
+  return EVX_OK;
+}
+
+extern "C" int evx_fft_plan_spectrum_elems(const evx_fft_plan* p, int64_t* elems) {
+  EVX_TRY(check_plan(p));
+  if (!elems) return fail(EVX_ERR_INVALID, "null out-pointer");
+  *elems = p->d.c * p->d.nx * p->d.ny * p->nh;
+  return EVX_OK;
+}
+
+extern "C" int evx_rfftn(evx_fft_plan* p, const void* x, void* spec, void* stream) {
+  EVX_TRY(check_plan(p));
+  if (!x || !spec) return fail(EVX_ERR_INVALID, "null buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Prefac pf{};
+  if (p->dtype == EVX_F32) {
+    EVX_TRY(run_z_r2c<float>(p, x, spec, s));
+    EVX_TRY((run_strided<float, 0>(p, 1, spec, pf, s)));
+    return run_strided<float, 0>(p, 0, spec, pf, s);
+  }
+  EVX_TRY(run_z_r2c<double>(p, x, spec, s));
+  EVX_TRY((run_strided<double, 0>(p, 1, spec, pf, s)));
+  return run_strided<double, 0>(p, 0, spec, pf, s);
+}
+
+extern "C" int evx_irfftn(evx_fft_plan* p, void* spec, void* y, void* stream) {
+  EVX_TRY(check_plan(p));
+  if (!y || !spec) return fail(EVX_ERR_INVALID, "null buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Prefac pf{};
+  const double scale = 1.0 / ((double)p->d.nx * (double)p->d.ny * (double)p->d.nz);
+  if (p->dtype == EVX_F32) {
+    EVX_TRY((run_strided<float, 1>(p, 0, spec, pf, s)));
+    EVX_TRY((run_strided<float, 1>(p, 1, spec, pf, s)));
+    return run_z_c2r<float>(p, spec, y, nullptr, scale, s);
+  }
+  EVX_TRY((run_strided<double, 1>(p, 0, spec, pf, s)));
+  EVX_TRY((run_strided<double, 1>(p, 1, spec, pf, s)));
+  return run_z_c2r<double>(p, spec, y, nullptr, scale, s);
+}
+
+extern "C" int evx_ch_imex_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
+                                void* stream) {
+  EVX_TRY(imex_common(p, u, u_out, ws, prm));
+  if (!(prm->eps != 0.0)) return fail(EVX_ERR_INVALID, "eps must be nonzero");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  double dd[3], dd2[3];
+  geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
+  void* rhs = ws;
+  void* spec = static_cast<char*>(ws) + real_bytes(p);
+  EVX_TRY(stencil_rhs_ch_periodic(p->dtype, u, rhs, p->d, dd, dd2, prm->eps, prm->D, s));
+  // problem_definition.py:303 -> ((( -2*eps)*D)*A) as a Python double
+  const Prefac pf = make_prefac(p, 1, ((-2.0 * prm->eps) * prm->D) * prm->A, prm->dt, prm->spacing);
+  return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
+                             : spectral_update<double>(p, rhs, spec, u, u_out, pf, s);
+}
+
+extern "C" int evx_rd_imex_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
+                                void* stream) {
+  EVX_TRY(imex_common(p, u, u_out, ws, prm));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  double dd[3], dd2[3];
+  geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
+  void* rhs = ws;
+  void* spec = static_cast<char*>(ws) + real_bytes(p);
+  EVX_TRY(stencil_rhs_rd_periodic(p->dtype, u, rhs, p->d, dd2, prm->D, s));
+  // problem_definition.py:206 -> (-D)*A as a Python double
+  const Prefac pf = make_prefac(p, 0, (-prm->D) * prm->A, prm->dt, prm->spacing);
+  return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
+                             : spectral_update<double>(p, rhs, spec, u, u_out, pf, s);
+}

@@ -1,0 +1,167 @@
+// Compensated float64 reductions: mass (sum of a field) and the discrete
+// Cahn-Hilliard free energy, both per channel, plus a NaN/Inf flag.
+//
+// Each thread keeps a Neumaier pair (sum, compensation); pairs are merged
+// with the error-free TwoSum inside a wavefront (64 lanes, xor shuffles),
+// across the wavefronts of a block through LDS, and finally across a fixed
+// number of per-block partials by one block per channel.  The order of all
+// merges is fixed, so results are bitwise reproducible run to run.
+#include "evx_common.hpp"
+
+#pragma clang fp contract(off)
+
+namespace evx {
+namespace {
+
+constexpr int RB = 256;     // threads per block
+constexpr int NBLK = 1024;  // partial blocks per channel
+
+struct Pair {
+  double s, c;
+};
+
+__device__ __forceinline__ void neumaier(Pair& p, double x) {
+  const double t = p.s + x;
+  if (fabs(p.s) >= fabs(x)) p.c += (p.s - t) + x;
+  else p.c += (x - t) + p.s;
+  p.s = t;
+}
+
+__device__ __forceinline__ Pair merge(Pair a, Pair b) {
+  const double s = a.s + b.s;
+  const double bb = s - a.s;
+  const double e = (a.s - (s - bb)) + (b.s - bb);
+  return Pair{s, (a.c + b.c) + e};
+}
+
+__device__ Pair block_reduce(Pair p) {
+  __shared__ double ss[RB / 64], sc[RB / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    Pair q{__shfl_xor(p.s, o, 64), __shfl_xor(p.c, o, 64)};
+    p = merge(p, q);
+  }
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) {
+    ss[w] = p.s;
+    sc[w] = p.c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Pair r{ss[0], sc[0]};
+    for (int i = 1; i < RB / 64; ++i) r = merge(r, Pair{ss[i], sc[i]});
+    p = r;
+  }
+  return p;
+}
+
+template <typename T>
+__global__ __launch_bounds__(RB) void sum_partial_kernel(const T* __restrict__ x, int64_t n, Pair* partial,
+                                                         int32_t* nonfinite) {
+  const int64_t ch = blockIdx.y;
+  const T* xc = x + ch * n;
+  Pair p{0.0, 0.0};
+  bool has_nan = false, has_inf = false;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)NBLK * RB) {
+    const double v = static_cast<double>(xc[i]);
+    has_nan |= isnan(v);
+    has_inf |= isinf(v);
+    neumaier(p, v);
+  }
+  // Every writer stores the same value, so plain stores suffice.
+  if (nonfinite && has_nan) nonfinite[0] = 1;
+  if (nonfinite && has_inf) nonfinite[1] = 1;
+  p = block_reduce(p);
+  if (threadIdx.x == 0) partial[ch * NBLK + blockIdx.x] = p;
+}
+
+__global__ __launch_bounds__(RB) void sum_final_kernel(const Pair* partial, double scale, double* out) {
+  const int64_t ch = blockIdx.x;
+  Pair p{0.0, 0.0};
+  for (int i = threadIdx.x; i < NBLK; i += RB) p = merge(p, partial[ch * NBLK + i]);
+  p = block_reduce(p);
+  if (threadIdx.x == 0) out[ch] = (p.s + p.c) * scale;
+}
+
+// Energy density eps*|grad_center c|^2 + 9/eps*c^2(1-c)^2 on pad_bc(c),
+// gradients as fd_stencils.py:44-60 evaluated in float64.
+template <typename T>
+__global__ __launch_bounds__(RB) void energy_partial_kernel(const T* __restrict__ c, Dims d, Bc bc, double idx,
+                                                            double idy, double idz, double eps, double s9,
+                                                            Pair* partial) {
+  const int64_t ch = blockIdx.y;
+  const T* cc = c + ch * d.vol();
+  const int64_t n = d.vol();
+  Pair p{0.0, 0.0};
+  auto at = [&](int64_t x, int64_t y, int64_t z) -> double {
+    Resolved<T> r = resolve<T>(x, y, z, d, bc);
+    return static_cast<double>(apply_rules(cc[r.off], r.o, bc));
+  };
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)NBLK * RB) {
+    const int64_t z = i % d.nz, t = i / d.nz;
+    const int64_t y = t % d.ny, x = t / d.ny;
+    const double gx = (0.5 * (at(x + 1, y, z) - at(x - 1, y, z))) * idx;
+    const double gy = (0.5 * (at(x, y + 1, z) - at(x, y - 1, z))) * idy;
+    const double gz = (0.5 * (at(x, y, z + 1) - at(x, y, z - 1))) * idz;
+    const double v = static_cast<double>(cc[i]);
+    const double g2 = (gx * gx + gy * gy) + gz * gz;
+    const double dens = eps * g2 + ((s9 * (v * v)) * ((1.0 - v) * (1.0 - v)));
+    neumaier(p, dens);
+  }
+  p = block_reduce(p);
+  if (threadIdx.x == 0) partial[ch * NBLK + blockIdx.x] = p;
+}
+
+}  // namespace
+}  // namespace evx
+
+using namespace evx;
+
+extern "C" int evx_reduce_workspace_bytes(int64_t channels, size_t* bytes) {
+  if (!bytes || channels < 1) return fail(EVX_ERR_INVALID, "bad arguments");
+  *bytes = (size_t)channels * NBLK * sizeof(Pair);
+  return EVX_OK;
+}
+
+extern "C" int evx_sum_f64(int32_t dtype, const void* x, int64_t channels, int64_t n, double* out,
+                           int32_t* nonfinite, void* workspace, void* stream) {
+  if (dtype != EVX_F32 && dtype != EVX_F64) return fail(EVX_ERR_INVALID, "bad dtype");
+  if (!x || !out || !workspace || channels < 1 || n < 1) return fail(EVX_ERR_INVALID, "bad arguments");
+  if (channels > 65535) return fail(EVX_ERR_UNSUPPORTED, "too many channels");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Pair* part = static_cast<Pair*>(workspace);
+  if (dtype == EVX_F32)
+    hipLaunchKernelGGL(sum_partial_kernel<float>, dim3(NBLK, (unsigned)channels), dim3(RB), 0, s,
+                       static_cast<const float*>(x), n, part, nonfinite);
+  else
+    hipLaunchKernelGGL(sum_partial_kernel<double>, dim3(NBLK, (unsigned)channels), dim3(RB), 0, s,
+                       static_cast<const double*>(x), n, part, nonfinite);
+  EVX_TRY(check_launch("sum_partial_kernel"));
+  hipLaunchKernelGGL(sum_final_kernel, dim3((unsigned)channels), dim3(RB), 0, s, part, 1.0, out);
+  return check_launch("sum_final_kernel");
+}
+
+extern "C" int evx_ch_energy(int32_t dtype, const void* c, const evx_grid* g, const evx_bc* bc, double eps,
+                             double dv, double* out, void* workspace, void* stream) {
+  Dims d;
+  Bc b;
+  if (dtype != EVX_F32 && dtype != EVX_F64) return fail(EVX_ERR_INVALID, "bad dtype");
+  if (!c || !out || !workspace || !bc) return fail(EVX_ERR_INVALID, "bad arguments");
+  EVX_TRY(make_dims(g, &d));
+  EVX_TRY(make_bc(bc, &b));
+  if (!(eps != 0.0)) return fail(EVX_ERR_INVALID, "eps must be nonzero");
+  if (d.c > 65535) return fail(EVX_ERR_UNSUPPORTED, "too many channels");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Pair* part = static_cast<Pair*>(workspace);
+  const double s9 = 9.0 / eps;
+  if (dtype == EVX_F32)
+    hipLaunchKernelGGL(energy_partial_kernel<float>, dim3(NBLK, (unsigned)d.c), dim3(RB), 0, s,
+                       static_cast<const float*>(c), d, b, g->div_dx[0], g->div_dx[1], g->div_dx[2], eps, s9, part);
+  else
+    hipLaunchKernelGGL(energy_partial_kernel<double>, dim3(NBLK, (unsigned)d.c), dim3(RB), 0, s,
+                       static_cast<const double*>(c), d, b, g->div_dx[0], g->div_dx[1], g->div_dx[2], eps, s9,
+                       part);
+  EVX_TRY(check_launch("energy_partial_kernel"));
+  hipLaunchKernelGGL(sum_final_kernel, dim3((unsigned)d.c), dim3(RB), 0, s, part, dv, out);
+  return check_launch("sum_final_kernel");
+}

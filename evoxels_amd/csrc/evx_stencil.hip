@@ -1,0 +1,643 @@
+// Finite-difference right-hand sides with in-kernel boundary conditions.
+//
+// Every kernel streams along x through a ring of (y,z) planes staged in LDS
+// (2.5-D tiling): a block owns a BY x 64 tile of (y,z) and an x-chunk, loads
+// each x-plane of the tile (plus halo) once, and keeps the planes the stencil
+// needs resident.  Ghost values never exist in HBM: out-of-grid points are
+// resolved on load (periodic wrap, Neumann copy, Dirichlet 2v-u, applied x
+// then y then z as in boundary_conditions.py:33-59).
+//
+// Floating-point contraction is off in this file so that every product and
+// sum rounds where the reference's eager torch expression rounds.
+#include "evx_common.hpp"
+
+#pragma clang fp contract(off)
+
+namespace evx {
+namespace {
+
+constexpr int BZ = 64;   // threads (and tile width) along z
+constexpr int NTY = 4;   // thread rows along y
+constexpr int NT = BZ * NTY;
+
+template <typename T>
+struct TileCfg;
+template <>
+struct TileCfg<float> { static constexpr int PY = 4; };   // BY = 16
+template <>
+struct TileCfg<double> { static constexpr int PY = 2; };  // BY = 8
+
+struct Epi {
+  int mode;
+  double coef;
+  const void* base;
+  const void* acc_in;
+  void* acc_out;
+};
+
+// Writes the fused output for right-hand side value r at flat index i.
+template <typename T>
+__device__ __forceinline__ void epilogue(const Epi& ep, const T* __restrict__ u,
+                                         T* __restrict__ out, int64_t i, T r) {
+  const T c = static_cast<T>(ep.coef);
+  switch (ep.mode) {
+    case EVX_OUT_RHS:
+      out[i] = r;
+      break;
+    case EVX_OUT_EULER:
+      out[i] = u[i] + c * r;
+      break;
+    case EVX_OUT_RK4_FIRST: {
+      const T* b = ep.base ? static_cast<const T*>(ep.base) : u;
+      out[i] = b[i] + c * r;
+      static_cast<T*>(ep.acc_out)[i] = r;
+      break;
+    }
+    case EVX_OUT_RK4_MID: {
+      const T* b = ep.base ? static_cast<const T*>(ep.base) : u;
+      const T a = static_cast<const T*>(ep.acc_in)[i];
+      out[i] = b[i] + c * r;
+      static_cast<T*>(ep.acc_out)[i] = a + T(2) * r;
+      break;
+    }
+    default: {  // EVX_OUT_RK4_LAST
+      const T* b = ep.base ? static_cast<const T*>(ep.base) : u;
+      const T a = static_cast<const T*>(ep.acc_in)[i];
+      out[i] = b[i] + c * (a + r);
+      break;
+    }
+  }
+}
+
+template <typename T>
+struct Geo {
+  T idx, idy, idz;       // div_dx
+  T idx2, idy2, idz2;    // div_dx2
+  T sum2;                // torch.sum(div_dx2)
+};
+
+template <typename T>
+__host__ Geo<T> make_geo(const evx_grid* g) {
+  Geo<T> o;
+  o.idx = static_cast<T>(g->div_dx[0]);
+  o.idy = static_cast<T>(g->div_dx[1]);
+  o.idz = static_cast<T>(g->div_dx[2]);
+  o.idx2 = static_cast<T>(g->div_dx2[0]);
+  o.idy2 = static_cast<T>(g->div_dx2[1]);
+  o.idz2 = static_cast<T>(g->div_dx2[2]);
+  o.sum2 = (o.idx2 + o.idy2) + o.idz2;
+  return o;
+}
+
+// 7-point Laplacian in the order of fd_stencils.py:70-74.
+template <typename T>
+__device__ __forceinline__ T lap7(T c, T xp, T xm, T yp, T ym, T zp, T zm, const Geo<T>& g) {
+  return (((xp + xm) * g.idx2 + (yp + ym) * g.idy2) + (zp + zm) * g.idz2) - (T(2) * c) * g.sum2;
+}
+
+// Loads the (y,z) tile [y0-H, y0+BY+H) x [z0-H, z0+BZ+H) of x-plane xg into
+// P (row pitch W), resolving out-of-grid points through the BC rules.  CLIP
+// applies torch.clip(.,0,1) before the ghost rules (pad_bc(clip(c))).
+template <typename T, int H, int ROWS, int W, bool CLIP>
+__device__ __forceinline__ void load_plane(T (*P)[W], const T* __restrict__ uc, int64_t xg,
+                                           int64_t y0, int64_t z0, const Dims& d, const Bc& bc,
+                                           bool tile_inside_yz, int tid) {
+  const bool x_in = xg >= 0 && xg < d.nx;
+  if (x_in && tile_inside_yz) {
+    const T* row0 = uc + (xg * d.ny + (y0 - H)) * d.nz + (z0 - H);
+    for (int i = tid; i < ROWS * W; i += NT) {
+      const int yy = i / W, zz = i - (i / W) * W;
+      T v = row0[(int64_t)yy * d.nz + zz];
+      P[yy][zz] = CLIP ? clip01(v) : v;
+    }
+  } else {
+    for (int i = tid; i < ROWS * W; i += NT) {
+      const int yy = i / W, zz = i - (i / W) * W;
+      Resolved<T> r = resolve<T>(xg, y0 + yy - H, z0 + zz - H, d, bc);
+      T v = uc[r.off];
+      if (CLIP) v = clip01(v);
+      P[yy][zz] = apply_rules(v, r.o, bc);
+    }
+  }
+}
+
+__device__ __forceinline__ int ring(int64_t rel, int n) { return (int)(((rel % n) + n) % n); }
+
+__host__ inline int pick_xchunk(const Dims& d, int64_t tiles_yz) {
+  // Enough blocks to fill 256 CUs a few times over, but at least 8 planes
+  // per block so the warm-up planes stay a small fraction of the traffic.
+  const int64_t target = 2048;
+  int64_t chunks = (target + tiles_yz * d.c - 1) / (tiles_yz * d.c);
+  if (chunks < 1) chunks = 1;
+  int64_t xc = (d.nx + chunks - 1) / chunks;
+  if (xc < 8) xc = 8;
+  if (xc > d.nx) xc = d.nx;
+  return (int)xc;
+}
+
+// ---------------------------------------------------------------------------
+// Reaction-diffusion (f == 0): r = D * laplace(pad_bc(u))
+// (problem_definition.py:226-229, fd_stencils.py:62-75)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NT) void rd_rhs_kernel(const T* __restrict__ u, T* __restrict__ out,
+                                                    Dims d, Bc bc, Geo<T> g, T D, Epi ep, int xc) {
+  constexpr int PY = TileCfg<T>::PY, BY = NTY * PY;
+  constexpr int W = BZ + 2, ROWS = BY + 2, NR = 4;
+  __shared__ T cs[NR][ROWS][W];
+  const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * BZ + tz;
+  const int64_t z0 = (int64_t)blockIdx.x * BZ, y0 = (int64_t)blockIdx.y * BY;
+  const int64_t nxc = (d.nx + xc - 1) / xc;
+  const int64_t ch = blockIdx.z / nxc;
+  const int64_t x0 = (blockIdx.z % nxc) * xc;
+  const int64_t x1 = min(x0 + (int64_t)xc, d.nx);
+  const T* uc = u + ch * d.vol();
+  T* oc = out + ch * d.vol();
+  const bool inside = y0 - 1 >= 0 && y0 + BY + 1 <= d.ny && z0 - 1 >= 0 && z0 + BZ + 1 <= d.nz;
+
+  load_plane<T, 1, ROWS, W, false>(cs[ring(-1, NR)], uc, x0 - 1, y0, z0, d, bc, inside, tid);
+  load_plane<T, 1, ROWS, W, false>(cs[ring(0, NR)], uc, x0, y0, z0, d, bc, inside, tid);
+  for (int64_t x = x0; x < x1; ++x) {
+    load_plane<T, 1, ROWS, W, false>(cs[ring(x + 1 - x0, NR)], uc, x + 1, y0, z0, d, bc, inside, tid);
+    __syncthreads();
+    const T(*Cm)[W] = cs[ring(x - 1 - x0, NR)];
+    const T(*C0)[W] = cs[ring(x - x0, NR)];
+    const T(*Cp)[W] = cs[ring(x + 1 - x0, NR)];
+    const int64_t gz = z0 + tz;
+#pragma unroll
+    for (int q = 0; q < PY; ++q) {
+      const int yy = ty + NTY * q;
+      const int64_t gy = y0 + yy;
+      if (gy >= d.ny || gz >= d.nz) continue;
+      const int a = yy + 1, b = tz + 1;
+      const T lap = lap7(C0[a][b], Cp[a][b], Cm[a][b], C0[a + 1][b], C0[a - 1][b], C0[a][b + 1],
+                         C0[a][b - 1], g);
+      const T r = D * lap;
+      epilogue(ep, uc, oc, (x * d.ny + gy) * d.nz + gz, r);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cahn-Hilliard (problem_definition.py:328-371):
+//   c = clip(u,0,1); mu = ((18/eps*c)*(1-c))*(1-2c) - (2eps)*lap(cB)
+//   r = D * sum_axes grad_face( (cf*(1-cf)) * grad_face(muB) )
+// mu lives in a 4-plane LDS ring with a one-voxel (y,z) halo; c in a 6-plane
+// ring with a two-voxel halo.  mu ghosts (non-periodic faces) are the BC rule
+// applied to mu at the adjacent interior voxel, as pad_bc(mu) does.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ChCoef {
+  T s18;      // 18/eps
+  T two_eps;  // 2*eps
+  T D;
+};
+
+template <typename T, int CW>
+__device__ __forceinline__ T ch_mu_at(const T (*Cm)[CW], const T (*C0)[CW], const T (*Cp)[CW],
+                                      int a, int b, const Geo<T>& g, const ChCoef<T>& k) {
+  const T c = C0[a][b];
+  const T lap = lap7(c, Cp[a][b], Cm[a][b], C0[a + 1][b], C0[a - 1][b], C0[a][b + 1], C0[a][b - 1], g);
+  return ((k.s18 * c) * (T(1) - c)) * (T(1) - T(2) * c) - k.two_eps * lap;
+}
+
+template <typename T>
+__device__ __forceinline__ T face_flux(T c_lo, T c_hi, T m_lo, T m_hi, T id) {
+  const T cf = T(0.5) * (c_hi + c_lo);
+  return (cf * (T(1) - cf)) * ((m_hi - m_lo) * id);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void ch_rhs_kernel(const T* __restrict__ u, T* __restrict__ out,
+                                                    Dims d, Bc bc, Geo<T> g, ChCoef<T> k, Epi ep,
+                                                    int xc) {
+  constexpr int PY = TileCfg<T>::PY, BY = NTY * PY;
+  constexpr int CW = BZ + 4, CR = BY + 4, NC = 6;
+  constexpr int MW = BZ + 2, MR = BY + 2, NM = 4;
+  __shared__ T cs[NC][CR][CW];
+  __shared__ T ms[NM][MR][MW];
+  const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * BZ + tz;
+  const int64_t z0 = (int64_t)blockIdx.x * BZ, y0 = (int64_t)blockIdx.y * BY;
+  const int64_t nxc = (d.nx + xc - 1) / xc;
+  const int64_t ch = blockIdx.z / nxc;
+  const int64_t x0 = (blockIdx.z % nxc) * xc;
+  const int64_t x1 = min(x0 + (int64_t)xc, d.nx);
+  const T* uc = u + ch * d.vol();
+  T* oc = out + ch * d.vol();
+  const bool inside = y0 - 2 >= 0 && y0 + BY + 2 <= d.ny && z0 - 2 >= 0 && z0 + BZ + 2 <= d.nz;
+  const bool xper = bc.kind[0] == EVX_BC_PERIODIC;
+
+  // mu of x-plane xm over the (y,z) tile plus a face halo of one voxel.
+  auto compute_mu = [&](int64_t xm) {
+    int8_t ox = 0;
+    int64_t xs = xm;
+    if (!xper && xm < 0) { ox = -1; xs = 0; }
+    if (!xper && xm >= d.nx) { ox = 1; xs = d.nx - 1; }
+    const T(*Cm)[CW] = cs[ring(xs - 1 - x0 + 2, NC)];
+    const T(*C0)[CW] = cs[ring(xs - x0 + 2, NC)];
+    const T(*Cp)[CW] = cs[ring(xs + 1 - x0 + 2, NC)];
+    T(*M)[MW] = ms[ring(xm - x0 + 1, NM)];
+    for (int i = tid; i < MR * MW; i += NT) {
+      const int yy = i / MW - 1, zz = i - (i / MW) * MW - 1;
+      const bool ey = yy < 0 || yy >= BY, ez = zz < 0 || zz >= BZ;
+      if (ey && ez) continue;  // tile edge: never read
+      int64_t gy = y0 + yy, gz = z0 + zz;
+      int8_t o[3] = {ox, 0, 0};
+      int a = yy + 2, b = zz + 2;
+      if (bc.kind[1] != EVX_BC_PERIODIC && (gy < 0 || gy >= d.ny)) {
+        o[1] = gy < 0 ? -1 : 1;
+        a += gy < 0 ? 1 : -1;
+      }
+      if (bc.kind[2] != EVX_BC_PERIODIC && (gz < 0 || gz >= d.nz)) {
+        o[2] = gz < 0 ? -1 : 1;
+        b += gz < 0 ? 1 : -1;
+      }
+      T m = ch_mu_at<T, CW>(Cm, C0, Cp, a, b, g, k);
+      if (o[0] | o[1] | o[2]) m = apply_rules(m, o, bc);
+      M[yy + 1][zz + 1] = m;
+    }
+  };
+
+  for (int64_t xg = x0 - 2; xg <= x0 + 1; ++xg)
+    load_plane<T, 2, CR, CW, true>(cs[ring(xg - x0 + 2, NC)], uc, xg, y0, z0, d, bc, inside, tid);
+  __syncthreads();
+  compute_mu(x0 - 1);
+  compute_mu(x0);
+  for (int64_t x = x0; x < x1; ++x) {
+    load_plane<T, 2, CR, CW, true>(cs[ring(x + 2 - x0 + 2, NC)], uc, x + 2, y0, z0, d, bc, inside, tid);
+    __syncthreads();
+    compute_mu(x + 1);
+    __syncthreads();
+    const T(*Cm)[CW] = cs[ring(x - 1 - x0 + 2, NC)];
+    const T(*C0)[CW] = cs[ring(x - x0 + 2, NC)];
+    const T(*Cp)[CW] = cs[ring(x + 1 - x0 + 2, NC)];
+    const T(*Mm)[MW] = ms[ring(x - 1 - x0 + 1, NM)];
+    const T(*M0)[MW] = ms[ring(x - x0 + 1, NM)];
+    const T(*Mp)[MW] = ms[ring(x + 1 - x0 + 1, NM)];
+    const int64_t gz = z0 + tz;
+#pragma unroll
+    for (int q = 0; q < PY; ++q) {
+      const int yy = ty + NTY * q;
+      const int64_t gy = y0 + yy;
+      if (gy >= d.ny || gz >= d.nz) continue;
+      const int a = yy + 2, b = tz + 2;   // c coordinates
+      const int am = yy + 1, bm = tz + 1; // mu coordinates
+      const T c0 = C0[a][b], m0 = M0[am][bm];
+      const T fxm = face_flux(Cm[a][b], c0, Mm[am][bm], m0, g.idx);
+      const T fxp = face_flux(c0, Cp[a][b], m0, Mp[am][bm], g.idx);
+      const T fym = face_flux(C0[a - 1][b], c0, M0[am - 1][bm], m0, g.idy);
+      const T fyp = face_flux(c0, C0[a + 1][b], m0, M0[am + 1][bm], g.idy);
+      const T fzm = face_flux(C0[a][b - 1], c0, M0[am][bm - 1], m0, g.idz);
+      const T fzp = face_flux(c0, C0[a][b + 1], m0, M0[am][bm + 1], g.idz);
+      const T div = ((fxp - fxm) * g.idx + (fyp - fym) * g.idy) + (fzp - fzm) * g.idz;
+      epilogue(ep, uc, oc, (x * d.ny + gy) * d.nz + gz, k.D * div);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Two-phase Allen-Cahn (problem_definition.py:440-447) with the 19-point
+// normal Laplacian of fd_stencils.py:77-103.  Edge ghosts are used here, so
+// the x-then-y-then-z composition of the ghost rules matters.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct AcCoef {
+  T s18;       // 18/eps
+  T eps;       // eps
+  T curv;      // curvature
+  T one_m_curv;// 1-curvature
+  T gab, M, force;
+  T s3;        // 3/eps
+};
+
+template <typename T, int W>
+__device__ __forceinline__ T normal_lap_at(const T (*Pm)[W], const T (*P0)[W], const T (*Pp)[W], int a,
+                                           int b, const Geo<T>& g) {
+  const T C = P0[a][b];
+  const T R = Pp[a][b], L = Pm[a][b];
+  const T Tp = P0[a + 1][b], Bm = P0[a - 1][b];
+  const T F = P0[a][b + 1], K = P0[a][b - 1];
+  const T gx = (T(0.5) * (R - L)) * g.idx;
+  const T gy = (T(0.5) * (Tp - Bm)) * g.idy;
+  const T gz = (T(0.5) * (F - K)) * g.idz;
+  const T t1 = ((gx * gx) * ((R - T(2) * C) + L)) * g.idx2;
+  const T t2 = ((gy * gy) * ((Tp - T(2) * C) + Bm)) * g.idy2;
+  const T t3 = ((gz * gz) * ((F - T(2) * C) + K)) * g.idz2;
+  const T cxy = ((Pp[a + 1][b] + Pm[a - 1][b]) - Pm[a + 1][b]) - Pp[a - 1][b];
+  const T cxz = ((Pp[a][b + 1] + Pm[a][b - 1]) - Pm[a][b + 1]) - Pp[a][b - 1];
+  const T cyz = ((P0[a + 1][b + 1] + P0[a - 1][b - 1]) - P0[a - 1][b + 1]) - P0[a + 1][b - 1];
+  const T t4 = ((((T(0.5) * gx) * gy) * cxy) * g.idx) * g.idy;
+  const T t5 = ((((T(0.5) * gx) * gz) * cxz) * g.idx) * g.idz;
+  const T t6 = ((((T(0.5) * gy) * gz) * cyz) * g.idy) * g.idz;
+  const T num = ((((t1 + t2) + t3) + t4) + t5) + t6;
+  T n2 = ((gx * gx) + (gy * gy)) + (gz * gz);
+  if (n2 <= static_cast<T>(1e-7)) n2 = T(1);
+  return num / n2;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void ac_rhs_kernel(const T* __restrict__ u, T* __restrict__ out,
+                                                    Dims d, Bc bc, Geo<T> g, AcCoef<T> k, Epi ep,
+                                                    int xc) {
+  constexpr int PY = TileCfg<T>::PY, BY = NTY * PY;
+  constexpr int W = BZ + 2, ROWS = BY + 2, NR = 4;
+  __shared__ T cs[NR][ROWS][W];
+  const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * BZ + tz;
+  const int64_t z0 = (int64_t)blockIdx.x * BZ, y0 = (int64_t)blockIdx.y * BY;
+  const int64_t nxc = (d.nx + xc - 1) / xc;
+  const int64_t ch = blockIdx.z / nxc;
+  const int64_t x0 = (blockIdx.z % nxc) * xc;
+  const int64_t x1 = min(x0 + (int64_t)xc, d.nx);
+  const T* uc = u + ch * d.vol();
+  T* oc = out + ch * d.vol();
+  const bool inside = y0 - 1 >= 0 && y0 + BY + 1 <= d.ny && z0 - 1 >= 0 && z0 + BZ + 1 <= d.nz;
+
+  load_plane<T, 1, ROWS, W, true>(cs[ring(-1, NR)], uc, x0 - 1, y0, z0, d, bc, inside, tid);
+  load_plane<T, 1, ROWS, W, true>(cs[ring(0, NR)], uc, x0, y0, z0, d, bc, inside, tid);
+  for (int64_t x = x0; x < x1; ++x) {
+    load_plane<T, 1, ROWS, W, true>(cs[ring(x + 1 - x0, NR)], uc, x + 1, y0, z0, d, bc, inside, tid);
+    __syncthreads();
+    const T(*Pm)[W] = cs[ring(x - 1 - x0, NR)];
+    const T(*P0)[W] = cs[ring(x - x0, NR)];
+    const T(*Pp)[W] = cs[ring(x + 1 - x0, NR)];
+    const int64_t gz = z0 + tz;
+#pragma unroll
+    for (int q = 0; q < PY; ++q) {
+      const int yy = ty + NTY * q;
+      const int64_t gy = y0 + yy;
+      if (gy >= d.ny || gz >= d.nz) continue;
+      const int a = yy + 1, b = tz + 1;
+      const T phi = P0[a][b];
+      const T pot = ((k.s18 * phi) * (T(1) - phi)) * (T(1) - T(2) * phi);
+      const T lap = k.curv * lap7(phi, Pp[a][b], Pm[a][b], P0[a + 1][b], P0[a - 1][b], P0[a][b + 1],
+                                  P0[a][b - 1], g);
+      const T nlap = k.one_m_curv * normal_lap_at<T, W>(Pm, P0, Pp, a, b, g);
+      const T df = k.gab * ((lap + nlap) - (pot / T(2)) / k.eps) +
+                   ((k.s3 * phi) * (T(1) - phi)) * k.force;
+      epilogue(ep, uc, oc, (x * d.ny + gy) * d.nz + gz, k.M * df);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Operator-level kernels for the generic path
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void pad_bc_kernel(const T* __restrict__ u, T* __restrict__ out, Dims d, Bc bc) {
+  const int64_t px = d.nx + 2, py = d.ny + 2, pz = d.nz + 2;
+  const int64_t n = d.c * px * py * pz;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t k = i % pz, t = i / pz;
+    int64_t j = t % py;
+    t /= py;
+    int64_t xi = t % px, ch = t / px;
+    Resolved<T> r = resolve<T>(xi - 1, j - 1, k - 1, d, bc);
+    out[i] = apply_rules(u[ch * d.vol() + r.off], r.o, bc);
+  }
+}
+
+template <typename T, bool NORMAL>
+__global__ void padded_stencil_kernel(const T* __restrict__ p, T* __restrict__ out, Dims d, Geo<T> g) {
+  const int64_t py = d.ny + 2, pz = d.nz + 2, pplane = py * pz, pvol = (d.nx + 2) * pplane;
+  const int64_t n = d.c * d.vol();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t k = i % d.nz, t = i / d.nz;
+    int64_t j = t % d.ny;
+    t /= d.ny;
+    int64_t xi = t % d.nx, ch = t / d.nx;
+    const T* q = p + ch * pvol + (xi + 1) * pplane + (j + 1) * pz + (k + 1);
+    auto at = [&](int dx, int dy, int dz) { return q[dx * pplane + dy * pz + dz]; };
+    if (!NORMAL) {
+      out[i] = lap7(at(0, 0, 0), at(1, 0, 0), at(-1, 0, 0), at(0, 1, 0), at(0, -1, 0), at(0, 0, 1),
+                    at(0, 0, -1), g);
+    } else {
+      T Pm[3][3], P0[3][3], Pp[3][3];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          Pm[a][b] = at(-1, a - 1, b - 1);
+          P0[a][b] = at(0, a - 1, b - 1);
+          Pp[a][b] = at(1, a - 1, b - 1);
+        }
+      out[i] = normal_lap_at<T, 3>(Pm, P0, Pp, 1, 1, g);
+    }
+  }
+}
+
+template <typename T>
+__global__ void axpy_kernel(const T* __restrict__ a, const T* __restrict__ b, T c, T* __restrict__ out,
+                            int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = a[i] + c * b[i];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+int check_common(int32_t dtype, const void* u, const void* out, const evx_grid* g, const evx_bc* bc,
+                 Dims* d, Bc* b) {
+  if (dtype != EVX_F32 && dtype != EVX_F64) return fail(EVX_ERR_INVALID, "dtype must be EVX_F32 or EVX_F64");
+  if (!u || !out) return fail(EVX_ERR_INVALID, "null field pointer");
+  EVX_TRY(make_dims(g, d));
+  if (bc) EVX_TRY(make_bc(bc, b));
+  return EVX_OK;
+}
+
+int check_epi(const evx_epilogue* e, const void* u, const void* out, Epi* ep) {
+  Epi x{EVX_OUT_RHS, 0.0, nullptr, nullptr, nullptr};
+  if (e) {
+    x.mode = e->mode;
+    x.coef = e->coef;
+    x.base = e->base;
+    x.acc_in = e->acc_in;
+    x.acc_out = e->acc_out;
+  }
+  if (x.mode < EVX_OUT_RHS || x.mode > EVX_OUT_RK4_LAST) return fail(EVX_ERR_INVALID, "bad epilogue mode %d", x.mode);
+  if ((x.mode == EVX_OUT_RK4_FIRST || x.mode == EVX_OUT_RK4_MID) && !x.acc_out)
+    return fail(EVX_ERR_INVALID, "RK4 stage needs acc_out");
+  if ((x.mode == EVX_OUT_RK4_MID || x.mode == EVX_OUT_RK4_LAST) && !x.acc_in)
+    return fail(EVX_ERR_INVALID, "RK4 stage needs acc_in");
+  if (u == out) return fail(EVX_ERR_INVALID, "stencil output must not alias its input");
+  *ep = x;
+  return EVX_OK;
+}
+
+template <typename T>
+dim3 stencil_grid(const Dims& d, int* xc) {
+  constexpr int BY = NTY * TileCfg<T>::PY;
+  const int64_t ty = (d.ny + BY - 1) / BY, tzn = (d.nz + BZ - 1) / BZ;
+  *xc = pick_xchunk(d, ty * tzn);
+  const int64_t nxc = (d.nx + *xc - 1) / *xc;
+  return dim3((unsigned)tzn, (unsigned)ty, (unsigned)(nxc * d.c));
+}
+
+template <typename T>
+int launch_rd(const void* u, void* out, const Dims& d, const Bc& bc, const evx_grid* g, double D,
+              const Epi& ep, hipStream_t s) {
+  int xc;
+  dim3 grid = stencil_grid<T>(d, &xc);
+  hipLaunchKernelGGL(rd_rhs_kernel<T>, grid, dim3(BZ, NTY), 0, s, static_cast<const T*>(u),
+                     static_cast<T*>(out), d, bc, make_geo<T>(g), static_cast<T>(D), ep, xc);
+  return check_launch("rd_rhs_kernel");
+}
+
+template <typename T>
+int launch_ch(const void* u, void* out, const Dims& d, const Bc& bc, const evx_grid* g, double eps,
+              double D, const Epi& ep, hipStream_t s) {
+  int xc;
+  dim3 grid = stencil_grid<T>(d, &xc);
+  ChCoef<T> k{static_cast<T>(18.0 / eps), static_cast<T>(2.0 * eps), static_cast<T>(D)};
+  hipLaunchKernelGGL(ch_rhs_kernel<T>, grid, dim3(BZ, NTY), 0, s, static_cast<const T*>(u),
+                     static_cast<T*>(out), d, bc, make_geo<T>(g), k, ep, xc);
+  return check_launch("ch_rhs_kernel");
+}
+
+template <typename T>
+int launch_ac(const void* u, void* out, const Dims& d, const Bc& bc, const evx_grid* g, double eps,
+              double gab, double M, double force, double curv, const Epi& ep, hipStream_t s) {
+  int xc;
+  dim3 grid = stencil_grid<T>(d, &xc);
+  AcCoef<T> k{static_cast<T>(18.0 / eps), static_cast<T>(eps), static_cast<T>(curv),
+              static_cast<T>(1.0 - curv), static_cast<T>(gab), static_cast<T>(M),
+              static_cast<T>(force), static_cast<T>(3.0 / eps)};
+  hipLaunchKernelGGL(ac_rhs_kernel<T>, grid, dim3(BZ, NTY), 0, s, static_cast<const T*>(u),
+                     static_cast<T*>(out), d, bc, make_geo<T>(g), k, ep, xc);
+  return check_launch("ac_rhs_kernel");
+}
+
+inline unsigned flat_blocks(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 65535LL * 16) b = 65535LL * 16;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+}  // namespace evx
+
+namespace evx {
+// Fully periodic right-hand sides written to a plain buffer; used by the
+// semi-implicit spectral steps (timesteppers.py:85-86).
+int stencil_rhs_ch_periodic(int32_t dtype, const void* u, void* out, const Dims& d, const double div_dx[3],
+                            const double div_dx2[3], double eps, double D, hipStream_t s) {
+  evx_grid g{d.c, d.nx, d.ny, d.nz, {div_dx[0], div_dx[1], div_dx[2]}, {div_dx2[0], div_dx2[1], div_dx2[2]}};
+  Bc b{};
+  const Epi ep{EVX_OUT_RHS, 0.0, nullptr, nullptr, nullptr};
+  return dtype == EVX_F32 ? launch_ch<float>(u, out, d, b, &g, eps, D, ep, s)
+                          : launch_ch<double>(u, out, d, b, &g, eps, D, ep, s);
+}
+
+int stencil_rhs_rd_periodic(int32_t dtype, const void* u, void* out, const Dims& d, const double div_dx2[3],
+                            double D, hipStream_t s) {
+  evx_grid g{d.c, d.nx, d.ny, d.nz, {0.0, 0.0, 0.0}, {div_dx2[0], div_dx2[1], div_dx2[2]}};
+  Bc b{};
+  const Epi ep{EVX_OUT_RHS, 0.0, nullptr, nullptr, nullptr};
+  return dtype == EVX_F32 ? launch_rd<float>(u, out, d, b, &g, D, ep, s)
+                          : launch_rd<double>(u, out, d, b, &g, D, ep, s);
+}
+}  // namespace evx
+
+using namespace evx;
+
+extern "C" int evx_pad_bc(int32_t dtype, const void* u, void* out, const evx_grid* g, const evx_bc* bc,
+                          void* stream) {
+  Dims d;
+  Bc b;
+  if (!bc) return fail(EVX_ERR_INVALID, "null bc");
+  EVX_TRY(check_common(dtype, u, out, g, bc, &d, &b));
+  const int64_t n = d.c * (d.nx + 2) * (d.ny + 2) * (d.nz + 2);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == EVX_F32)
+    hipLaunchKernelGGL(pad_bc_kernel<float>, dim3(flat_blocks(n)), dim3(256), 0, s,
+                       static_cast<const float*>(u), static_cast<float*>(out), d, b);
+  else
+    hipLaunchKernelGGL(pad_bc_kernel<double>, dim3(flat_blocks(n)), dim3(256), 0, s,
+                       static_cast<const double*>(u), static_cast<double*>(out), d, b);
+  return check_launch("pad_bc_kernel");
+}
+
+template <bool NORMAL>
+static int padded_stencil(int32_t dtype, const void* p, void* out, const evx_grid* g, void* stream) {
+  Dims d;
+  Bc b;
+  EVX_TRY(check_common(dtype, p, out, g, nullptr, &d, &b));
+  const int64_t n = d.c * d.vol();
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == EVX_F32)
+    hipLaunchKernelGGL((padded_stencil_kernel<float, NORMAL>), dim3(flat_blocks(n)), dim3(256), 0, s,
+                       static_cast<const float*>(p), static_cast<float*>(out), d, make_geo<float>(g));
+  else
+    hipLaunchKernelGGL((padded_stencil_kernel<double, NORMAL>), dim3(flat_blocks(n)), dim3(256), 0, s,
+                       static_cast<const double*>(p), static_cast<double*>(out), d, make_geo<double>(g));
+  return check_launch("padded_stencil_kernel");
+}
+
+extern "C" int evx_laplace_padded(int32_t dtype, const void* padded, void* out, const evx_grid* g,
+                                  void* stream) {
+  return padded_stencil<false>(dtype, padded, out, g, stream);
+}
+
+extern "C" int evx_normal_laplace_padded(int32_t dtype, const void* padded, void* out, const evx_grid* g,
+                                         void* stream) {
+  return padded_stencil<true>(dtype, padded, out, g, stream);
+}
+
+extern "C" int evx_rd_rhs(int32_t dtype, const void* u, void* out, const evx_grid* g, const evx_bc* bc,
+                          double D, const evx_epilogue* e, void* stream) {
+  Dims d;
+  Bc b;
+  Epi ep;
+  if (!bc) return fail(EVX_ERR_INVALID, "null bc");
+  EVX_TRY(check_common(dtype, u, out, g, bc, &d, &b));
+  EVX_TRY(check_epi(e, u, out, &ep));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return dtype == EVX_F32 ? launch_rd<float>(u, out, d, b, g, D, ep, s)
+                          : launch_rd<double>(u, out, d, b, g, D, ep, s);
+}
+
+extern "C" int evx_ch_rhs(int32_t dtype, const void* u, void* out, const evx_grid* g, const evx_bc* bc,
+                          double eps, double D, const evx_epilogue* e, void* stream) {
+  Dims d;
+  Bc b;
+  Epi ep;
+  if (!bc) return fail(EVX_ERR_INVALID, "null bc");
+  EVX_TRY(check_common(dtype, u, out, g, bc, &d, &b));
+  EVX_TRY(check_epi(e, u, out, &ep));
+  if (!(eps != 0.0)) return fail(EVX_ERR_INVALID, "eps must be nonzero");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return dtype == EVX_F32 ? launch_ch<float>(u, out, d, b, g, eps, D, ep, s)
+                          : launch_ch<double>(u, out, d, b, g, eps, D, ep, s);
+}
+
+extern "C" int evx_ac_rhs(int32_t dtype, const void* u, void* out, const evx_grid* g, const evx_bc* bc,
+                          double eps, double gab, double M, double force, double curvature,
+                          const evx_epilogue* e, void* stream) {
+  Dims d;
+  Bc b;
+  Epi ep;
+  if (!bc) return fail(EVX_ERR_INVALID, "null bc");
+  EVX_TRY(check_common(dtype, u, out, g, bc, &d, &b));
+  EVX_TRY(check_epi(e, u, out, &ep));
+  if (!(eps != 0.0)) return fail(EVX_ERR_INVALID, "eps must be nonzero");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return dtype == EVX_F32 ? launch_ac<float>(u, out, d, b, g, eps, gab, M, force, curvature, ep, s)
+                          : launch_ac<double>(u, out, d, b, g, eps, gab, M, force, curvature, ep, s);
+}
+
+extern "C" int evx_axpy(int32_t dtype, const void* a, const void* b, double coef, void* out, int64_t n,
+                        void* stream) {
+  if (dtype != EVX_F32 && dtype != EVX_F64) return fail(EVX_ERR_INVALID, "bad dtype");
+  if (n < 0 || (n > 0 && (!a || !b || !out))) return fail(EVX_ERR_INVALID, "bad axpy arguments");
+  if (n == 0) return EVX_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == EVX_F32)
+    hipLaunchKernelGGL(axpy_kernel<float>, dim3(flat_blocks(n)), dim3(256), 0, s,
+                       static_cast<const float*>(a), static_cast<const float*>(b), static_cast<float>(coef),
+                       static_cast<float*>(out), n);
+  else
+    hipLaunchKernelGGL(axpy_kernel<double>, dim3(flat_blocks(n)), dim3(256), 0, s,
+                       static_cast<const double*>(a), static_cast<const double*>(b), coef,
+                       static_cast<double*>(out), n);
+  return check_launch("axpy_kernel");
+}

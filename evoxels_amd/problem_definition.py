@@ -1,0 +1,347 @@
+"""Problem definitions for the 'hip' backend (mirror of evoxels.problem_definition).
+
+Constructor arguments, defaults, boundary-condition validation and warnings
+follow evoxels/problem_definition.py:13-447.  With the default closures
+(``f``, ``mu_hom``, ``potential`` left as None) ``rhs`` runs one fused HIP
+kernel that evaluates the whole right-hand side with in-kernel ghost cells;
+time steppers can additionally fuse the state update into that kernel
+(``launch``).  A user-supplied closure is evaluated as torch code on the
+device inside a generic composition of HIP operators (pad, Laplacian,
+normal Laplacian), the same composition as the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import warnings
+from abc import ABC, abstractmethod
+from dataclasses import dataclass, field
+from typing import Any, Callable
+
+import torch
+
+from . import _lib
+from .voxelgrid import VoxelGridHip
+
+try:  # analytic right-hand sides for convergence tests (problem_definition.py:223-224,322-326,408-419)
+    import sympy as sp
+    import sympy.vector as spv
+except ImportError:  # pragma: no cover
+    sp = spv = None
+
+
+class ODE(ABC):
+    """problem_definition.py:13-120."""
+
+    @property
+    @abstractmethod
+    def order(self) -> int:
+        ...
+
+    @abstractmethod
+    def rhs_analytic(self, t, u):
+        ...
+
+    @abstractmethod
+    def rhs(self, t, u):
+        ...
+
+    def pad_bc(self, u):
+        return self._pad_bc(u)
+
+    def initialize_boundary_conditions(self):
+        """BC normalisation and validation (problem_definition.py:60-115).  All
+        specialised paddings of the reference are one HIP kernel here."""
+        bc = getattr(self, "bc", None)
+        if bc is None or bc == "fully_periodic":
+            self.bc = (("periodic", None), ("periodic", None), ("periodic", None))
+            self._pad_bc = lambda f: self.vg.bc.pad_bc(f, self.bc)
+            return
+        if len(bc) != 3:
+            raise ValueError("bc must contain exactly three axis entries ordered as (x, y, z).")
+        normalized = []
+        for axis_bc in bc:
+            if isinstance(axis_bc, str):
+                if axis_bc == "dirichlet":
+                    raise ValueError("Dirichlet BCs require explicit values ('dirichlet', (left, right)).")
+                if axis_bc not in {"periodic", "neumann"}:
+                    raise ValueError(f"Unsupported BC type: {axis_bc}")
+                normalized.append((axis_bc, None))
+                continue
+            if len(axis_bc) != 2:
+                raise ValueError("Each axis boundary specification must be either a string or "
+                                 "a tuple like ('dirichlet', (left, right)).")
+            kind, values = axis_bc
+            if kind not in {"periodic", "dirichlet", "neumann"}:
+                raise ValueError(f"Unsupported BC type: {kind}")
+            if kind == "dirichlet":
+                if values is None or len(values) != 2:
+                    raise ValueError("Dirichlet BCs require two boundary values.")
+                if self.vg.convention == "cell_center":
+                    warnings.warn("Applying Dirichlet BCs on a cell_center grid "
+                                  "reduces the spatial order of convergence to 0.5!")
+                normalized.append((kind, tuple(values)))
+            else:
+                if values is not None:
+                    raise ValueError(f"{kind} BCs do not accept boundary values.")
+                normalized.append((kind, None))
+        self.bc = tuple(normalized)
+        self._pad_bc = lambda f: self.vg.bc.pad_bc(f, self.bc)
+
+    @property
+    def bc_type(self):
+        return tuple(axis_bc[0] for axis_bc in self.bc)
+
+    # ---- fused-kernel protocol used by the time steppers -----------------
+    fused = False
+
+    def launch(self, u, out, epilogue):  # pragma: no cover - overridden
+        raise NotImplementedError
+
+    def _check_state(self, u):
+        if not isinstance(u, torch.Tensor) or u.device.type != "cuda":
+            raise ValueError("state must be a torch tensor on the ROCm device")
+        if u.dim() != 4 or tuple(u.shape[1:]) != tuple(self.vg.shape):
+            raise ValueError(f"state must have shape (C, {self.vg.shape[0]}, {self.vg.shape[1]}, "
+                             f"{self.vg.shape[2]}), got {tuple(u.shape)}")
+        return u.contiguous()
+
+    def _grid(self, u):
+        return self.vg._grid_desc(u.shape)
+
+
+class SemiLinearODE(ODE):
+    """problem_definition.py:123-180."""
+
+    @property
+    @abstractmethod
+    def fourier_symbol(self):
+        ...
+
+    def verify_fft_bc_config(self):
+        x_bc, _, _ = self.bc_type
+        nonperiodic_axes = tuple(axis for axis, kind in zip(("x", "y", "z"), self.bc_type) if kind != "periodic")
+        if len(nonperiodic_axes) > 1:
+            raise ValueError("FFT-based timesteppers currently support at most one non-periodic axis, "
+                             f"got {self.bc_type}.")
+        if len(nonperiodic_axes) == 1 and nonperiodic_axes[0] != "x":
+            raise NotImplementedError(
+                "FFT-based timesteppers currently only implement the single non-periodic axis "
+                f"case for x; got {self.bc_type}. Axis permutation is not implemented yet.")
+        if x_bc == "periodic":
+            self._pad_fft_bc = self.vg.bc.pad_fft_periodic
+        elif x_bc == "dirichlet":
+            self._pad_fft_bc = self.vg.bc.pad_fft_dirichlet_periodic
+        elif x_bc == "neumann":
+            self._pad_fft_bc = self.vg.bc.pad_fft_zero_flux_periodic
+        else:
+            raise ValueError("FFT-based timesteppers only support periodic, dirichlet, or "
+                             f"neumann boundary conditions in x, got {x_bc}.")
+
+    def k_squared(self):
+        if self.bc_type[0] in {"dirichlet", "neumann"}:
+            return self.vg.rfft_k_squared_nonperiodic()
+        return self.vg.rfft_k_squared()
+
+    def pad_fft_bc(self, u):
+        return self._pad_fft_bc(u)
+
+
+def _default(fn):
+    fn._evx_default = True
+    return fn
+
+
+def _is_default(fn) -> bool:
+    return getattr(fn, "_evx_default", False)
+
+
+@dataclass
+class ReactionDiffusion(SemiLinearODE):
+    """problem_definition.py:191-229."""
+    vg: VoxelGridHip
+    D: float
+    f: Callable | None = None
+    A: float = 0.25
+    bc: tuple = ("periodic", "periodic", "periodic")
+    _fourier_symbol: Any = field(init=False, repr=False, default=None)
+
+    def __post_init__(self):
+        if self.f is None:
+            self.f = _default(lambda c=None, t=None, lib=None: 0)
+        self.initialize_boundary_conditions()
+
+    @property
+    def fused(self):
+        return _is_default(self.f)
+
+    @property
+    def order(self):
+        return 2
+
+    @property
+    def fourier_symbol(self):
+        if self._fourier_symbol is None:
+            self._fourier_symbol = -self.D * self.A * self.k_squared()
+        return self._fourier_symbol
+
+    def _eval_f(self, t, c, lib):
+        try:
+            return self.f(t, c, lib)
+        except TypeError:
+            return self.f(t, c)
+
+    def rhs_analytic(self, t, u):
+        return self.D * spv.laplacian(u) + self._eval_f(t, u, sp)
+
+    def launch(self, u, out, epilogue):
+        g = self._grid(u)
+        b = _lib.make_bc(self.bc)
+        _lib.call("evx_rd_rhs", _lib.dtype_code(u.dtype), _lib.ptr(u), _lib.ptr(out), C.byref(g), C.byref(b),
+                  float(self.D), C.byref(epilogue), _lib.stream_handle(u.device))
+
+    def rhs(self, t, u):
+        u = self._check_state(u)
+        if self.fused:
+            out = torch.empty_like(u)
+            self.launch(u, out, _lib.make_epilogue())
+            return out
+        laplace = self.vg.laplace(self.pad_bc(u))
+        return self.D * laplace + self._eval_f(t, u, self.vg.lib)
+
+
+@dataclass
+class CahnHilliard(SemiLinearODE):
+    """problem_definition.py:290-371."""
+    vg: VoxelGridHip
+    eps: float = 3.0
+    D: float = 1.0
+    mu_hom: Callable | None = None
+    A: float = 0.25
+    bc: tuple = ("periodic", "periodic", "periodic")
+    _fourier_symbol: Any = field(init=False, repr=False, default=None)
+
+    def __post_init__(self):
+        self.initialize_boundary_conditions()
+        if self.mu_hom is None:
+            self.mu_hom = _default(lambda c, lib=None: 18 / self.eps * c * (1 - c) * (1 - 2 * c))
+
+    @property
+    def fused(self):
+        return _is_default(self.mu_hom)
+
+    @property
+    def order(self):
+        return 2
+
+    @property
+    def fourier_symbol(self):
+        if self._fourier_symbol is None:
+            self._fourier_symbol = -2 * self.eps * self.D * self.A * self.k_squared() ** 2
+        return self._fourier_symbol
+
+    def _eval_mu(self, c, lib):
+        try:
+            return self.mu_hom(c, lib)
+        except TypeError:
+            return self.mu_hom(c)
+
+    def rhs_analytic(self, t, c):
+        mu = self._eval_mu(c, sp) - 2 * self.eps * spv.laplacian(c)
+        fluxes = self.D * c * (1 - c) * spv.gradient(mu)
+        return spv.divergence(fluxes)
+
+    def launch(self, u, out, epilogue):
+        g = self._grid(u)
+        b = _lib.make_bc(self.bc)
+        _lib.call("evx_ch_rhs", _lib.dtype_code(u.dtype), _lib.ptr(u), _lib.ptr(out), C.byref(g), C.byref(b),
+                  float(self.eps), float(self.D), C.byref(epilogue), _lib.stream_handle(u.device))
+
+    def rhs(self, t, c):
+        c = self._check_state(c)
+        if self.fused:
+            out = torch.empty_like(c)
+            self.launch(c, out, _lib.make_epilogue())
+            return out
+        vg = self.vg
+        c = torch.clip(c, 0, 1)
+        c_bc = self.pad_bc(c)
+        laplace = vg.laplace(c_bc)
+        mu = self._eval_mu(c, vg.lib) - 2 * self.eps * laplace
+        mu = self.pad_bc(mu)
+        div = vg.grad_x_face(vg.to_x_face(c_bc) * (1 - vg.to_x_face(c_bc)) * vg.grad_x_face(mu))[:, :, 1:-1, 1:-1]
+        div += vg.grad_y_face(vg.to_y_face(c_bc) * (1 - vg.to_y_face(c_bc)) * vg.grad_y_face(mu))[:, 1:-1, :, 1:-1]
+        div += vg.grad_z_face(vg.to_z_face(c_bc) * (1 - vg.to_z_face(c_bc)) * vg.grad_z_face(mu))[:, 1:-1, 1:-1, :]
+        return self.D * div
+
+
+@dataclass
+class TwoPhaseAllenCahn(SemiLinearODE):
+    """problem_definition.py:374-447."""
+    vg: VoxelGridHip
+    eps: float = 2.0
+    gab: float = 1.0
+    M: float = 1.0
+    force: float = 0.0
+    curvature: float = 0.01
+    potential: Callable | None = None
+    bc: tuple = ("neumann", "neumann", "neumann")
+    _fourier_symbol: Any = field(init=False, repr=False, default=None)
+
+    def __post_init__(self):
+        self.initialize_boundary_conditions()
+        if self.potential is None:
+            self.potential = _default(lambda u, lib=None: 18 / self.eps * u * (1 - u) * (1 - 2 * u))
+
+    @property
+    def fused(self):
+        return _is_default(self.potential)
+
+    @property
+    def order(self):
+        return 2
+
+    @property
+    def fourier_symbol(self):
+        if self._fourier_symbol is None:
+            self._fourier_symbol = -self.M * self.gab * self.k_squared()
+        return self._fourier_symbol
+
+    def _eval_potential(self, phi, lib):
+        try:
+            return self.potential(phi, lib)
+        except TypeError:
+            return self.potential(phi)
+
+    def rhs_analytic(self, t, phi):
+        grad = spv.gradient(phi)
+        laplace = spv.laplacian(phi)
+        norm_grad = sp.sqrt(grad.dot(grad))
+        unit_normal = grad / norm_grad
+        curv = norm_grad * spv.divergence(unit_normal)
+        n_laplace = laplace - (1 - self.curvature) * curv
+        df_dphi = self.gab * (n_laplace - self._eval_potential(phi, sp) / (2 * self.eps)) \
+            + 3 / self.eps * phi * (1 - phi) * self.force
+        return self.M * df_dphi
+
+    def launch(self, u, out, epilogue):
+        g = self._grid(u)
+        b = _lib.make_bc(self.bc)
+        _lib.call("evx_ac_rhs", _lib.dtype_code(u.dtype), _lib.ptr(u), _lib.ptr(out), C.byref(g), C.byref(b),
+                  float(self.eps), float(self.gab), float(self.M), float(self.force), float(self.curvature),
+                  C.byref(epilogue), _lib.stream_handle(u.device))
+
+    def rhs(self, t, phi):
+        phi = self._check_state(phi)
+        if self.fused:
+            out = torch.empty_like(phi)
+            self.launch(phi, out, _lib.make_epilogue())
+            return out
+        vg = self.vg
+        phi = torch.clip(phi, 0, 1)
+        potential = self._eval_potential(phi, vg.lib)
+        phi_pad = self.pad_bc(phi)
+        laplace = self.curvature * vg.laplace(phi_pad)
+        n_laplace = (1 - self.curvature) * vg.normal_laplace(phi_pad)
+        df_dphi = self.gab * (laplace + n_laplace - potential / 2 / self.eps) \
+            + 3 / self.eps * phi * (1 - phi) * self.force
+        return self.M * df_dphi

@@ -1,0 +1,163 @@
+"""Time steppers for the 'hip' backend (mirror of evoxels.timesteppers).
+
+``step(t, u) -> u`` keeps the contract of evoxels/timesteppers.py:9-29.  For a
+problem with a fused kernel (default closures) the state update is fused into
+the right-hand-side kernel: ForwardEuler is one launch, RungeKutta4 four, and
+PseudoSpectralIMEX on a fully periodic Cahn-Hilliard or reaction-diffusion
+problem is the five-sweep spectral pipeline of evx_*_imex_step.  Otherwise
+the reference's composition runs over the HIP operators.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from abc import ABC, abstractmethod
+from dataclasses import dataclass, field
+from typing import Any
+
+import torch
+
+from . import _lib
+from .problem_definition import ODE, SemiLinearODE, CahnHilliard, ReactionDiffusion
+
+State = Any
+
+
+class TimeStepper(ABC):
+    """timesteppers.py:9-29."""
+
+    @property
+    @abstractmethod
+    def order(self) -> int:
+        ...
+
+    @abstractmethod
+    def step(self, t: float, u: State) -> State:
+        ...
+
+    #: True when ``step`` issues only HIP launches whose arguments do not
+    #: depend on ``t`` (so a sequence of steps may be captured in a hipGraph).
+    capturable = False
+
+
+@dataclass
+class ForwardEuler(TimeStepper):
+    """u + dt*rhs(t, u) (timesteppers.py:32-43)."""
+    problem: ODE
+    dt: float
+
+    @property
+    def order(self) -> int:
+        return 1
+
+    @property
+    def capturable(self):
+        return self.problem.fused
+
+    def step(self, t: float, u: State) -> State:
+        if self.problem.fused:
+            u = self.problem._check_state(u)
+            out = torch.empty_like(u)
+            self.problem.launch(u, out, _lib.make_epilogue(_lib.OUT_EULER, self.dt))
+            return out
+        return u + self.dt * self.problem.rhs(t, u)
+
+
+@dataclass
+class RungeKutta4(TimeStepper):
+    """Classical RK4 (timesteppers.py:46-61); fused: four RHS launches whose
+    epilogues form the stage inputs and the running sum k1 + 2k2 + 2k3 (+ k4)."""
+    problem: ODE
+    dt: float
+
+    @property
+    def order(self) -> int:
+        return 4
+
+    @property
+    def capturable(self):
+        return self.problem.fused
+
+    def step(self, t: float, u: State) -> State:
+        if not self.problem.fused:
+            k1 = self.problem.rhs(t, u)
+            k2 = self.problem.rhs(t + 0.5 * self.dt, u + 0.5 * self.dt * k1)
+            k3 = self.problem.rhs(t + 0.5 * self.dt, u + 0.5 * self.dt * k2)
+            k4 = self.problem.rhs(t + self.dt, u + self.dt * k3)
+            return u + (self.dt / 6) * (k1 + 2 * k2 + 2 * k3 + k4)
+        p = self.problem
+        u = p._check_state(u)
+        ya, yb, acc, out = (torch.empty_like(u) for _ in range(4))
+        half = 0.5 * self.dt
+        p.launch(u, ya, _lib.make_epilogue(_lib.OUT_RK4_FIRST, half, base=u, acc_out=acc))
+        p.launch(ya, yb, _lib.make_epilogue(_lib.OUT_RK4_MID, half, base=u, acc_in=acc, acc_out=acc))
+        p.launch(yb, ya, _lib.make_epilogue(_lib.OUT_RK4_MID, self.dt, base=u, acc_in=acc, acc_out=acc))
+        p.launch(ya, out, _lib.make_epilogue(_lib.OUT_RK4_LAST, self.dt / 6, base=u, acc_in=acc))
+        return out
+
+
+@dataclass
+class PseudoSpectralIMEX(TimeStepper):
+    """First-order semi-implicit Fourier spectral step (timesteppers.py:64-89;
+    Zhu and Chen 1999).  prefac = dt/(1 - dt*sigma(k)) stays float32 like the
+    reference's k-table."""
+    problem: SemiLinearODE
+    dt: float
+    _fft_prefac: Any = field(init=False, repr=False, default=None)
+
+    def __post_init__(self):
+        self.problem.verify_fft_bc_config()
+        self.pad = self.problem.pad_fft_bc
+        p = self.problem
+        self._kind = None
+        if p.fused and p.bc_type == ("periodic", "periodic", "periodic"):
+            if isinstance(p, CahnHilliard):
+                self._kind = "ch"
+            elif isinstance(p, ReactionDiffusion):
+                self._kind = "rd"
+        if self._kind is None:
+            self._fft_prefac = self.dt / (1 - self.dt * p.fourier_symbol)
+        self._ws = None
+
+    @property
+    def order(self) -> int:
+        return 1
+
+    @property
+    def capturable(self):
+        return self._kind is not None
+
+    def _params(self):
+        prm = _lib.EvxImexParams()
+        prm.dt = float(self.dt)
+        for a in range(3):
+            prm.spacing[a] = float(self.problem.vg.spacing[a])
+        p = self.problem
+        prm.D = float(p.D)
+        prm.A = float(p.A)
+        prm.eps = float(p.eps) if self._kind == "ch" else 0.0
+        return prm
+
+    def workspace(self, u):
+        plan = self.problem.vg.plan(u.shape, u.dtype)
+        if self._ws is None or self._ws.numel() < plan.workspace_bytes or self._ws.device != u.device:
+            self._ws = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device=u.device)
+        return plan, self._ws
+
+    def step_into(self, u, out):
+        """Fused step writing u + update into ``out`` (``out`` may be ``u``)."""
+        plan, ws = self.workspace(u)
+        prm = self._params()
+        fn = "evx_ch_imex_step" if self._kind == "ch" else "evx_rd_imex_step"
+        _lib.call(fn, plan.handle, _lib.ptr(u), _lib.ptr(out), _lib.ptr(ws), C.byref(prm),
+                  _lib.stream_handle(u.device))
+        return out
+
+    def step(self, t: float, u: State) -> State:
+        if self._kind is not None:
+            u = self.problem._check_state(u)
+            return self.step_into(u, torch.empty_like(u))
+        vg = self.problem.vg
+        dc = self.pad(self.problem.rhs(t, u))
+        dc_fft = self._fft_prefac * vg.rfftn(dc, dc.shape)
+        update = vg.irfftn(dc_fft, dc.shape)[:, :u.shape[1]]
+        return u + update

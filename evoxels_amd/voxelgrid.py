@@ -1,0 +1,349 @@
+"""HIP backend adapter: the 'hip' counterpart of evoxels.voxelgrid.VoxelGridTorch.
+
+``VoxelGridHip`` keeps the method names and array shapes of the reference's
+``VoxelGrid`` / ``VoxelGridTorch`` (voxelgrid.py:19-246) so that problem
+classes written against that interface run unchanged.  Fields are torch
+tensors on a ROCm device; boundary padding, Laplacians, FFTs and reductions
+are HIP kernels of libevoxels_hip.so.  Slicing, concatenation and the face
+operators of fd_stencils.py are plain tensor views/elementwise ops exactly as
+in the reference.  There is no CPU path: a missing GPU or library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import weakref
+from dataclasses import dataclass
+from typing import Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+
+__ = slice(None)
+_i_ = slice(1, -1)
+CENTER = (__, _i_, _i_, _i_)
+LEFT = (__, slice(None, -2), _i_, _i_)
+RIGHT = (__, slice(2, None), _i_, _i_)
+BOTTOM = (__, _i_, slice(None, -2), _i_)
+TOP = (__, _i_, slice(2, None), _i_)
+BACK = (__, _i_, _i_, slice(None, -2))
+FRONT = (__, _i_, _i_, slice(2, None))
+
+PERIODIC3 = (("periodic", None),) * 3
+
+
+@dataclass
+class Grid:
+    """voxelgrid.py:9-15."""
+    shape: Tuple[int, int, int]
+    origin: Tuple[float, float, float]
+    spacing: Tuple[float, float, float]
+    convention: str
+
+
+class FDStencils:
+    """Face/centre operators of fd_stencils.py:20-60 (elementwise tensor ops);
+    laplace and normal_laplace dispatch to HIP kernels (fd_stencils.py:62-103)."""
+
+    def to_x_face(self, field):
+        return 0.5 * (field[:, 1:, :, :] + field[:, :-1, :, :])
+
+    def to_y_face(self, field):
+        return 0.5 * (field[:, :, 1:, :] + field[:, :, :-1, :])
+
+    def to_z_face(self, field):
+        return 0.5 * (field[:, :, :, 1:] + field[:, :, :, :-1])
+
+    def grad_x_face(self, field):
+        return (field[:, 1:, :, :] - field[:, :-1, :, :]) * self.div_dx[0]
+
+    def grad_y_face(self, field):
+        return (field[:, :, 1:, :] - field[:, :, :-1, :]) * self.div_dx[1]
+
+    def grad_z_face(self, field):
+        return (field[:, :, :, 1:] - field[:, :, :, :-1]) * self.div_dx[2]
+
+    def grad_x_center(self, field):
+        return 0.5 * (field[RIGHT] - field[LEFT]) * self.div_dx[0]
+
+    def grad_y_center(self, field):
+        return 0.5 * (field[TOP] - field[BOTTOM]) * self.div_dx[1]
+
+    def grad_z_center(self, field):
+        return 0.5 * (field[FRONT] - field[BACK]) * self.div_dx[2]
+
+    def gradient_norm_squared(self, field):
+        return self.grad_x_center(field) ** 2 + self.grad_y_center(field) ** 2 + \
+            self.grad_z_center(field) ** 2
+
+    def laplace(self, field):
+        """7-point Laplacian of a padded field (HIP kernel)."""
+        out = torch.empty(self._interior_shape(field), dtype=field.dtype, device=field.device)
+        g = self._grid_desc(out.shape)
+        _lib.call("evx_laplace_padded", _lib.dtype_code(field.dtype), _lib.ptr(field.contiguous()),
+                  _lib.ptr(out), C.byref(g), _lib.stream_handle(field.device))
+        return out
+
+    def normal_laplace(self, field):
+        """19-point normal Laplacian of a padded field (HIP kernel)."""
+        out = torch.empty(self._interior_shape(field), dtype=field.dtype, device=field.device)
+        g = self._grid_desc(out.shape)
+        _lib.call("evx_normal_laplace_padded", _lib.dtype_code(field.dtype), _lib.ptr(field.contiguous()),
+                  _lib.ptr(out), C.byref(g), _lib.stream_handle(field.device))
+        return out
+
+    @staticmethod
+    def _interior_shape(padded):
+        if padded.dim() != 4 or min(padded.shape[1:]) < 3:
+            raise ValueError("expected a padded (C, Nx+2, Ny+2, Nz+2) field")
+        c, a, b, d = padded.shape
+        return (c, a - 2, b - 2, d - 2)
+
+
+class CellCenteredBCs:
+    """Ghost padding of boundary_conditions.py:5-83, evaluated in one HIP kernel."""
+
+    def __init__(self, vg):
+        self.vg = vg
+
+    def pad_periodic(self, field):
+        return self.vg.pad_bc(field, PERIODIC3)
+
+    def pad_dirichlet_periodic(self, field, bc0=0, bc1=0):
+        return self.vg.pad_bc(field, (("dirichlet", (bc0, bc1)), ("periodic", None), ("periodic", None)))
+
+    def pad_zero_flux_periodic(self, field):
+        return self.vg.pad_bc(field, (("neumann", None), ("periodic", None), ("periodic", None)))
+
+    def pad_bc(self, field, bc):
+        return self.vg.pad_bc(field, bc)
+
+    def pad_fft_periodic(self, field):
+        return field
+
+    def pad_fft_dirichlet_periodic(self, field):
+        return torch.cat((field, -torch.flip(field, [1])), 1)
+
+    def pad_fft_zero_flux_periodic(self, field):
+        return torch.cat((field, torch.flip(field, [1])), 1)
+
+    def trim_boundary_nodes(self, field):
+        return field
+
+    def trim_ghost_nodes(self, field):
+        if field[0, _i_, _i_, _i_].shape == self.vg.shape:
+            return field[:, _i_, _i_, _i_]
+        raise ValueError(f"The provided field has the wrong shape {self.vg.shape}.")
+
+
+class _FFTPlan:
+    """Owns one evx_fft_plan (twiddle tables only)."""
+
+    def __init__(self, dtype_code, c, nx, ny, nz):
+        lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(lib.evx_fft_plan_create(C.byref(h), dtype_code, c, nx, ny, nz), "evx_fft_plan_create")
+        self.handle = h
+        self.shape = (c, nx, ny, nz)
+        ws = C.c_size_t()
+        _lib.check(lib.evx_fft_plan_workspace_bytes(h, C.byref(ws)), "evx_fft_plan_workspace_bytes")
+        self.workspace_bytes = ws.value
+        self._fin = weakref.finalize(self, lib.evx_fft_plan_destroy, h)
+
+
+def _hip_device(device) -> torch.device:
+    dev = torch.device(device)
+    if dev.type != "cuda":
+        raise RuntimeError(f"the 'hip' backend runs on a ROCm GPU; got device {device!r}")
+    if not torch.cuda.is_available():
+        raise RuntimeError("the 'hip' backend needs a ROCm GPU, and none is visible to torch")
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    return dev
+
+
+class VoxelGridHip(FDStencils):
+    """Grid adapter on a ROCm device (voxelgrid.py:19-152,155-246)."""
+
+    def __init__(self, grid: Grid, precision="float32", device="cuda"):
+        _lib.load()
+        self.torch = torch
+        self.lib = torch
+        self.device = _hip_device(device)
+        if precision == "float32":
+            self.precision = torch.float32
+        elif precision == "float64":
+            self.precision = torch.float64
+        else:
+            raise ValueError(f"Unsupported precision: {precision}")
+        self.shape = tuple(int(n) for n in grid.shape)
+        self.origin = grid.origin
+        self.spacing = grid.spacing
+        self.convention = grid.convention
+        if self.convention != "cell_center":
+            raise NotImplementedError("the 'hip' backend implements the cell_center convention")
+        # voxelgrid.py:29-30: spacing cast to the working precision, then 1/x and 1/x**2.
+        sp = torch.tensor(np.array(self.spacing), dtype=self.precision)
+        dd, dd2 = 1 / sp, 1 / sp ** 2
+        self.div_dx_host = tuple(float(v) for v in dd)
+        self.div_dx2_host = tuple(float(v) for v in dd2)
+        self.div_dx = dd.to(self.device)
+        self.div_dx2 = dd2.to(self.device)
+        self.bc = CellCenteredBCs(self)
+        self._plans = {}
+        self._reduce_ws = None
+
+    # ---------------------------------------------------------- descriptors
+    def _grid_desc(self, shape4):
+        return _lib.make_grid(shape4, self.div_dx_host, self.div_dx2_host)
+
+    def plan(self, shape4, dtype=None):
+        dtype = dtype or self.precision
+        key = (dtype, tuple(int(v) for v in shape4))
+        if key not in self._plans:
+            self._plans[key] = _FFTPlan(_lib.dtype_code(dtype), *key[1])
+        return self._plans[key]
+
+    def reduce_workspace(self, channels):
+        lib = _lib.load()
+        n = C.c_size_t()
+        _lib.check(lib.evx_reduce_workspace_bytes(channels, C.byref(n)), "evx_reduce_workspace_bytes")
+        if self._reduce_ws is None or self._reduce_ws.numel() < n.value:
+            self._reduce_ws = torch.empty(n.value, dtype=torch.uint8, device=self.device)
+        return self._reduce_ws
+
+    # -------------------------------------------------------------- arrays
+    def to_backend(self, np_arr):
+        return torch.tensor(np_arr, dtype=self.precision, device=self.device)
+
+    def to_numpy(self, field):
+        return field.cpu().numpy()
+
+    def pad_bc(self, field, bc):
+        field = field.contiguous()
+        c, nx, ny, nz = field.shape
+        out = torch.empty((c, nx + 2, ny + 2, nz + 2), dtype=field.dtype, device=field.device)
+        g = self._grid_desc(field.shape)
+        b = _lib.make_bc(bc)
+        _lib.call("evx_pad_bc", _lib.dtype_code(field.dtype), _lib.ptr(field), _lib.ptr(out), C.byref(g),
+                  C.byref(b), _lib.stream_handle(field.device))
+        return out
+
+    def pad_periodic(self, field):
+        return self.pad_bc(field, PERIODIC3)
+
+    def pad_zeros(self, field):
+        return torch.nn.functional.pad(field, (1, 1, 1, 1, 1, 1), mode="constant", value=0)
+
+    def rfftn(self, field, shape):
+        """Per-channel real 3-D FFT (voxelgrid.py:203-204)."""
+        field = field.contiguous()
+        if tuple(shape) != tuple(field.shape):
+            raise NotImplementedError("rfftn with a shape different from the field's")
+        p = self.plan(field.shape, field.dtype)
+        c, nx, ny, nz = field.shape
+        cdt = torch.complex64 if field.dtype == torch.float32 else torch.complex128
+        spec = torch.empty((c, nx, ny, nz // 2 + 1), dtype=cdt, device=field.device)
+        _lib.call("evx_rfftn", p.handle, _lib.ptr(field), _lib.ptr(spec), _lib.stream_handle(field.device))
+        return spec
+
+    def irfftn(self, spec, shape):
+        """Per-channel inverse real 3-D FFT (voxelgrid.py:206-207)."""
+        c, nx, ny, nz = (int(v) for v in shape)
+        if spec.shape != (c, nx, ny, nz // 2 + 1):
+            raise NotImplementedError("irfftn with a spectrum shape that does not match `shape`")
+        rdt = torch.float32 if spec.dtype == torch.complex64 else torch.float64
+        work = spec.contiguous().clone()
+        p = self.plan((c, nx, ny, nz), rdt)
+        out = torch.empty((c, nx, ny, nz), dtype=rdt, device=spec.device)
+        _lib.call("evx_irfftn", p.handle, _lib.ptr(work), _lib.ptr(out), _lib.stream_handle(spec.device))
+        return out
+
+    def expand_dim(self, field, dim):
+        return field.unsqueeze(dim)
+
+    def squeeze(self, field, dim):
+        return torch.squeeze(field, dim)
+
+    def concatenate(self, fieldlist, dim):
+        return torch.cat(fieldlist, dim=dim)
+
+    def set(self, field, index, value):
+        field[index] = value
+        return field
+
+    def mean(self, field, dim=None, keepdim=False):
+        return torch.mean(field, dim=dim, keepdim=keepdim)
+
+    def sum(self, field, dim=None, keepdim=False):
+        return torch.sum(field, dim=dim, keepdim=keepdim)
+
+    def arange(self, start, stop):
+        return torch.arange(start, stop, dtype=self.precision, device=self.device)
+
+    # ---------------------------------------------------------- k-space
+    def axes(self):
+        return tuple(torch.arange(0, n, dtype=self.precision, device=self.device) * self.spacing[i]
+                     + self.origin[i] for i, n in enumerate(self.shape))
+
+    def fft_axes(self):
+        return tuple(2 * np.pi * torch.fft.fftfreq(n, d, device=self.device)
+                     for n, d in zip(self.shape, self.spacing))
+
+    def rfft_axes(self):
+        return tuple(2 * np.pi * torch.fft.rfftfreq(n, d, device=self.device)
+                     for n, d in zip(self.shape, self.spacing))
+
+    def rfft_k_squared(self):
+        """float32 table, as the reference builds it (voxelgrid.py:110-114)."""
+        a_x, a_y, _ = self.fft_axes()
+        _, _, a_z = self.rfft_axes()
+        kx, ky, kz = torch.meshgrid(a_x, a_y, a_z, indexing="ij")
+        return kx ** 2 + ky ** 2 + kz ** 2
+
+    def rfft_k_squared_nonperiodic(self):
+        """voxelgrid.py:116-124 (cell_center mirror extension of x)."""
+        a_x = 2 * np.pi * torch.fft.fftfreq(2 * self.shape[0], d=self.spacing[0], device=self.device)
+        _, a_y, _ = self.fft_axes()
+        _, _, a_z = self.rfft_axes()
+        kx, ky, kz = torch.meshgrid(a_x, a_y, a_z, indexing="ij")
+        return kx ** 2 + ky ** 2 + kz ** 2
+
+    # ---------------------------------------------------------- fields
+    def init_scalar_field(self, array):
+        return self.expand_dim(self.to_backend(array), 0)
+
+    def export_scalar_field_to_numpy(self, field):
+        return self.to_numpy(self.squeeze(field, 0))
+
+    def channel_sums(self, field, nonfinite=None):
+        """Compensated float64 sum per channel (HIP); optional NaN/Inf flags."""
+        field = field.contiguous()
+        c = field.shape[0]
+        out = torch.empty(c, dtype=torch.float64, device=field.device)
+        ws = self.reduce_workspace(c)
+        _lib.call("evx_sum_f64", _lib.dtype_code(field.dtype), _lib.ptr(field), c, field[0].numel(),
+                  _lib.ptr(out), _lib.ptr(nonfinite), _lib.ptr(ws), _lib.stream_handle(field.device))
+        return out
+
+    def average(self, field):
+        """Spatial mean per channel (voxelgrid.py:137-152); compensated float64
+        accumulation, returned in the working precision."""
+        if tuple(field.shape[1:]) != self.shape:
+            raise ValueError(f"The provided field must have the shape {self.shape}.")
+        s = self.channel_sums(field)
+        return (s / float(np.prod(self.shape))).to(field.dtype)
+
+    def ch_energy(self, field, eps, bc=PERIODIC3):
+        """Discrete CH free energy per channel (float64)."""
+        field = field.contiguous()
+        c = field.shape[0]
+        out = torch.empty(c, dtype=torch.float64, device=field.device)
+        ws = self.reduce_workspace(c)
+        g = self._grid_desc(field.shape)
+        b = _lib.make_bc(bc)
+        dv = float(np.prod(np.asarray(self.spacing, dtype=np.float64)))
+        _lib.call("evx_ch_energy", _lib.dtype_code(field.dtype), _lib.ptr(field), C.byref(g), C.byref(b),
+                  float(eps), dv, _lib.ptr(out), _lib.ptr(ws), _lib.stream_handle(field.device))
+        return out

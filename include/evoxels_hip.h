@@ -1,0 +1,205 @@
+/*
+ * evoxels_hip.h - C ABI of libevoxels_hip.so, the MI355X (gfx950) kernels
+ * behind evoxels' time-stepping hot path.
+ *
+ * The reference (evoxels 1.0.0) is pure Python on torch; it has no FFI.  Each
+ * entry point below replaces one reference operator or one fused composition
+ * of them, cited as file:line of the reference package.
+ *
+ * Conventions
+ *   - All pointers are device pointers into caller-owned memory, except the
+ *     evx_grid / evx_bc / evx_epilogue / evx_imex_params descriptors and the
+ *     out-parameters of plan functions, which are host pointers.
+ *   - Fields are C-contiguous (channels, nx, ny, nz), z fastest; padded
+ *     fields are (channels, nx+2, ny+2, nz+2).
+ *   - dtype is EVX_F32 or EVX_F64 and applies to every real buffer of a call.
+ *   - stream is a hipStream_t (NULL = legacy default stream).  Every call is
+ *     asynchronous on that stream, performs no device allocation and no host
+ *     synchronisation, and is therefore legal under hipGraph capture.
+ *   - Return value: EVX_OK or a negative status; evx_last_error() gives the
+ *     message of the last failing call on the calling thread.
+ */
+#ifndef EVOXELS_HIP_H
+#define EVOXELS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EVX_ABI_VERSION 1
+
+enum evx_status {
+  EVX_OK = 0,
+  EVX_ERR_INVALID = -1,     /* bad argument, shape or descriptor            */
+  EVX_ERR_HIP = -2,         /* a HIP runtime call or kernel launch failed   */
+  EVX_ERR_UNSUPPORTED = -3, /* valid request this build does not implement  */
+  EVX_ERR_NOMEM = -4        /* plan-time allocation failed                  */
+};
+
+enum evx_dtype { EVX_F32 = 0, EVX_F64 = 1 };
+
+/* Boundary kinds of one axis (problem_definition.py:60-103). */
+enum evx_bc_kind { EVX_BC_PERIODIC = 0, EVX_BC_NEUMANN = 1, EVX_BC_DIRICHLET = 2 };
+
+/* Per-axis boundary description for the cell_center convention.
+ * lo/hi are the Dirichlet values (ignored otherwise).  Ghosts follow
+ * CellCenteredBCs.pad_bc (boundary_conditions.py:33-59): periodic wrap, then
+ * x, y, z rules in that order (Neumann u0 := u1, Dirichlet u0 := 2v - u1). */
+typedef struct evx_bc {
+  int32_t kind[3];
+  double lo[3];
+  double hi[3];
+} evx_bc;
+
+/* Grid geometry.  div_dx / div_dx2 must already be rounded to the working
+ * precision exactly as VoxelGrid.__init__ does (voxelgrid.py:29-30):
+ * 1/float(spacing) and 1/float(spacing)**2 in that precision. */
+typedef struct evx_grid {
+  int64_t channels, nx, ny, nz;
+  double div_dx[3];
+  double div_dx2[3];
+} evx_grid;
+
+/* Fused output of a right-hand-side kernel (r = rhs(u) at each voxel):
+ *   EVX_OUT_RHS        out = r
+ *   EVX_OUT_EULER      out = u + coef*r                 (timesteppers.py:42-43)
+ *   EVX_OUT_RK4_FIRST  out = base + coef*r, acc_out = r  (timesteppers.py:57-58)
+ *   EVX_OUT_RK4_MID    out = base + coef*r, acc_out = acc_in + 2*r
+ *   EVX_OUT_RK4_LAST   out = base + coef*(acc_in + r)    (timesteppers.py:61)
+ * coef is the Python-double scalar of the reference expression (dt, 0.5*dt,
+ * dt/6); it is rounded to the working precision in-kernel.  base == NULL
+ * means u.  out must not alias u (stencil reads neighbours). */
+enum evx_out_mode {
+  EVX_OUT_RHS = 0, EVX_OUT_EULER = 1, EVX_OUT_RK4_FIRST = 2,
+  EVX_OUT_RK4_MID = 3, EVX_OUT_RK4_LAST = 4
+};
+typedef struct evx_epilogue {
+  int32_t mode;
+  double coef;
+  const void* base;
+  const void* acc_in;
+  void* acc_out;
+} evx_epilogue;
+
+/* ---- library ------------------------------------------------------------ */
+int evx_version(void);
+const char* evx_last_error(void);
+
+/* ---- boundary padding and stencils (operator level) --------------------- */
+
+/* out(C,nx+2,ny+2,nz+2) = pad_bc(u).  Replaces CellCenteredBCs.pad_bc /
+ * pad_periodic / pad_dirichlet_periodic / pad_zero_flux_periodic
+ * (boundary_conditions.py:9-59; voxelgrid.py:194-195). */
+int evx_pad_bc(int32_t dtype, const void* u, void* out, const evx_grid* g,
+               const evx_bc* bc, void* stream);
+
+/* out(C,nx,ny,nz) = laplace(padded) for an already padded input.
+ * Replaces FDStencils.laplace (fd_stencils.py:62-75). */
+int evx_laplace_padded(int32_t dtype, const void* padded, void* out,
+                       const evx_grid* g, void* stream);
+
+/* out(C,nx,ny,nz) = normal_laplace(padded).  Replaces
+ * FDStencils.normal_laplace (fd_stencils.py:77-103). */
+int evx_normal_laplace_padded(int32_t dtype, const void* padded, void* out,
+                              const evx_grid* g, void* stream);
+
+/* ---- fused right-hand sides --------------------------------------------- */
+
+/* ReactionDiffusion with f == 0: r = D*laplace(pad_bc(u)).
+ * Replaces ReactionDiffusion.rhs (problem_definition.py:226-229) composed
+ * with ForwardEuler / RungeKutta4 through the epilogue. */
+int evx_rd_rhs(int32_t dtype, const void* u, void* out, const evx_grid* g,
+               const evx_bc* bc, double D, const evx_epilogue* ep, void* stream);
+
+/* CahnHilliard with the default mu_hom:
+ * r = D*div(cf(1-cf) grad mu), mu = 18/eps c(1-c)(1-2c) - 2 eps lap c,
+ * c = clip(u,0,1).  Replaces CahnHilliard.rhs (problem_definition.py:328-371). */
+int evx_ch_rhs(int32_t dtype, const void* u, void* out, const evx_grid* g,
+               const evx_bc* bc, double eps, double D, const evx_epilogue* ep,
+               void* stream);
+
+/* TwoPhaseAllenCahn with the default potential.  Replaces
+ * TwoPhaseAllenCahn.rhs (problem_definition.py:440-447). */
+int evx_ac_rhs(int32_t dtype, const void* u, void* out, const evx_grid* g,
+               const evx_bc* bc, double eps, double gab, double M, double force,
+               double curvature, const evx_epilogue* ep, void* stream);
+
+/* Elementwise out = a + coef*b (coef rounded to the working precision).
+ * Used for generic-path Euler / RK4 stage updates (timesteppers.py:42-61). */
+int evx_axpy(int32_t dtype, const void* a, const void* b, double coef,
+             void* out, int64_t n, void* stream);
+
+/* ---- spectral (real 3-D FFT) ---------------------------------------------- */
+typedef struct evx_fft_plan evx_fft_plan;
+
+/* Plan for per-channel real 3-D transforms of a (channels,nx,ny,nz) field,
+ * norm "backward" (unscaled forward, 1/N inverse).  Replaces
+ * VoxelGridTorch.rfftn / irfftn (voxelgrid.py:203-207) for C == 1 and their
+ * per-channel equivalent for C > 1.  Owns only twiddle tables. */
+int evx_fft_plan_create(evx_fft_plan** plan, int32_t dtype, int64_t channels,
+                        int64_t nx, int64_t ny, int64_t nz);
+int evx_fft_plan_destroy(evx_fft_plan* plan);
+/* Bytes of caller-provided device workspace needed by evx_ch_imex_step and
+ * evx_rd_imex_step (spectrum plus real staging). */
+int evx_fft_plan_workspace_bytes(const evx_fft_plan* plan, size_t* bytes);
+/* Number of complex values of one spectrum (channels*nx*ny*(nz/2+1)). */
+int evx_fft_plan_spectrum_elems(const evx_fft_plan* plan, int64_t* elems);
+
+/* spec(C,nx,ny,nz/2+1) complex = rfftn(x) (voxelgrid.py:203-204). */
+int evx_rfftn(evx_fft_plan* plan, const void* x, void* spec, void* stream);
+/* y(C,nx,ny,nz) = irfftn(spec) (voxelgrid.py:206-207); spec is overwritten.
+ * Imaginary parts of the kz=0 (and kz=nz/2) bins are ignored as C2R does. */
+int evx_irfftn(evx_fft_plan* plan, void* spec, void* y, void* stream);
+
+/* Semi-implicit step parameters.  The k-space symbol follows the reference
+ * quirk: k^2 is float32 (voxelgrid.py:84-114), sigma and
+ * prefac = dt/(1-dt*sigma) are float32 (timesteppers.py:77). */
+typedef struct evx_imex_params {
+  double dt;
+  double spacing[3];  /* grid spacing (fftfreq's d) */
+  double eps;         /* CahnHilliard eps  (problem_definition.py:293)    */
+  double D;           /* diffusivity / mobility prefactor                 */
+  double A;           /* stabilisation constant (problem_definition.py:296) */
+} evx_imex_params;
+
+/* u_out = u + irfftn(prefac * rfftn(ch_rhs(u)))  for a fully periodic
+ * CahnHilliard + PseudoSpectralIMEX step (timesteppers.py:85-89,
+ * problem_definition.py:303,328-371).  u_out may alias u. */
+int evx_ch_imex_step(evx_fft_plan* plan, const void* u, void* u_out,
+                     void* workspace, const evx_imex_params* p, void* stream);
+
+/* Same for ReactionDiffusion (f == 0; symbol -D*A*k^2, problem_definition.py:206). */
+int evx_rd_imex_step(evx_fft_plan* plan, const void* u, void* u_out,
+                     void* workspace, const evx_imex_params* p, void* stream);
+
+/* ---- reductions ----------------------------------------------------------- */
+
+/* Bytes of device workspace evx_sum_f64 / evx_ch_energy need for `channels`. */
+int evx_reduce_workspace_bytes(int64_t channels, size_t* bytes);
+
+/* out[c] = sum over channel c of x, accumulated in compensated float64
+ * (the numerator of VoxelGrid.average, voxelgrid.py:137-141).  out is a
+ * device double[channels]; nonfinite, if not NULL, is a device int32[2]:
+ * [0] is set to 1 when any element is NaN (the per-frame check of
+ * solvers.py:163-165), [1] when any is +-Inf; both are left untouched
+ * otherwise, so the caller zeroes them.  Deterministic: fixed reduction order. */
+int evx_sum_f64(int32_t dtype, const void* x, int64_t channels,
+                int64_t elems_per_channel, double* out, int32_t* nonfinite,
+                void* workspace, void* stream);
+
+/* Discrete CH free energy per channel in float64:
+ *   sum_voxels (eps*|grad_center c|^2 + 9/eps*c^2(1-c)^2) * dv
+ * with centred gradients (fd_stencils.py:44-60) on pad_bc(c).  The reference
+ * defines the functional (docs/notebooks/01-using-solvers.ipynb) but no
+ * function for it; this entry point has no reference counterpart. */
+int evx_ch_energy(int32_t dtype, const void* c, const evx_grid* g,
+                  const evx_bc* bc, double eps, double dv, double* out,
+                  void* workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EVOXELS_HIP_H */

@@ -1,0 +1,165 @@
+"""Parity of the HIP real 3-D FFT and of the fused semi-implicit steps with the
+CPU oracle (torch.fft / pocketfft in float64), through the C ABI."""
+import numpy as np
+import pytest
+import torch
+
+import evoxels_amd as evx
+from evoxels_amd.problem_definition import CahnHilliard, ReactionDiffusion
+from evoxels_amd.timesteppers import PseudoSpectralIMEX
+from evoxels_amd.voxelgrid import VoxelGridHip
+from oracle import evoxels_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+P3 = (("periodic", None),) * 3
+FFT_SHAPES = [(4, 4, 4), (1, 1, 8), (2, 3, 1), (1, 7, 1), (5, 6, 7), (8, 12, 10), (16, 9, 25), (3, 11, 13),
+              (7, 29, 31), (32, 48, 64), (100, 20, 100), (96, 1, 128), (64, 256, 30), (6, 1024, 3)]
+
+
+def grid_for(shape, precision, spacing=(1.0, 1.0, 1.0)):
+    vf = evx.VoxelFields(shape, tuple(s * n for s, n in zip(spacing, shape)))
+    vf.precision = precision
+    return vf, VoxelGridHip(vf.grid_info(), precision=precision, device="cuda")
+
+
+def rel_max(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+@pytest.mark.parametrize("shape", FFT_SHAPES)
+def test_rfftn_irfftn_against_pocketfft(shape, precision):
+    _, vg = grid_for(shape, precision)
+    rng = np.random.default_rng(sum(shape))
+    x = rng.standard_normal((2,) + shape)
+    xt = torch.tensor(x, dtype=vg.precision, device="cuda")
+    spec = vg.rfftn(xt, xt.shape)
+    want = np.stack([np.fft.rfftn(x[c]) for c in range(2)])
+    tol = 2e-6 if precision == "float32" else 1e-13
+    assert rel_max(spec.cpu().numpy(), want) < tol
+    # inverse of a spectrum whose DC / Nyquist bins carry imaginary garbage
+    s2 = want.copy()
+    s2[..., 0] += 0.3j
+    if shape[2] % 2 == 0:
+        s2[..., -1] -= 0.2j
+    back = vg.irfftn(torch.tensor(s2, device="cuda").to(spec.dtype), xt.shape)
+    want_b = np.stack([np.fft.irfftn(s2[c], shape) for c in range(2)])
+    assert rel_max(back.cpu().numpy(), want_b) < tol
+    rt = vg.irfftn(spec, xt.shape)
+    assert rel_max(rt.cpu().numpy(), x) < tol
+
+
+def imex_pair(kind, shape, precision, spacing, seed, dt, amp=0.05, **kw):
+    vf, vg = grid_for(shape, precision, spacing)
+    rng = np.random.default_rng(seed)
+    arr = 0.5 + amp * rng.standard_normal(shape)
+    if kind == "ch":
+        prob = CahnHilliard(vg, **kw)
+    else:
+        prob = ReactionDiffusion(vg, **kw)
+    step = PseudoSpectralIMEX(prob, dt)
+    assert step.capturable
+    return vf, vg, prob, step, arr
+
+
+def oracle_imex(kind, arr, spacing_abs, precision, dt, kw):
+    og = O.OracleGrid(arr.shape, spacing_abs, precision)
+    k2 = O.rfft_k_squared(arr.shape, spacing_abs)
+    if kind == "ch":
+        pre = O.imex_prefac(dt, O.ch_symbol(k2, kw["eps"], kw["D"]))
+        rhs = lambda t, v: O.ch_rhs(v, og, P3, kw["eps"], kw["D"])  # noqa: E731
+    else:
+        pre = O.imex_prefac(dt, O.rd_symbol(k2, kw["D"]))
+        rhs = lambda t, v: O.rd_rhs(v, og, P3, kw["D"])  # noqa: E731
+    return og, pre, rhs
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+@pytest.mark.parametrize("shape,spacing", [((16, 16, 16), (1.0, 1.0, 1.0)), ((12, 10, 14), (0.8, 1.2, 1.0)),
+                                           ((20, 9, 33), (1.0, 1.0, 1.0)), ((48, 32, 96), (1.0, 1.0, 1.0))])
+@pytest.mark.parametrize("kind", ["ch", "rd"])
+def test_imex_step_matches_oracle(kind, shape, spacing, precision):
+    kw = {"eps": 3.0, "D": 1.0} if kind == "ch" else {"D": 0.8}
+    dt = 0.1
+    vf, vg, prob, step, arr = imex_pair(kind, shape, precision, spacing, 3, dt, **kw)
+    u = vg.init_scalar_field(arr)
+    got = step.step(0.0, u).cpu().numpy()[0]
+    og64, pre64, rhs64 = oracle_imex(kind, arr, vf.spacing, "float64", dt, kw)
+    ref = O.imex_step(og64.field(arr), rhs64, pre64)[0].numpy()
+    upd_ref = ref - arr
+    upd = got.astype(np.float64) - np.asarray(u.cpu()[0], np.float64)
+    field_err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    upd_err = np.linalg.norm(upd - upd_ref) / np.linalg.norm(upd_ref)
+    print(f"{kind} {shape} {precision}: field {field_err:.3e} update {upd_err:.3e}")
+    if precision == "float64":
+        assert field_err < 1e-6 and upd_err < 1e-10
+    else:
+        assert field_err < 1e-5 and upd_err < 5e-4
+
+
+def test_ch_imex_100cubed_float64_config():
+    """100^3, domain 100, c0 = 0.5 + 0.1*U[0,1) seed 0, dt = 0.1, float64: 3 steps."""
+    n = 100
+    vf = evx.VoxelFields((n, n, n), (n, n, n))
+    vf.precision = "float64"
+    vg = VoxelGridHip(vf.grid_info(), precision="float64", device="cuda")
+    step = PseudoSpectralIMEX(CahnHilliard(vg, eps=3.0, D=1.0), 0.1)
+    arr = 0.5 + 0.1 * np.random.default_rng(0).random((n, n, n))
+    og, pre, rhs = oracle_imex("ch", arr, vf.spacing, "float64", 0.1, {"eps": 3.0, "D": 1.0})
+    u, ur = vg.init_scalar_field(arr), og.field(arr)
+    for i in range(3):
+        u_prev = ur
+        u = step.step(0.0, u)
+        ur = O.imex_step(ur, rhs, pre)
+        got = u.cpu()
+        ferr = float(torch.linalg.norm(got - ur) / torch.linalg.norm(ur))
+        uerr = float(torch.linalg.norm((got - u_prev) - (ur - u_prev)) / torch.linalg.norm(ur - u_prev))
+        print(f"step {i + 1}: field {ferr:.3e} update {uerr:.3e}")
+        assert ferr < 1e-6 and uerr < 1e-10
+
+
+def test_ch_imex_512_properties():
+    """512^3 float32 (the benchmark size): properties the oracle needs no run for.
+    A constant field is a fixed point; mass is conserved (k=0 prefac = dt and
+    the RHS telescopes); the step commutes with periodic shifts; the relative
+    change per step stays small for c0 = 0.5 + 0.05*N(0,1)."""
+    n = 512
+    vf = evx.VoxelFields((n, n, n), (n, n, n))
+    vg = VoxelGridHip(vf.grid_info(), precision="float32", device="cuda")
+    step = PseudoSpectralIMEX(CahnHilliard(vg, eps=3.0, D=1.0), 0.05)
+    const = torch.full((1, n, n, n), 0.375, device="cuda")
+    assert torch.equal(step.step(0.0, const), const)
+    g = torch.Generator(device="cuda").manual_seed(2)
+    u = 0.5 + 0.05 * torch.randn((1, n, n, n), device="cuda", generator=g)
+    m0 = float(vg.channel_sums(u)[0])
+    v = step.step(0.0, u)
+    m1 = float(vg.channel_sums(v)[0])
+    assert abs(m1 - m0) < 1e-6 * abs(m0)
+    sh = (37, -101, 5)
+    w = step.step(0.0, torch.roll(u, sh, (1, 2, 3)))
+    d = (torch.roll(v, sh, (1, 2, 3)) - w).abs().max().item()
+    assert d < 1e-5
+    rel = float(torch.linalg.norm(v - u) / torch.linalg.norm(u))
+    assert 0 < rel < 0.05
+
+
+def test_rd_imex_512_single_mode_closed_form():
+    """RD + IMEX at 512^3 on one Fourier mode: u1 = u0*(1 + prefac*D*lam_h)."""
+    n, D, dt = 512, 1.0, 0.05
+    vf = evx.VoxelFields((n, n, n), (n, n, n))
+    vg = VoxelGridHip(vf.grid_info(), precision="float32", device="cuda")
+    step = PseudoSpectralIMEX(ReactionDiffusion(vg, D=D), dt)
+    a = (3, 5, 7)
+    ax = [torch.arange(n, device="cuda", dtype=torch.float64) for _ in range(3)]
+    X, Y, Z = torch.meshgrid(*ax, indexing="ij")
+    u0 = torch.cos(2 * np.pi * (a[0] * X + a[1] * Y + a[2] * Z) / n).float()[None]
+    del X, Y, Z
+    lam = -sum(4 * np.sin(np.pi * a[i] / n) ** 2 for i in range(3))
+    k = [np.float32(2 * np.pi) * (np.float32(a[i]) * np.float32(1.0 / n)) for i in range(3)]
+    k2 = (k[0] * k[0] + k[1] * k[1]) + k[2] * k[2]
+    pre = (np.float32(1) / (np.float32(1) - np.float32(dt) * (np.float32(-D * 0.25) * k2))) * np.float32(dt)
+    u1 = step.step(0.0, u0)
+    expect = u0 * float(1 + float(pre) * D * lam)
+    assert (u1 - expect).abs().max().item() < 2e-6

@@ -1,0 +1,108 @@
+"""Host-side logic of the 'hip' backend that needs no GPU: the VoxelFields
+mirror (tests/test_fields.py of the reference), boundary-condition
+validation, FFT-BC checks and backend selection."""
+import types
+import warnings
+
+import numpy as np
+import pytest
+import torch
+
+import evoxels_amd as evx
+from evoxels_amd.problem_definition import CahnHilliard, ReactionDiffusion, TwoPhaseAllenCahn
+from evoxels_amd.solvers import TimeDependentSolver
+from evoxels_amd.timesteppers import PseudoSpectralIMEX
+from evoxels_amd.voxelgrid import CellCenteredBCs, VoxelGridHip
+
+
+def test_voxelfields_geometry():
+    """test_fields.py:8-33."""
+    vf = evx.VoxelFields((10, 10, 10))
+    assert vf.shape == (10, 10, 10) and (vf.Nx, vf.Ny, vf.Nz) == (10, 10, 10)
+    vf = evx.VoxelFields((10, 10, 10), (10, 10, 10))
+    assert (vf.domain_size, vf.spacing) == ((10, 10, 10), (1, 1, 1))
+    vf = evx.VoxelFields((10, 5, 7), (10, 5, 7))
+    x, y, z = vf.meshgrid()
+    assert x[-1, 0, 0] == 10 - vf.spacing[0] / 2
+    vf = evx.VoxelFields((11, 5, 7), (10, 5, 7), convention="staggered_x")
+    x, y, z = vf.meshgrid()
+    assert x[-1, 0, 0] == 10 and y[0, -1, 0] == 5 - vf.spacing[1] / 2
+
+
+def test_voxelfields_fields_and_errors():
+    vf = evx.VoxelFields((10, 10, 10), (10, 10, 10))
+    vf.add_field("c", 0.123 * np.ones((10, 10, 10)))
+    assert (vf.fields["c"][1, 2, 3], *vf.fields["c"].shape) == (0.123, 10, 10, 10)
+    with pytest.raises(ValueError):
+        vf.set_field("c", np.ones((3, 3, 3)))
+    with pytest.raises(TypeError):
+        vf.set_field("c", [1, 2])
+    with pytest.raises(ValueError):
+        evx.VoxelFields((10, 10))
+    with pytest.raises(ValueError):
+        evx.VoxelFields((4, 4, 4), convention="bad")
+    with pytest.warns(UserWarning, match="questionable"):
+        evx.VoxelFields((4, 4, 4), (1, 1, 100))
+
+
+def test_sphere_volume_fraction():
+    """test_fields.py:60-65."""
+    vf = evx.VoxelFields((6, 5, 5), convention="staggered_x")
+    vf.add_field("sphere")
+    vf.set_voxel_sphere("sphere", center=(0.5, 0.5, 0.5), radius=0.31, label=1)
+    assert np.count_nonzero(vf.fields["sphere"] == 1) == 20
+    assert vf.average("sphere") == 0.16
+
+
+def _stub_vg(convention="cell_center"):
+    vg = types.SimpleNamespace(convention=convention, shape=(4, 4, 4))
+    vg.bc = CellCenteredBCs(vg)
+    return vg
+
+
+def test_bc_normalisation_matches_reference():
+    """test_solvers.py:85-102 and problem_definition.py:60-103."""
+    with pytest.warns(UserWarning, match="reduces the spatial order of convergence to 0.5!"):
+        p = ReactionDiffusion(_stub_vg(), D=1.0, bc=(("dirichlet", (1, -1)), "periodic", "periodic"))
+    assert p.bc == (("dirichlet", (1, -1)), ("periodic", None), ("periodic", None))
+    assert p.bc_type == ("dirichlet", "periodic", "periodic")
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        p = CahnHilliard(_stub_vg())
+    assert p.bc_type == ("periodic",) * 3 and p.fused
+    for bad, msg in [(("periodic",) * 2, "exactly three"), (("dirichlet", "periodic", "periodic"), "explicit values"),
+                     (("robin", "periodic", "periodic"), "Unsupported BC type"),
+                     ((("neumann", (1, 2)), "periodic", "periodic"), "do not accept")]:
+        with pytest.raises(ValueError, match=msg):
+            ReactionDiffusion(_stub_vg(), D=1.0, bc=bad)
+
+
+def test_fft_bc_validation():
+    """test_solvers.py:165-171 and problem_definition.py:138-167."""
+    with pytest.raises(ValueError, match="support at most one non-periodic axis"):
+        PseudoSpectralIMEX(TwoPhaseAllenCahn(_stub_vg()), 0.1)
+    p = ReactionDiffusion(_stub_vg(), D=1.0, bc=("periodic", "neumann", "periodic"))
+    with pytest.raises(NotImplementedError, match="only implement the single non-periodic axis"):
+        p.verify_fft_bc_config()
+
+
+def test_custom_closures_disable_fusion():
+    assert not CahnHilliard(_stub_vg(), mu_hom=lambda c: c).fused
+    assert not ReactionDiffusion(_stub_vg(), D=1.0, f=lambda t, c, lib: c).fused
+    assert TwoPhaseAllenCahn(_stub_vg()).fused
+
+
+def test_backend_selection():
+    vf = evx.VoxelFields((4, 4, 4))
+    vf.add_field("c")
+    with pytest.raises(ValueError, match="Unsupported backend: torch"):
+        TimeDependentSolver(vf, "c", backend="torch")
+
+
+def test_hip_grid_refuses_cpu_device():
+    vf = evx.VoxelFields((4, 4, 4))
+    with pytest.raises(RuntimeError, match="ROCm"):
+        VoxelGridHip(vf.grid_info(), device="cpu")
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError, match="ROCm GPU"):
+            VoxelGridHip(vf.grid_info(), device="cuda")

@@ -1,0 +1,332 @@
+"""Pins of the CPU oracle (oracle/evoxels_oracle.py) against the reference's own
+known answers and against what the mathematics fixes.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+import sympy as sp
+import torch
+
+from oracle import evoxels_oracle as O
+from tests import mms
+
+CS = mms.CS
+P3 = (("periodic", None),) * 3
+
+
+def grid(shape, domain=None, precision="float64"):
+    domain = domain or tuple(float(n) for n in shape)
+    return O.OracleGrid.from_domain(shape, domain, precision)
+
+
+# ----------------------------------------------------------------- padding --
+def test_pad_kat_reference_mixed_bc():
+    """test_solvers.py:105-126, the reference's exact padded array."""
+    with pytest.warns(UserWarning, match="order of convergence to 0.5"):
+        bc = O.normalize_bc((("dirichlet", (10.0, 20.0)), "neumann", "periodic"))
+    f = torch.tensor(np.arange(1, 9, dtype=np.float32).reshape(1, 2, 2, 2))
+    padded = O.pad_bc(f, bc)[0].numpy()
+    expected = np.pad(np.arange(1, 9, dtype=np.float32).reshape(2, 2, 2), 1, mode="wrap")
+    expected[0, :, :] = 2.0 * 10.0 - expected[1, :, :]
+    expected[-1, :, :] = 2.0 * 20.0 - expected[-2, :, :]
+    expected[:, 0, :] = expected[:, 1, :]
+    expected[:, -1, :] = expected[:, -2, :]
+    assert np.array_equal(padded, expected)
+
+
+def test_bc_normalisation_and_errors():
+    """problem_definition.py:60-103 and test_solvers.py:85-102."""
+    with pytest.warns(UserWarning, match="reduces the spatial order of convergence to 0.5!"):
+        bc = O.normalize_bc((("dirichlet", (1, -1)), "periodic", "periodic"))
+    assert bc == (("dirichlet", (1, -1)), ("periodic", None), ("periodic", None))
+    assert O.normalize_bc(None) == P3
+    for bad, msg in [(("periodic",) * 2, "exactly three"), (("dirichlet", "periodic", "periodic"), "explicit values"),
+                     (("robin", "periodic", "periodic"), "Unsupported BC type"),
+                     ((("neumann", (1, 2)), "periodic", "periodic"), "do not accept"),
+                     ((("dirichlet", (1,)), "periodic", "periodic"), "two boundary values")]:
+        with pytest.raises(ValueError, match=msg):
+            O.normalize_bc(bad)
+
+
+def test_pad_ghost_values_by_hand():
+    """Hand-computed ghost values, including edges/corners where rules compose."""
+    u = torch.arange(1, 28, dtype=torch.float64).reshape(1, 3, 3, 3)  # u[i,j,k] = 1 + 9i + 3j + k
+    bc = (("dirichlet", (1.0, 2.0)), ("neumann", None), ("dirichlet", (3.0, 4.0)))
+    p = O.pad_bc(u, bc)[0]
+    U = lambda i, j, k: 1 + 9 * i + 3 * j + k  # noqa: E731
+    # face ghosts
+    assert p[0, 2, 2] == 2 * 1.0 - U(0, 1, 1)
+    assert p[4, 2, 2] == 2 * 2.0 - U(2, 1, 1)
+    assert p[2, 0, 2] == U(1, 0, 1)
+    assert p[2, 2, 0] == 2 * 3.0 - U(1, 1, 0)
+    # x-y edge: x Dirichlet then y Neumann copy of the x ghost
+    assert p[0, 0, 2] == 2 * 1.0 - U(0, 0, 1)
+    # corner: z rule applied to the x-ruled, y-copied value
+    assert p[0, 0, 0] == 2 * 3.0 - (2 * 1.0 - U(0, 0, 0))
+    assert p[4, 4, 4] == 2 * 4.0 - (2 * 2.0 - U(2, 2, 2))
+    # periodic y with Neumann x: edge = x copy of the y-wrapped value
+    bc2 = (("neumann", None), ("periodic", None), ("periodic", None))
+    p2 = O.pad_bc(u, bc2)[0]
+    assert p2[0, 0, 2] == U(0, 2, 1)
+    assert p2[4, 4, 0] == U(2, 0, 2)
+
+
+# --------------------------------------------------------------- Laplacian --
+@pytest.mark.parametrize("shape,spacing", [((8, 6, 10), (0.7, 1.3, 0.9)), ((5, 7, 9), (1.0, 1.0, 1.0))])
+def test_laplace_periodic_fourier_eigenvalue(shape, spacing):
+    """Discrete Fourier modes are exact eigenfunctions of the periodic 7-point Laplacian."""
+    g = O.OracleGrid(shape, spacing, "float64")
+    a = (2, 1, 3)
+    idx = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    phase = sum(2 * np.pi * a[i] * idx[i] / shape[i] for i in range(3))
+    u = torch.tensor(np.cos(phase))[None]
+    lam = -sum(4 / spacing[i] ** 2 * np.sin(np.pi * a[i] / shape[i]) ** 2 for i in range(3))
+    lap = O.laplace(O.pad_bc(u, P3), g.div_dx2)
+    np.testing.assert_allclose(lap.numpy(), lam * u.numpy(), rtol=0, atol=1e-12 * abs(lam))
+
+
+def _oracle_rhs(kind, bc, precision, **kw):
+    def f(arr, n):
+        g = O.OracleGrid.from_domain((n, n, n), (1, 1, 1), precision)
+        u = g.field(arr)
+        if kind == "rd":
+            r = O.rd_rhs(u, g, bc, kw.get("D", 1.0))
+        elif kind == "ch":
+            r = O.ch_rhs(u, g, bc, **kw)
+        else:
+            r = O.ac_rhs(u, g, bc, **kw)
+        return r[0].numpy()
+    return f
+
+
+def test_mms_periodic_laplace_slope():
+    """test_laplace.py:21-33 (float32, periodic)."""
+    e = sp.sin(2 * sp.pi * CS.x) * sp.sin(4 * sp.pi * CS.y) * sp.sin(6 * sp.pi * CS.z) / (56 * sp.pi ** 2)
+    _, _, slope = mms.convergence(e, mms.rd_analytic(e), _oracle_rhs("rd", P3, "float32"), "float32")
+    assert abs(slope - 2) < 0.1
+
+
+def test_mms_neumann_laplace_matches_notebook_slope():
+    """Zero-flux 7-point Laplacian, cell_center, float32: the reference's notebook
+    reports slope 1.9533270430146406 (10-appendix-convergence-testing.ipynb, cell 9)."""
+    e = sp.cos(sp.pi * CS.x) ** 3
+    bc = (("neumann", None), ("periodic", None), ("periodic", None))
+    _, _, slope = mms.convergence(e, mms.rd_analytic(e), _oracle_rhs("rd", bc, "float32"), "float32")
+    assert abs(slope - 1.9533270430146406) < 2e-3
+
+
+def test_mms_dirichlet_laplace_matches_notebook_slope():
+    """Zero Dirichlet, cell_center, float32: notebook slope 0.4954937374912848 (cell 11)."""
+    e = (CS.x * (1 - CS.x)) ** 2
+    bc = (("dirichlet", (0, 0)), ("periodic", None), ("periodic", None))
+    _, _, slope = mms.convergence(e, mms.rd_analytic(e), _oracle_rhs("rd", bc, "float32"), "float32")
+    assert abs(slope - 0.4954937374912848) < 2e-3
+
+
+def test_mms_cahn_hilliard_slope():
+    """test_rhs.py:13-21 (float64)."""
+    e = 0.4 + 0.1 * sp.sin(2 * sp.pi * CS.x)
+    _, _, slope = mms.convergence(e, mms.ch_analytic(e, 3.0, 1.0), _oracle_rhs("ch", P3, "float64", eps=3.0, D=1.0))
+    assert abs(slope - 2) < 0.1
+
+
+def test_mms_allen_cahn_matches_notebook_errors():
+    """test_rhs.py:29-37 slope, and the error array of
+    10-appendix-convergence-testing.ipynb cell 16 (curvature 0.5, force 0)."""
+    e = 0.5 + 0.3 * sp.cos(4 * sp.pi * CS.x) * sp.cos(2 * sp.pi * CS.y) * (CS.z ** 2 / 2 - CS.z ** 3 / 3)
+    bc = (("neumann", None),) * 3
+    _, errs, slope = mms.convergence(e, mms.ac_analytic(e, eps=3.0, curvature=0.5),
+                                     _oracle_rhs("ac", bc, "float64", eps=3.0, curvature=0.5))
+    np.testing.assert_allclose(errs, [0.19028211, 0.05396869, 0.01346211, 0.00334846, 0.00102839], rtol=5e-6)
+    assert abs(slope - 1.907376693930398) < 1e-5
+    _, _, slope_f = mms.convergence(e, mms.ac_analytic(e, eps=3.0, curvature=0.5, force=1.0),
+                                    _oracle_rhs("ac", bc, "float64", eps=3.0, curvature=0.5, force=1.0))
+    assert abs(slope_f - 2) < 0.1
+
+
+# ------------------------------------------------------------ CH invariants --
+def _rand_field(shape, seed, lo=0.2, hi=0.8):
+    rng = np.random.default_rng(seed)
+    return torch.tensor(lo + (hi - lo) * rng.random(shape))[None]
+
+
+@pytest.mark.parametrize("bc", [P3, (("neumann", None), ("periodic", None), ("neumann", None))])
+def test_ch_rhs_conserves_mass(bc):
+    g = grid((9, 8, 7))
+    r = O.ch_rhs(_rand_field((9, 8, 7), 1), g, bc, eps=2.5, D=1.3)
+    assert abs(float(r.sum())) < 1e-12 * float(r.abs().sum())
+
+
+def test_ch_rhs_constant_field_and_translation():
+    g = grid((8, 8, 8))
+    assert float(O.ch_rhs(torch.full((1, 8, 8, 8), 0.37, dtype=torch.float64), g, P3).abs().max()) == 0.0
+    u = _rand_field((8, 8, 8), 2)
+    r = O.ch_rhs(u, g, P3)
+    rr = O.ch_rhs(torch.roll(u, (2, -3, 1), (1, 2, 3)), g, P3)
+    assert torch.equal(torch.roll(r, (2, -3, 1), (1, 2, 3)), rr)
+
+
+def test_ch_rhs_single_mode_linearisation():
+    """rhs(cbar + d*phi) = D*cbar(1-cbar)*lam*(f''(cbar) - 2*eps*lam)*d*phi + O(d^2)."""
+    shape, sp_ = (16, 12, 10), (0.8, 1.1, 0.9)
+    g = O.OracleGrid(shape, sp_, "float64")
+    a = (1, 2, 1)
+    idx = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    phi = np.cos(sum(2 * np.pi * a[i] * idx[i] / shape[i] for i in range(3)))
+    lam = -sum(4 / sp_[i] ** 2 * np.sin(np.pi * a[i] / shape[i]) ** 2 for i in range(3))
+    cbar, eps, D = 0.4, 2.0, 1.5
+    fpp = 18 / eps * (1 - 6 * cbar + 6 * cbar ** 2)
+    for d in (1e-4, 1e-5):
+        r = O.ch_rhs(torch.tensor(cbar + d * phi)[None], g, P3, eps=eps, D=D)[0].numpy()
+        lin = D * cbar * (1 - cbar) * lam * (fpp - 2 * eps * lam) * d * phi
+        assert np.abs(r - lin).max() < 50 * d * np.abs(lin).max()
+
+
+# ---------------------------------------------------------- normal Laplacian --
+def test_normal_laplace_planar_profile_and_bulk():
+    g = grid((12, 5, 6), (12.0, 5.0, 6.0))
+    x = np.arange(12) + 0.5
+    prof = np.tanh((x - 6.0) / 2.0)
+    u = torch.tensor(np.broadcast_to(prof[:, None, None], (12, 5, 6)).copy())[None]
+    bc = (("neumann", None), ("periodic", None), ("periodic", None))
+    pu = O.pad_bc(u, bc)
+    nl = O.normal_laplace(pu, g.div_dx, g.div_dx2)
+    lx = (pu[:, 2:, 1:-1, 1:-1] - 2 * pu[:, 1:-1, 1:-1, 1:-1] + pu[:, :-2, 1:-1, 1:-1])
+    grad = 0.5 * (pu[:, 2:, 1:-1, 1:-1] - pu[:, :-2, 1:-1, 1:-1])
+    mask = (grad ** 2 > 1e-7)
+    np.testing.assert_allclose(nl[mask].numpy(), lx[mask].numpy(), rtol=1e-12, atol=1e-14)
+    const = O.pad_bc(torch.full((1, 4, 4, 4), 0.3, dtype=torch.float64), P3)
+    assert float(O.normal_laplace(const, g.div_dx, g.div_dx2).abs().max()) == 0.0
+
+
+def test_allen_cahn_tanh_equilibrium():
+    """test_solvers.py:30-82 (Allen-Cahn leg): RK4, 10 steps of 0.5 from a step profile."""
+    nx, eps = 16, 3.0
+    g = grid((nx, 1, 1), (nx, 1, 1))
+    phi = np.zeros((nx, 1, 1), dtype=np.float32)
+    phi[: nx // 2] = 1.0
+    u = torch.tensor(phi)[None]
+    g32 = O.OracleGrid.from_domain((nx, 1, 1), (nx, 1, 1), "float32")
+    bc = (("neumann", None),) * 3
+    rhs = lambda t, v: O.ac_rhs(v, g32, bc, eps=eps)  # noqa: E731
+    for i in range(10):
+        u = O.rk4_step(u, rhs, 0.5, t=i * 0.5)
+    x = np.arange(nx) + 0.5
+    ana = 0.5 - 0.5 * np.tanh(3 * (x - 0.5 * nx) / 2 / eps)
+    assert np.linalg.norm(u[0, :, 0, 0].numpy() - ana) < 0.05
+
+
+# ------------------------------------------------------- k tables, steppers --
+def test_k_table_float32_quirk():
+    for n, d in [(7, 0.3), (100, 1.0), (512, 1.0), (256, 1 / 256)]:
+        f = torch.fft.fftfreq(n, d)
+        assert f.dtype == torch.float32
+        j = np.arange(n)
+        js = np.where(j < (n + 1) // 2, j, j - n).astype(np.float32)
+        assert np.array_equal(f.numpy(), js * np.float32(1.0 / (n * d)))
+    k2 = O.rfft_k_squared((6, 5, 8), (1.0, 0.5, 2.0))
+    assert k2.dtype == torch.float32 and k2.shape == (6, 5, 5)
+    x = torch.tensor([3.0, 0.7, 11.0], dtype=torch.float32)
+    assert torch.equal(O.imex_prefac(0.1, x), torch.reciprocal(1 - 0.1 * x) * 0.1)
+
+
+def _mode(shape, a):
+    idx = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    return np.cos(sum(2 * np.pi * a[i] * idx[i] / shape[i] for i in range(3)))
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_rd_imex_single_mode_closed_form(precision):
+    """RD + IMEX on one Fourier mode: u1 = u0*(1 + prefac(k)*D*lam_h(k)) with
+    prefac from float32 k^2 (computed here independently with numpy)."""
+    shape, spc, D, dt = (8, 6, 10), (0.5, 1.0, 2.0), 0.7, 0.3
+    a = (1, 2, 3)
+    g = O.OracleGrid(shape, spc, precision)
+    u0 = torch.tensor(_mode(shape, a), dtype=g.dtype)[None]
+    lam = -sum(4 / spc[i] ** 2 * np.sin(np.pi * a[i] / shape[i]) ** 2 for i in range(3))
+    k = [np.float32(2 * np.pi) * (np.float32(a[i]) * np.float32(1.0 / (shape[i] * spc[i]))) for i in range(3)]
+    k2 = (k[0] * k[0] + k[1] * k[1]) + k[2] * k[2]
+    sym = np.float32(-D * 0.25) * k2
+    prefac = (np.float32(1) / (np.float32(1) - np.float32(dt) * sym)) * np.float32(dt)
+    k2t = O.rfft_k_squared(shape, spc)
+    pre = O.imex_prefac(dt, O.rd_symbol(k2t, D))
+    u1 = O.imex_step(u0, lambda t, v: O.rd_rhs(v, g, P3, D), pre)
+    expect = u0.numpy() * (1 + float(prefac) * D * lam)
+    tol = 1e-12 if precision == "float64" else 2e-6
+    np.testing.assert_allclose(u1.numpy(), expect, atol=tol)
+
+
+def _dft3(x, inverse=False):
+    """Brute-force multidimensional DFT (O(N^2) per axis)."""
+    out = x.astype(complex)
+    for ax in range(x.ndim):
+        n = x.shape[ax]
+        s = 1 if inverse else -1
+        W = np.exp(s * 2j * np.pi * np.outer(np.arange(n), np.arange(n)) / n)
+        out = np.moveaxis(np.tensordot(W, np.moveaxis(out, ax, 0), axes=(1, 0)), 0, ax)
+    return out / (np.prod(x.shape) if inverse else 1)
+
+
+def test_imex_step_matches_brute_force_dft():
+    shape = (3, 4, 5)
+    g = O.OracleGrid(shape, (1.0, 1.0, 1.0), "float64")
+    u = _rand_field(shape, 4)
+    k2 = O.rfft_k_squared(shape, g.spacing)
+    pre = O.imex_prefac(0.2, O.ch_symbol(k2, 3.0, 1.0))
+    rhs = lambda t, v: O.ch_rhs(v, g, P3, 3.0, 1.0)  # noqa: E731
+    u1 = O.imex_step(u, rhs, pre)[0].numpy()
+    # full-spectrum prefac by Hermitian symmetry of the float32 table
+    kk = [2 * np.pi * np.fft.fftfreq(n, 1.0) for n in shape]
+    KX, KY, KZ = np.meshgrid(*kk, indexing="ij")
+    full = np.empty(shape)
+    nh = shape[2] // 2 + 1
+    for i in range(shape[0]):
+        for j in range(shape[1]):
+            for l in range(shape[2]):
+                if l < nh:
+                    full[i, j, l] = pre[i, j, l]
+                else:
+                    full[i, j, l] = pre[(-i) % shape[0], (-j) % shape[1], shape[2] - l]
+    r = rhs(0, u)[0].numpy()
+    upd = _dft3(full * _dft3(r), inverse=True).real
+    np.testing.assert_allclose(u1, u[0].numpy() + upd, atol=1e-13)
+
+
+def test_fe_and_rk4_closed_forms_on_a_mode():
+    shape, D, dt = (8, 8, 8), 0.9, 0.05
+    g = grid(shape)
+    a = (1, 0, 2)
+    u0 = torch.tensor(_mode(shape, a))[None]
+    lam = -sum(4 * np.sin(np.pi * a[i] / shape[i]) ** 2 for i in range(3))
+    z = dt * D * lam
+    rhs = lambda t, v: O.rd_rhs(v, g, P3, D)  # noqa: E731
+    np.testing.assert_allclose(O.fe_step(u0, rhs, dt).numpy(), (1 + z) * u0.numpy(), atol=1e-13)
+    np.testing.assert_allclose(O.rk4_step(u0, rhs, dt).numpy(),
+                               (1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24) * u0.numpy(), atol=1e-13)
+
+
+def test_run_loop_semantics():
+    """test_solvers.py:14-27 (u+1 with two fields) and the frame schedule of solvers.py:192-208."""
+    u = torch.stack([torch.ones(4, 4, 4), torch.zeros(4, 4, 4)])
+    seen = []
+    out = O.run_loop(u, lambda t, v: v + 1, 0.1, 1, 1, lambda v, f, t: seen.append((f, round(t, 12))))
+    assert torch.all(out[0] == 2) and torch.all(out[1] == 1)
+    assert seen == [(0, 0.0), (1, 0.1)]
+    seen.clear()
+    O.run_loop(u, lambda t, v: v, 0.5, 3, 10, lambda v, f, t: seen.append((f, t)))
+    assert [f for f, _ in seen] == [0, 1, 2, 3, 4] and seen[-1][1] == 5.0
+    with pytest.raises(ZeroDivisionError):
+        O.run_loop(u, lambda t, v: v, 0.5, 20, 10)
+
+
+def test_ch_energy_gradient_term_closed_form():
+    shape, L = (16, 4, 4), 16.0
+    g = grid(shape, (L, 4.0, 4.0))
+    x = np.arange(16) + 0.5
+    delta, eps = 0.05, 2.0
+    c = 0.5 + delta * np.cos(2 * np.pi * x / L)
+    u = torch.tensor(np.broadcast_to(c[:, None, None], shape).copy())[None]
+    e = float(O.ch_energy(u, g, P3, eps)[0])
+    grad_part = eps * delta ** 2 * np.sin(2 * np.pi / L) ** 2 * (16 * 16 / 2)
+    bulk = 9 / eps * np.sum(np.broadcast_to((c ** 2 * (1 - c) ** 2)[:, None, None], shape))
+    assert abs(e - (grad_part + bulk)) < 1e-12 * e
