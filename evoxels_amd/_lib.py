@@ -74,6 +74,7 @@ SIGNATURES = {
     "evx_irfftn": [_P, _P, _P, _P],
     "evx_ch_imex_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
     "evx_rd_imex_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
+    "evx_fft_plan_trace_next_step": [_P, C.POINTER(_P), C.c_int],
     "evx_reduce_workspace_bytes": [_I64, C.POINTER(C.c_size_t)],
     "evx_sum_f64": [_I32, _P, _I64, _I64, _P, _P, _P, _P],
     "evx_ch_energy": [_I32, _P, C.POINTER(EvxGrid), C.POINTER(EvxBC), _D, _D, _P, _P, _P],

@@ -429,6 +429,9 @@ struct evx_fft_plan {
   evx::Dims d;
   int64_t nh;
   evx::AxisHost ax[3];  // x, y, z (z: complex length nz)
+  // One-shot stage trace: events recorded around the kernels of the next step.
+  hipEvent_t trace[8] = {};
+  int ntrace = 0;
 };
 
 namespace evx {
@@ -578,14 +581,29 @@ size_t spec_bytes(const evx_fft_plan* p) {
   return align_up((size_t)p->d.c * p->d.nx * p->d.ny * p->nh * (p->dtype == EVX_F32 ? 8 : 16));
 }
 
+// Records trace event k (if the caller armed one) on the step's stream.
+int mark(evx_fft_plan* p, int k, hipStream_t s) {
+  if (k < p->ntrace && p->trace[k]) {
+    if (hipEventRecord(p->trace[k], s) != hipSuccess) return fail(EVX_ERR_HIP, "hipEventRecord failed");
+  }
+  return EVX_OK;
+}
+
 template <typename T>
 int spectral_update(evx_fft_plan* p, const void* rhs, void* spec, const void* u, void* u_out, const Prefac& pf,
                     hipStream_t s) {
   EVX_TRY(run_z_r2c<T>(p, rhs, spec, s));
+  EVX_TRY(mark(p, 2, s));
   EVX_TRY((run_strided<T, 0>(p, 1, spec, pf, s)));
+  EVX_TRY(mark(p, 3, s));
   EVX_TRY((run_strided<T, 2>(p, 0, spec, pf, s)));
+  EVX_TRY(mark(p, 4, s));
   EVX_TRY((run_strided<T, 1>(p, 1, spec, pf, s)));
-  return run_z_c2r<T>(p, spec, u_out, u, 1.0, s);
+  EVX_TRY(mark(p, 5, s));
+  EVX_TRY(run_z_c2r<T>(p, spec, u_out, u, 1.0, s));
+  EVX_TRY(mark(p, 6, s));
+  p->ntrace = 0;
+  return EVX_OK;
 }
 
 int check_plan(const evx_fft_plan* p) {
@@ -729,7 +747,9 @@ extern "C" int evx_ch_imex_step(evx_fft_plan* p, const void* u, void* u_out, voi
   geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
   void* rhs = ws;
   void* spec = static_cast<char*>(ws) + real_bytes(p);
+  EVX_TRY(mark(p, 0, s));
   EVX_TRY(stencil_rhs_ch_periodic(p->dtype, u, rhs, p->d, dd, dd2, prm->eps, prm->D, s));
+  EVX_TRY(mark(p, 1, s));
   // problem_definition.py:303 -> ((( -2*eps)*D)*A) as a Python double
   const Prefac pf = make_prefac(p, 1, ((-2.0 * prm->eps) * prm->D) * prm->A, prm->dt, prm->spacing);
   return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
@@ -744,9 +764,19 @@ extern "C" int evx_rd_imex_step(evx_fft_plan* p, const void* u, void* u_out, voi
   geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
   void* rhs = ws;
   void* spec = static_cast<char*>(ws) + real_bytes(p);
+  EVX_TRY(mark(p, 0, s));
   EVX_TRY(stencil_rhs_rd_periodic(p->dtype, u, rhs, p->d, dd2, prm->D, s));
+  EVX_TRY(mark(p, 1, s));
   // problem_definition.py:206 -> (-D)*A as a Python double
   const Prefac pf = make_prefac(p, 0, (-prm->D) * prm->A, prm->dt, prm->spacing);
   return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
                              : spectral_update<double>(p, rhs, spec, u, u_out, pf, s);
+}
+
+extern "C" int evx_fft_plan_trace_next_step(evx_fft_plan* p, void* const* events, int n_events) {
+  EVX_TRY(check_plan(p));
+  if (n_events < 0 || n_events > 8 || (n_events > 0 && !events)) return fail(EVX_ERR_INVALID, "bad trace events");
+  for (int i = 0; i < n_events; ++i) p->trace[i] = static_cast<hipEvent_t>(events[i]);
+  p->ntrace = n_events;
+  return EVX_OK;
 }

@@ -175,6 +175,16 @@ int evx_ch_imex_step(evx_fft_plan* plan, const void* u, void* u_out,
 int evx_rd_imex_step(evx_fft_plan* plan, const void* u, void* u_out,
                      void* workspace, const evx_imex_params* p, void* stream);
 
+/* Stage timing for benchmarks (no reference counterpart; the reference times
+ * whole runs with a wall clock, solvers.py:108-114).  Arms the next
+ * evx_ch_imex_step / evx_rd_imex_step on this plan to record the caller's
+ * hipEvent_t handles events[0..n_events) around its kernels: [0] before the
+ * RHS, [1] after it, [2] after the z R2C sweep, [3] after forward y, [4] after
+ * the fused x sweep, [5] after inverse y, [6] after the z C2R sweep.  The
+ * arming is consumed by that one step. */
+int evx_fft_plan_trace_next_step(evx_fft_plan* plan, void* const* events,
+                                 int n_events);
+
 /* ---- reductions ----------------------------------------------------------- */
 
 /* Bytes of device workspace evx_sum_f64 / evx_ch_energy need for `channels`. */
