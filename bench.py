@@ -40,16 +40,18 @@ def log(msg):
 
 
 def stage_bytes(n, s=4):
-    """Algorithmic HBM bytes per voxel of each kernel of the step."""
+    """(kernel, algorithmic HBM bytes per voxel) for the trace intervals of a
+    step: [0,1] RHS, [1,2] z R2C, [2,3] y forward, [3,4] x forward*prefac*inverse,
+    [4,5] y inverse, [5,6] z C2R + add."""
     nh = n // 2 + 1
     spec = 2 * s * nh / n  # one half-spectrum array, per voxel
     return [
-        ("ch_rhs_kernel", 2 * s),                      # read c, write rhs
-        ("z_r2c_kernel", s + spec),                    # read rhs, write spectrum
-        ("strided_kernel<y,fwd>", 2 * spec),           # read + write spectrum
-        ("strided_kernel<x,fwd*prefac*inv>", 2 * spec),
-        ("strided_kernel<y,inv>", 2 * spec),
-        ("z_c2r_kernel", spec + 2 * s),                # read spectrum + u, write u
+        ("ch_rhs_kernel", 2 * s),                    # read c, write rhs
+        ("z_r2c_fast", s + spec),                    # read rhs, write spectrum
+        ("strided_fast<y,fwd>", 2 * spec),           # read + write spectrum
+        ("strided_fast<x,fwd*prefac*inv>", 2 * spec),
+        ("strided_fast<y,inv>", 2 * spec),
+        ("z_c2r_fast", spec + 2 * s),                # read spectrum + u, write u
     ]
 
 
@@ -165,7 +167,7 @@ def main():
         kernels.append({"kernel": name, "ms": round(float(ms), 4), "alg_bytes_per_voxel": round(bpv, 4),
                         "alg_bytes_per_launch": int(round(bpv * V)),
                         "gbps": round(bpv * V / (ms * 1e-3) / 1e9, 1)})
-    dom = int(np.argmax(mean_ms))
+    dom = int(np.argmax([k["ms"] for k in kernels]))
     dk = kernels[dom]
     roof = {"bound": "hbm", "kernel": dk["kernel"], "achieved": dk["gbps"], "peak": PEAK_GBPS, "unit": "GB/s",
             "frac": round(dk["gbps"] / PEAK_GBPS, 4), "traffic": traffic_from_profiles(dk["kernel"]),

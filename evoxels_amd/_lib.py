@@ -29,6 +29,8 @@ BC_KIND = {"periodic": 0, "neumann": 1, "dirichlet": 2}
 
 OUT_RHS, OUT_EULER, OUT_RK4_FIRST, OUT_RK4_MID, OUT_RK4_LAST = range(5)
 
+OPT_SPECIALIZED = 1
+
 
 class EvxBC(C.Structure):
     _fields_ = [("kind", C.c_int32 * 3), ("lo", C.c_double * 3), ("hi", C.c_double * 3)]
@@ -75,6 +77,7 @@ SIGNATURES = {
     "evx_ch_imex_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
     "evx_rd_imex_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
     "evx_fft_plan_trace_next_step": [_P, C.POINTER(_P), C.c_int],
+    "evx_fft_plan_set_option": [_P, C.c_int, C.c_int],
     "evx_reduce_workspace_bytes": [_I64, C.POINTER(C.c_size_t)],
     "evx_sum_f64": [_I32, _P, _I64, _I64, _P, _P, _P, _P],
     "evx_ch_energy": [_I32, _P, C.POINTER(EvxGrid), C.POINTER(EvxBC), _D, _D, _P, _P, _P],

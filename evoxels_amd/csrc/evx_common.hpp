@@ -60,6 +60,19 @@ struct Resolved {
   int8_t o[3];   // -1 low ghost, +1 high ghost, 0 inside (non-periodic axes)
 };
 
+// Periodic wrap for points at most one period outside the grid (the stencil
+// halos); falls back to a division only for degenerate tiny axes.
+__host__ __device__ inline int64_t wrap_near(int64_t i, int64_t n) {
+  if (i < 0) {
+    i += n;
+    if (i < 0) i = wrap(i, n);
+  } else if (i >= n) {
+    i -= n;
+    if (i >= n) i = wrap(i, n);
+  }
+  return i;
+}
+
 template <typename T>
 __device__ __forceinline__ Resolved<T> resolve(int64_t x, int64_t y, int64_t z,
                                                const Dims& d, const Bc& bc) {
@@ -71,7 +84,7 @@ __device__ __forceinline__ Resolved<T> resolve(int64_t x, int64_t y, int64_t z,
     r.o[a] = 0;
     if (p[a] < 0 || p[a] >= n[a]) {
       if (bc.kind[a] == EVX_BC_PERIODIC) {
-        p[a] = wrap(p[a], n[a]);
+        p[a] = wrap_near(p[a], n[a]);
       } else {
         r.o[a] = p[a] < 0 ? -1 : 1;
         p[a] = p[a] < 0 ? 0 : n[a] - 1;

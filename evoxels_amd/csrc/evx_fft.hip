@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "evx_common.hpp"
+#include "evx_fft_engine.hpp"
 
 namespace evx {
 
@@ -42,84 +43,6 @@ struct Axis {
   int radix[MAXPASS];
   const cpx<T>* tw;  // n roots of unity exp(-2 pi i m / n)
 };
-
-// ------------------------------------------------------------ butterflies --
-template <typename T, bool INV>
-__device__ __forceinline__ cpx<T> mul_mi(cpx<T> z) {  // times -i (forward) or +i (inverse)
-  return INV ? cpx<T>{-z.y, z.x} : cpx<T>{z.y, -z.x};
-}
-
-template <typename T, bool INV>
-__device__ __forceinline__ void dft2(cpx<T>* x) {
-  const cpx<T> a = x[0], b = x[1];
-  x[0] = cadd(a, b);
-  x[1] = csub(a, b);
-}
-
-template <typename T, bool INV>
-__device__ __forceinline__ void dft4(cpx<T>& a0, cpx<T>& a1, cpx<T>& a2, cpx<T>& a3) {
-  const cpx<T> s0 = cadd(a0, a2), d0 = csub(a0, a2), s1 = cadd(a1, a3), d1 = mul_mi<T, INV>(csub(a1, a3));
-  a0 = cadd(s0, s1);
-  a2 = csub(s0, s1);
-  a1 = cadd(d0, d1);
-  a3 = csub(d0, d1);
-}
-
-template <typename T, bool INV>
-__device__ __forceinline__ void dft3(cpx<T>* x) {
-  const T h = T(0.86602540378443864676);  // sin(2 pi / 3)
-  const cpx<T> a = cadd(x[1], x[2]), b = csub(x[1], x[2]);
-  const cpx<T> y0 = cadd(x[0], a);
-  const cpx<T> t = {x[0].x - T(0.5) * a.x, x[0].y - T(0.5) * a.y};
-  const cpx<T> v = mul_mi<T, INV>(cscale(b, h));
-  x[0] = y0;
-  x[1] = cadd(t, v);
-  x[2] = csub(t, v);
-}
-
-template <typename T, bool INV>
-__device__ __forceinline__ void dft5(cpx<T>* x) {
-  const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);  // cos(2pi/5), cos(4pi/5)
-  const T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);   // sin(2pi/5), sin(4pi/5)
-  const cpx<T> a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
-  const cpx<T> a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
-  const cpx<T> x0 = x[0];
-  const cpx<T> t1 = {x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y};
-  const cpx<T> t2 = {x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y};
-  const cpx<T> u1 = mul_mi<T, INV>(cpx<T>{s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y});
-  const cpx<T> u2 = mul_mi<T, INV>(cpx<T>{s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y});
-  x[0] = cadd(x0, cadd(a1, a2));
-  x[1] = cadd(t1, u1);
-  x[4] = csub(t1, u1);
-  x[2] = cadd(t2, u2);
-  x[3] = csub(t2, u2);
-}
-
-template <typename T, bool INV>
-__device__ __forceinline__ void dft8(cpx<T>* x) {
-  const T h = T(0.70710678118654752440);  // sqrt(2)/2
-  cpx<T> a0 = cadd(x[0], x[4]), b0 = csub(x[0], x[4]);
-  cpx<T> a1 = cadd(x[1], x[5]), b1 = csub(x[1], x[5]);
-  cpx<T> a2 = cadd(x[2], x[6]), b2 = csub(x[2], x[6]);
-  cpx<T> a3 = cadd(x[3], x[7]), b3 = csub(x[3], x[7]);
-  // b_r *= w8^r (forward w8 = e^{-i pi/4}; inverse uses the conjugate)
-  b1 = INV ? cpx<T>{(b1.x - b1.y) * h, (b1.y + b1.x) * h} : cpx<T>{(b1.x + b1.y) * h, (b1.y - b1.x) * h};
-  b2 = mul_mi<T, INV>(b2);
-  b3 = INV ? cpx<T>{(-b3.x - b3.y) * h, (b3.x - b3.y) * h} : cpx<T>{(b3.y - b3.x) * h, (-b3.x - b3.y) * h};
-  dft4<T, INV>(a0, a1, a2, a3);
-  dft4<T, INV>(b0, b1, b2, b3);
-  x[0] = a0; x[2] = a1; x[4] = a2; x[6] = a3;
-  x[1] = b0; x[3] = b1; x[5] = b2; x[7] = b3;
-}
-
-template <typename T, bool INV, int R>
-__device__ __forceinline__ void dft_fixed(cpx<T>* x) {
-  if constexpr (R == 2) dft2<T, INV>(x);
-  else if constexpr (R == 3) dft3<T, INV>(x);
-  else if constexpr (R == 4) dft4<T, INV>(x[0], x[1], x[2], x[3]);
-  else if constexpr (R == 5) dft5<T, INV>(x);
-  else dft8<T, INV>(x);
-}
 
 // One Stockham pass of radix R over every line of the workgroup.
 // p = product of the radices already applied.  Butterfly i (0 <= i < n/R)
@@ -376,6 +299,245 @@ __global__ void strided_kernel(cpx<T>* spec, int64_t nlines, LineMap lm, Axis<T>
     for (int j = tl; j < n; j += jstep) spec[start + j * lm.es] = buf[j * lpw + line];
 }
 
+
+// --------------------------------------------------------- specialised sizes --
+// Compile-time radix plans: L = lcm of the radices (minimum values per thread).
+template <int N> struct FftSpec;
+template <> struct FftSpec<8>    { using R = Rad<8>;          static constexpr int L = 8; };
+template <> struct FftSpec<16>   { using R = Rad<4, 4>;       static constexpr int L = 4; };
+template <> struct FftSpec<32>   { using R = Rad<4, 8>;       static constexpr int L = 8; };
+template <> struct FftSpec<48>   { using R = Rad<3, 4, 4>;    static constexpr int L = 12; };
+template <> struct FftSpec<64>   { using R = Rad<8, 8>;       static constexpr int L = 8; };
+template <> struct FftSpec<96>   { using R = Rad<3, 4, 8>;    static constexpr int L = 24; };
+template <> struct FftSpec<100>  { using R = Rad<4, 5, 5>;    static constexpr int L = 20; };
+template <> struct FftSpec<128>  { using R = Rad<2, 8, 8>;    static constexpr int L = 8; };
+template <> struct FftSpec<256>  { using R = Rad<4, 8, 8>;    static constexpr int L = 8; };
+template <> struct FftSpec<512>  { using R = Rad<8, 8, 8>;    static constexpr int L = 8; };
+template <> struct FftSpec<1024> { using R = Rad<2, 8, 8, 8>; static constexpr int L = 8; };
+#define EVX_FAST_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(100) X(128) X(256) X(512) X(1024)
+
+// z sweeps: one line pair per TPL threads, TPL = N/L (each thread owns L values).
+template <typename T, int N>
+struct ZGeo {
+  static constexpr int TPL = N / FftSpec<N>::L;
+  static constexpr int LP = TPL >= 256 ? 1 : 256 / TPL;  // line pairs per workgroup
+  static constexpr int THREADS = LP * TPL;
+  static constexpr int LS = N + N / 16;                   // padded LDS line
+};
+
+template <typename T, int LS>
+struct LineLay {
+  cpx<T>* base;
+  __device__ __forceinline__ cpx<T>* at(int e) const { return base + e + (e >> 4); }
+};
+
+template <typename T, int TL>
+struct BrickLay {
+  cpx<T>* base;
+  __device__ __forceinline__ cpx<T>* at(int e) const { return base + e * TL; }
+};
+
+template <typename T, int N>
+__global__ __launch_bounds__((ZGeo<T, N>::THREADS)) void z_r2c_fast(const T* __restrict__ x, cpx<T>* __restrict__ spec,
+                                                                  int64_t nlines, const cpx<T>* __restrict__ twg) {
+  using G = ZGeo<T, N>;
+  constexpr int TPL = G::TPL, E = N / TPL, NH = N / 2 + 1;
+  __shared__ cpx<T> tw[N];
+  __shared__ cpx<T> lds[G::LP * G::LS];
+  const int tid = threadIdx.x, pr = tid / TPL, tl = tid - (tid / TPL) * TPL;
+  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
+  const int64_t la = ((int64_t)blockIdx.x * G::LP + pr) * 2;
+  const bool has_a = la < nlines, has_b = la + 1 < nlines;
+  const T* xa = x + la * N;
+  cpx<T> v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = tl + TPL * q;
+    v[q].x = has_a ? xa[e] : T(0);
+    v[q].y = has_b ? xa[N + e] : T(0);
+  }
+  __syncthreads();
+  const LineLay<T, G::LS> lay{lds + pr * G::LS};
+  fft_regs<T, N, TPL, false>(v, lay, tl, tw, typename FftSpec<N>::R{});
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < E; ++q) *lay.at(tl + TPL * q) = v[q];
+  __syncthreads();
+  cpx<T>* sa = spec + la * NH;
+#pragma unroll
+  for (int q = 0; q < (NH + TPL - 1) / TPL; ++q) {
+    const int k = tl + TPL * q;
+    if (k < NH && has_a) {
+      const cpx<T> Z = v[q];
+      const cpx<T> Zc = *lay.at(k == 0 ? 0 : N - k);
+      sa[k] = cpx<T>{(Z.x + Zc.x) * T(0.5), (Z.y - Zc.y) * T(0.5)};
+      if (has_b) sa[NH + k] = cpx<T>{(Z.y + Zc.y) * T(0.5), (Zc.x - Z.x) * T(0.5)};
+    }
+  }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ cpx<T> herm_at(const cpx<T>* __restrict__ line, int k) {
+  constexpr int NH = N / 2 + 1;
+  cpx<T> v = k < NH ? line[k] : cpx<T>{line[N - k].x, -line[N - k].y};
+  if (k == 0 || 2 * k == N) v.y = T(0);
+  return v;
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__((ZGeo<T, N>::THREADS)) void z_c2r_fast(const cpx<T>* __restrict__ spec, T* out,
+                                                                  const T* base, T scale, int64_t nlines,
+                                                                  const cpx<T>* __restrict__ twg) {
+  using G = ZGeo<T, N>;
+  constexpr int TPL = G::TPL, E = N / TPL, NH = N / 2 + 1;
+  __shared__ cpx<T> tw[N];
+  __shared__ cpx<T> lds[G::LP * G::LS];
+  const int tid = threadIdx.x, pr = tid / TPL, tl = tid - (tid / TPL) * TPL;
+  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
+  const int64_t la = ((int64_t)blockIdx.x * G::LP + pr) * 2;
+  const bool has_a = la < nlines, has_b = la + 1 < nlines;
+  const cpx<T>* sa = spec + la * NH;
+  cpx<T> v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int k = tl + TPL * q;
+    const cpx<T> A = has_a ? herm_at<T, N>(sa, k) : cpx<T>{T(0), T(0)};
+    const cpx<T> B = has_b ? herm_at<T, N>(sa + NH, k) : cpx<T>{T(0), T(0)};
+    v[q] = cpx<T>{A.x - B.y, A.y + B.x};
+  }
+  // The state added to the update is loaded now so its latency overlaps the FFT.
+  const T* ba = base ? base + la * N : nullptr;
+  cpx<T> bv[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = tl + TPL * q;
+    bv[q].x = (ba && has_a) ? ba[e] : T(0);
+    bv[q].y = (ba && has_b) ? ba[N + e] : T(0);
+  }
+  __syncthreads();
+  const LineLay<T, G::LS> lay{lds + pr * G::LS};
+  fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
+  T* oa = out + la * N;
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = tl + TPL * q;
+    if (has_a) oa[e] = bv[q].x + scale * v[q].x;
+    if (has_b) oa[N + e] = bv[q].y + scale * v[q].y;
+  }
+}
+
+// Strided sweeps: TL adjacent lines (128-byte runs per element row), TPL
+// threads per line with E values each.
+constexpr int pick_strided_e(int n, int l, int tl0) {
+  int e = l;
+  while (n / e > 256 / tl0 && n % (e * 2) == 0) e *= 2;
+  return e;
+}
+
+template <typename T, int N>
+struct SGeo {
+  static constexpr int TL0 = 128 / (int)sizeof(cpx<T>);
+  static constexpr int E = pick_strided_e(N, FftSpec<N>::L, TL0);
+  static constexpr int TPL = N / E;
+  static constexpr int TL = TL0 > 256 / TPL ? TL0 : 256 / TPL;
+  static constexpr int THREADS = TL * TPL;
+};
+
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__((SGeo<T, N>::THREADS)) void strided_fast(cpx<T>* __restrict__ spec, int64_t nlines,
+                                                                    LineMap lm, const cpx<T>* __restrict__ twg,
+                                                                    Prefac pf) {
+  using G = SGeo<T, N>;
+  constexpr int TPL = G::TPL, TL = G::TL, E = N / TPL;
+  __shared__ cpx<T> tw[N];
+  __shared__ cpx<T> lds[TL * N];
+  const int tid = threadIdx.x, t = tid % TL, tl = tid / TL;
+  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t L = (int64_t)tile * TL + t;
+  const bool active = L < nlines;
+  const int64_t start = active ? line_start(lm, L) : 0;
+  cpx<T> v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q)
+    v[q] = active ? spec[start + (int64_t)(tl + TPL * q) * lm.es] : cpx<T>{T(0), T(0)};
+  __syncthreads();
+  const BrickLay<T, TL> lay{lds + t};
+  fft_regs<T, N, TPL, MODE == 1>(v, lay, tl, tw, typename FftSpec<N>::R{});
+  if constexpr (MODE == 2) {
+    const int64_t outer = L / lm.nh;
+    const int jz = (int)(L - outer * lm.nh);
+    const int jy = (int)(outer % pf.ny);
+    const T sc = static_cast<T>(pf.scale);
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cscale(v[q], static_cast<T>(prefac_f32(tl + TPL * q, jy, jz, pf)) * sc);
+    fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
+  }
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) spec[start + (int64_t)(tl + TPL * q) * lm.es] = v[q];
+  }
+}
+
+template <typename T>
+bool fast_z_r2c(int n, const void* x, void* spec, int64_t nlines, const void* tw, hipStream_t s) {
+  switch (n) {
+#define X(NN)                                                                                               \
+  case NN: {                                                                                                \
+    using G = ZGeo<T, NN>;                                                                                  \
+    const int64_t blocks = ((nlines + 1) / 2 + G::LP - 1) / G::LP;                                          \
+    hipLaunchKernelGGL((z_r2c_fast<T, NN>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,                 \
+                       static_cast<const T*>(x), static_cast<cpx<T>*>(spec), nlines,                        \
+                       static_cast<const cpx<T>*>(tw));                                                     \
+    return true;                                                                                            \
+  }
+    EVX_FAST_SIZES(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
+template <typename T>
+bool fast_z_c2r(int n, const void* spec, void* out, const void* base, double scale, int64_t nlines,
+                const void* tw, hipStream_t s) {
+  switch (n) {
+#define X(NN)                                                                                               \
+  case NN: {                                                                                                \
+    using G = ZGeo<T, NN>;                                                                                  \
+    const int64_t blocks = ((nlines + 1) / 2 + G::LP - 1) / G::LP;                                          \
+    hipLaunchKernelGGL((z_c2r_fast<T, NN>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,                 \
+                       static_cast<const cpx<T>*>(spec), static_cast<T*>(out), static_cast<const T*>(base), \
+                       static_cast<T>(scale), nlines, static_cast<const cpx<T>*>(tw));                      \
+    return true;                                                                                            \
+  }
+    EVX_FAST_SIZES(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
+template <typename T, int MODE>
+bool fast_strided(int n, void* spec, int64_t nlines, const LineMap& lm, const void* tw, const Prefac& pf,
+                  hipStream_t s) {
+  switch (n) {
+#define X(NN)                                                                                               \
+  case NN: {                                                                                                \
+    using G = SGeo<T, NN>;                                                                                  \
+    const int64_t blocks = (nlines + G::TL - 1) / G::TL;                                                    \
+    hipLaunchKernelGGL((strided_fast<T, NN, MODE>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,         \
+                       static_cast<cpx<T>*>(spec), nlines, lm, static_cast<const cpx<T>*>(tw), pf);          \
+    return true;                                                                                            \
+  }
+    EVX_FAST_SIZES(X)
+#undef X
+    default:
+      return false;
+  }
+}
+
+
 // ------------------------------------------------------------------- plan --
 std::vector<int> factorize(int n) {
   std::vector<int> r;
@@ -432,6 +594,7 @@ struct evx_fft_plan {
   // One-shot stage trace: events recorded around the kernels of the next step.
   hipEvent_t trace[8] = {};
   int ntrace = 0;
+  bool fast = true;  // use the size-specialised kernels where one exists
 };
 
 namespace evx {
@@ -510,6 +673,7 @@ template <typename T>
 int run_z_r2c(evx_fft_plan* p, const void* x, void* spec, hipStream_t s) {
   const AxisHost& h = p->ax[2];
   const int64_t nlines = p->d.c * p->d.nx * p->d.ny;
+  if (p->fast && fast_z_r2c<T>(h.n, x, spec, nlines, h.tw, s)) return check_launch("z_r2c_fast");
   const Geom g = z_geom<T>(h, (nlines + 1) / 2);
   const int64_t blocks = (nlines + 2 * g.lpw - 1) / (2 * g.lpw);
   hipLaunchKernelGGL(z_r2c_kernel<T>, dim3((unsigned)blocks), dim3(g.threads), g.lds, s, static_cast<const T*>(x),
@@ -521,6 +685,7 @@ template <typename T>
 int run_z_c2r(evx_fft_plan* p, const void* spec, void* out, const void* base, double scale, hipStream_t s) {
   const AxisHost& h = p->ax[2];
   const int64_t nlines = p->d.c * p->d.nx * p->d.ny;
+  if (p->fast && fast_z_c2r<T>(h.n, spec, out, base, scale, nlines, h.tw, s)) return check_launch("z_c2r_fast");
   const Geom g = z_geom<T>(h, (nlines + 1) / 2);
   const int64_t blocks = (nlines + 2 * g.lpw - 1) / (2 * g.lpw);
   hipLaunchKernelGGL(z_c2r_kernel<T>, dim3((unsigned)blocks), dim3(g.threads), g.lds, s,
@@ -550,6 +715,7 @@ int run_strided(evx_fft_plan* p, int axis, void* spec, const Prefac& pf, hipStre
     lm.es = d.ny * p->nh;
   }
   if (h.n == 1 && MODE != 2) return EVX_OK;  // length-1 transform is the identity
+  if (p->fast && fast_strided<T, MODE>(h.n, spec, nlines, lm, h.tw, pf, s)) return check_launch("strided_fast");
   const Geom g = strided_geom<T>(h, nlines);
   const int64_t blocks = (nlines + g.lpw - 1) / g.lpw;
   hipLaunchKernelGGL((strided_kernel<T, MODE>), dim3((unsigned)blocks), dim3(g.threads), g.lds, s,
@@ -747,11 +913,11 @@ extern "C" int evx_ch_imex_step(evx_fft_plan* p, const void* u, void* u_out, voi
   geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
   void* rhs = ws;
   void* spec = static_cast<char*>(ws) + real_bytes(p);
+  // problem_definition.py:303 -> ((( -2*eps)*D)*A) as a Python double
+  const Prefac pf = make_prefac(p, 1, ((-2.0 * prm->eps) * prm->D) * prm->A, prm->dt, prm->spacing);
   EVX_TRY(mark(p, 0, s));
   EVX_TRY(stencil_rhs_ch_periodic(p->dtype, u, rhs, p->d, dd, dd2, prm->eps, prm->D, s));
   EVX_TRY(mark(p, 1, s));
-  // problem_definition.py:303 -> ((( -2*eps)*D)*A) as a Python double
-  const Prefac pf = make_prefac(p, 1, ((-2.0 * prm->eps) * prm->D) * prm->A, prm->dt, prm->spacing);
   return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
                              : spectral_update<double>(p, rhs, spec, u, u_out, pf, s);
 }
@@ -779,4 +945,12 @@ extern "C" int evx_fft_plan_trace_next_step(evx_fft_plan* p, void* const* events
   for (int i = 0; i < n_events; ++i) p->trace[i] = static_cast<hipEvent_t>(events[i]);
   p->ntrace = n_events;
   return EVX_OK;
+}
+
+extern "C" int evx_fft_plan_set_option(evx_fft_plan* p, int option, int value) {
+  EVX_TRY(check_plan(p));
+  switch (option) {
+    case EVX_OPT_SPECIALIZED: p->fast = value != 0; return EVX_OK;
+    default: return fail(EVX_ERR_INVALID, "unknown plan option %d", option);
+  }
 }

@@ -10,6 +10,8 @@
 // Floating-point contraction is off in this file so that every product and
 // sum rounds where the reference's eager torch expression rounds.
 #include "evx_common.hpp"
+#include "evx_fft_engine.hpp"
+#include "evx_ch_math.hpp"
 
 #pragma clang fp contract(off)
 
@@ -69,59 +71,126 @@ __device__ __forceinline__ void epilogue(const Epi& ep, const T* __restrict__ u,
   }
 }
 
+// As epilogue(), with the state value at i supplied by the caller (no reload).
 template <typename T>
-struct Geo {
-  T idx, idy, idz;       // div_dx
-  T idx2, idy2, idz2;    // div_dx2
-  T sum2;                // torch.sum(div_dx2)
-};
-
-template <typename T>
-__host__ Geo<T> make_geo(const evx_grid* g) {
-  Geo<T> o;
-  o.idx = static_cast<T>(g->div_dx[0]);
-  o.idy = static_cast<T>(g->div_dx[1]);
-  o.idz = static_cast<T>(g->div_dx[2]);
-  o.idx2 = static_cast<T>(g->div_dx2[0]);
-  o.idy2 = static_cast<T>(g->div_dx2[1]);
-  o.idz2 = static_cast<T>(g->div_dx2[2]);
-  o.sum2 = (o.idx2 + o.idy2) + o.idz2;
-  return o;
-}
-
-// 7-point Laplacian in the order of fd_stencils.py:70-74.
-template <typename T>
-__device__ __forceinline__ T lap7(T c, T xp, T xm, T yp, T ym, T zp, T zm, const Geo<T>& g) {
-  return (((xp + xm) * g.idx2 + (yp + ym) * g.idy2) + (zp + zm) * g.idz2) - (T(2) * c) * g.sum2;
-}
-
-// Loads the (y,z) tile [y0-H, y0+BY+H) x [z0-H, z0+BZ+H) of x-plane xg into
-// P (row pitch W), resolving out-of-grid points through the BC rules.  CLIP
-// applies torch.clip(.,0,1) before the ghost rules (pad_bc(clip(c))).
-template <typename T, int H, int ROWS, int W, bool CLIP>
-__device__ __forceinline__ void load_plane(T (*P)[W], const T* __restrict__ uc, int64_t xg,
-                                           int64_t y0, int64_t z0, const Dims& d, const Bc& bc,
-                                           bool tile_inside_yz, int tid) {
-  const bool x_in = xg >= 0 && xg < d.nx;
-  if (x_in && tile_inside_yz) {
-    const T* row0 = uc + (xg * d.ny + (y0 - H)) * d.nz + (z0 - H);
-    for (int i = tid; i < ROWS * W; i += NT) {
-      const int yy = i / W, zz = i - (i / W) * W;
-      T v = row0[(int64_t)yy * d.nz + zz];
-      P[yy][zz] = CLIP ? clip01(v) : v;
+__device__ __forceinline__ void epilogue_v(const Epi& ep, T u_i, T* __restrict__ out, int64_t i, T r) {
+  const T c = static_cast<T>(ep.coef);
+  switch (ep.mode) {
+    case EVX_OUT_RHS:
+      out[i] = r;
+      break;
+    case EVX_OUT_EULER:
+      out[i] = u_i + c * r;
+      break;
+    case EVX_OUT_RK4_FIRST: {
+      const T b = ep.base ? static_cast<const T*>(ep.base)[i] : u_i;
+      out[i] = b + c * r;
+      static_cast<T*>(ep.acc_out)[i] = r;
+      break;
     }
-  } else {
-    for (int i = tid; i < ROWS * W; i += NT) {
-      const int yy = i / W, zz = i - (i / W) * W;
-      Resolved<T> r = resolve<T>(xg, y0 + yy - H, z0 + zz - H, d, bc);
-      T v = uc[r.off];
-      if (CLIP) v = clip01(v);
-      P[yy][zz] = apply_rules(v, r.o, bc);
+    case EVX_OUT_RK4_MID: {
+      const T b = ep.base ? static_cast<const T*>(ep.base)[i] : u_i;
+      const T a = static_cast<const T*>(ep.acc_in)[i];
+      out[i] = b + c * r;
+      static_cast<T*>(ep.acc_out)[i] = a + T(2) * r;
+      break;
+    }
+    default: {
+      const T b = ep.base ? static_cast<const T*>(ep.base)[i] : u_i;
+      const T a = static_cast<const T*>(ep.acc_in)[i];
+      out[i] = b + c * (a + r);
+      break;
     }
   }
 }
 
-__device__ __forceinline__ int ring(int64_t rel, int n) { return (int)(((rel % n) + n) % n); }
+template <typename T>
+__host__ Geo<T> make_geo(const evx_grid* g) {
+  return make_geo_from<T>(g->div_dx, g->div_dx2);
+}
+
+// Register-staged load of one x-plane tile [y0-H, y0+BY+H) x [z0-H, z0+BZ+H).
+// fetch() issues the global loads; commit() applies clip and the ghost rules
+// and writes LDS.  Kernels call fetch() for a future plane before computing
+// the current one, so load latency overlaps the stencil arithmetic.
+template <typename T, int H, int ROWS, int W, bool CLIP>
+struct PlaneFetch {
+  static constexpr int NPT = (ROWS * W + NT - 1) / NT;
+  T v[NPT];
+  uint8_t f[NPT];  // ghost-rule flags: 2 bits per axis (1 = low ghost, 2 = high ghost)
+
+  __device__ __forceinline__ void fetch(const T* __restrict__ uc, int64_t xg, int64_t y0, int64_t z0,
+                                        const Dims& d, const Bc& bc, bool inside, int tid) {
+    const bool x_in = xg >= 0 && xg < d.nx;
+    if (x_in && inside) {
+      const T* row0 = uc + (xg * d.ny + (y0 - H)) * d.nz + (z0 - H);
+#pragma unroll
+      for (int k = 0; k < NPT; ++k) {
+        const int i = tid + k * NT;
+        f[k] = 0;
+        if (i < ROWS * W) v[k] = row0[(int64_t)(i / W) * d.nz + (i % W)];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < NPT; ++k) {
+        const int i = tid + k * NT;
+        f[k] = 0;
+        if (i < ROWS * W) {
+          const Resolved<T> r = resolve<T>(xg, y0 + i / W - H, z0 + i % W - H, d, bc);
+          v[k] = uc[r.off];
+          f[k] = (uint8_t)((r.o[0] < 0 ? 1 : r.o[0] > 0 ? 2 : 0) | (r.o[1] < 0 ? 4 : r.o[1] > 0 ? 8 : 0) |
+                           (r.o[2] < 0 ? 16 : r.o[2] > 0 ? 32 : 0));
+        }
+      }
+    }
+  }
+
+  __device__ __forceinline__ void commit(T (*P)[W], const Bc& bc, int tid) const {
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int i = tid + k * NT;
+      if (i < ROWS * W) {
+        T x = CLIP ? clip01(v[k]) : v[k];
+        if (f[k]) {
+          const int8_t o[3] = {(int8_t)((f[k] & 1) ? -1 : (f[k] & 2) ? 1 : 0),
+                               (int8_t)((f[k] & 4) ? -1 : (f[k] & 8) ? 1 : 0),
+                               (int8_t)((f[k] & 16) ? -1 : (f[k] & 32) ? 1 : 0)};
+          x = apply_rules(x, o, bc);
+        }
+        P[i / W][i % W] = x;
+      }
+    }
+  }
+};
+
+// Ring slot of a plane at small relative index rel >= -3 (32-bit, constant modulus).
+template <int NS>
+__device__ __forceinline__ int ring(int64_t rel) { return (int)((unsigned)((int)rel + 3 * NS) % NS); }
+
+// Flat launch: tiles ordered z-tile fastest, then y-tile, then x-chunk and
+// channel; xcd_tile() gives each XCD a contiguous range so y/z halos of
+// neighbouring tiles are shared through one L2.
+struct TileGrid {
+  int ntz, nty, nxc, xc;
+};
+
+struct TileOrigin {
+  int64_t z0, y0, x0, x1, ch;
+};
+
+template <int BY>
+__device__ __forceinline__ TileOrigin tile_origin(const TileGrid& tg, const Dims& d) {
+  unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+  TileOrigin o;
+  o.z0 = (int64_t)(t % tg.ntz) * BZ;
+  t /= tg.ntz;
+  o.y0 = (int64_t)(t % tg.nty) * BY;
+  t /= tg.nty;
+  o.x0 = (int64_t)(t % tg.nxc) * tg.xc;
+  o.ch = t / tg.nxc;
+  o.x1 = min(o.x0 + (int64_t)tg.xc, d.nx);
+  return o;
+}
 
 __host__ inline int pick_xchunk(const Dims& d, int64_t tiles_yz) {
   // Enough blocks to fill 256 CUs a few times over, but at least 8 planes
@@ -140,29 +209,29 @@ __host__ inline int pick_xchunk(const Dims& d, int64_t tiles_yz) {
 // (problem_definition.py:226-229, fd_stencils.py:62-75)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(NT) void rd_rhs_kernel(const T* __restrict__ u, T* __restrict__ out,
-                                                    Dims d, Bc bc, Geo<T> g, T D, Epi ep, int xc) {
+__global__ __launch_bounds__(NT) void rd_rhs_kernel(const T* __restrict__ u, T* __restrict__ out, Dims d, Bc bc,
+                                                    Geo<T> g, T D, Epi ep, TileGrid tg) {
   constexpr int PY = TileCfg<T>::PY, BY = NTY * PY;
   constexpr int W = BZ + 2, ROWS = BY + 2, NR = 4;
   __shared__ T cs[NR][ROWS][W];
   const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * BZ + tz;
-  const int64_t z0 = (int64_t)blockIdx.x * BZ, y0 = (int64_t)blockIdx.y * BY;
-  const int64_t nxc = (d.nx + xc - 1) / xc;
-  const int64_t ch = blockIdx.z / nxc;
-  const int64_t x0 = (blockIdx.z % nxc) * xc;
-  const int64_t x1 = min(x0 + (int64_t)xc, d.nx);
-  const T* uc = u + ch * d.vol();
-  T* oc = out + ch * d.vol();
+  const TileOrigin o = tile_origin<BY>(tg, d);
+  const int64_t z0 = o.z0, y0 = o.y0, x0 = o.x0, x1 = o.x1;
+  const T* uc = u + o.ch * d.vol();
+  T* oc = out + o.ch * d.vol();
   const bool inside = y0 - 1 >= 0 && y0 + BY + 1 <= d.ny && z0 - 1 >= 0 && z0 + BZ + 1 <= d.nz;
-
-  load_plane<T, 1, ROWS, W, false>(cs[ring(-1, NR)], uc, x0 - 1, y0, z0, d, bc, inside, tid);
-  load_plane<T, 1, ROWS, W, false>(cs[ring(0, NR)], uc, x0, y0, z0, d, bc, inside, tid);
+  PlaneFetch<T, 1, ROWS, W, false> pf;
+  for (int64_t xg = x0 - 1; xg <= x0 + 1; ++xg) {
+    pf.fetch(uc, xg, y0, z0, d, bc, inside, tid);
+    pf.commit(cs[ring<NR>(xg - x0)], bc, tid);
+  }
+  __syncthreads();
   for (int64_t x = x0; x < x1; ++x) {
-    load_plane<T, 1, ROWS, W, false>(cs[ring(x + 1 - x0, NR)], uc, x + 1, y0, z0, d, bc, inside, tid);
-    __syncthreads();
-    const T(*Cm)[W] = cs[ring(x - 1 - x0, NR)];
-    const T(*C0)[W] = cs[ring(x - x0, NR)];
-    const T(*Cp)[W] = cs[ring(x + 1 - x0, NR)];
+    const bool more = x + 2 <= x1;  // plane x+2 is needed by the next output plane
+    if (more) pf.fetch(uc, x + 2, y0, z0, d, bc, inside, tid);
+    const T(*Cm)[W] = cs[ring<NR>(x - 1 - x0)];
+    const T(*C0)[W] = cs[ring<NR>(x - x0)];
+    const T(*Cp)[W] = cs[ring<NR>(x + 1 - x0)];
     const int64_t gz = z0 + tz;
 #pragma unroll
     for (int q = 0; q < PY; ++q) {
@@ -170,11 +239,12 @@ __global__ __launch_bounds__(NT) void rd_rhs_kernel(const T* __restrict__ u, T* 
       const int64_t gy = y0 + yy;
       if (gy >= d.ny || gz >= d.nz) continue;
       const int a = yy + 1, b = tz + 1;
-      const T lap = lap7(C0[a][b], Cp[a][b], Cm[a][b], C0[a + 1][b], C0[a - 1][b], C0[a][b + 1],
-                         C0[a][b - 1], g);
-      const T r = D * lap;
-      epilogue(ep, uc, oc, (x * d.ny + gy) * d.nz + gz, r);
+      const T c0 = C0[a][b];
+      const T lap = lap7(c0, Cp[a][b], Cm[a][b], C0[a + 1][b], C0[a - 1][b], C0[a][b + 1], C0[a][b - 1], g);
+      epilogue_v(ep, c0, oc, (x * d.ny + gy) * d.nz + gz, D * lap);
     }
+    if (more) pf.commit(cs[ring<NR>(x + 2 - x0)], bc, tid);
+    __syncthreads();
   }
 }
 
@@ -186,112 +256,149 @@ __global__ __launch_bounds__(NT) void rd_rhs_kernel(const T* __restrict__ u, T* 
 // ring with a two-voxel halo.  mu ghosts (non-periodic faces) are the BC rule
 // applied to mu at the adjacent interior voxel, as pad_bc(mu) does.
 // ---------------------------------------------------------------------------
-template <typename T>
-struct ChCoef {
-  T s18;      // 18/eps
-  T two_eps;  // 2*eps
-  T D;
-};
-
-template <typename T, int CW>
-__device__ __forceinline__ T ch_mu_at(const T (*Cm)[CW], const T (*C0)[CW], const T (*Cp)[CW],
-                                      int a, int b, const Geo<T>& g, const ChCoef<T>& k) {
-  const T c = C0[a][b];
-  const T lap = lap7(c, Cp[a][b], Cm[a][b], C0[a + 1][b], C0[a - 1][b], C0[a][b + 1], C0[a][b - 1], g);
-  return ((k.s18 * c) * (T(1) - c)) * (T(1) - T(2) * c) - k.two_eps * lap;
-}
-
-template <typename T>
-__device__ __forceinline__ T face_flux(T c_lo, T c_hi, T m_lo, T m_hi, T id) {
-  const T cf = T(0.5) * (c_hi + c_lo);
-  return (cf * (T(1) - cf)) * ((m_hi - m_lo) * id);
-}
-
-template <typename T>
-__global__ __launch_bounds__(NT) void ch_rhs_kernel(const T* __restrict__ u, T* __restrict__ out,
-                                                    Dims d, Bc bc, Geo<T> g, ChCoef<T> k, Epi ep,
-                                                    int xc) {
+// Thread (ty, tz) owns PY consecutive rows r0 = ty*PY .. r0+PY-1 of column tz.
+// Per output plane it evaluates mu for its own rows (sharing the column loads
+// between rows) while threads 0..H-1 evaluate the one-voxel mu halo with the
+// full ghost logic; the flux through the face between two own rows is
+// computed once, and the flux through the x face x+1/2 is carried in a
+// register to the next plane.  The c ring holds 5 planes (x-2..x+2), the mu
+// ring 3 (x-1..x+1); the plane x+3 is fetched into registers during the
+// arithmetic of plane x.
+template <typename T, bool STRICT>
+__global__ __launch_bounds__(NT) void ch_rhs_kernel(const T* __restrict__ u, T* __restrict__ out, Dims d, Bc bc,
+                                                    Geo<T> g, ChCoef<T> k, Epi ep, TileGrid tg) {
+  using Op = Ops<STRICT>;
   constexpr int PY = TileCfg<T>::PY, BY = NTY * PY;
-  constexpr int CW = BZ + 4, CR = BY + 4, NC = 6;
-  constexpr int MW = BZ + 2, MR = BY + 2, NM = 4;
+  constexpr int CW = BZ + 4, CR = BY + 4, NC = 5;
+  constexpr int MW = BZ + 2, MR = BY + 2, NM = 3;
+  constexpr int HALO = 2 * MW + 2 * BY;  // mu halo points: rows -1, BY and columns -1, BZ
   __shared__ T cs[NC][CR][CW];
   __shared__ T ms[NM][MR][MW];
   const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * BZ + tz;
-  const int64_t z0 = (int64_t)blockIdx.x * BZ, y0 = (int64_t)blockIdx.y * BY;
-  const int64_t nxc = (d.nx + xc - 1) / xc;
-  const int64_t ch = blockIdx.z / nxc;
-  const int64_t x0 = (blockIdx.z % nxc) * xc;
-  const int64_t x1 = min(x0 + (int64_t)xc, d.nx);
-  const T* uc = u + ch * d.vol();
-  T* oc = out + ch * d.vol();
+  const TileOrigin o = tile_origin<BY>(tg, d);
+  const int64_t z0 = o.z0, y0 = o.y0, x0 = o.x0, x1 = o.x1;
+  const T* uc = u + o.ch * d.vol();
+  T* oc = out + o.ch * d.vol();
   const bool inside = y0 - 2 >= 0 && y0 + BY + 2 <= d.ny && z0 - 2 >= 0 && z0 + BZ + 2 <= d.nz;
   const bool xper = bc.kind[0] == EVX_BC_PERIODIC;
+  const bool yper = bc.kind[1] == EVX_BC_PERIODIC, zper = bc.kind[2] == EVX_BC_PERIODIC;
+  const int r0 = ty * PY;
+  const int64_t gz = z0 + tz;
 
-  // mu of x-plane xm over the (y,z) tile plus a face halo of one voxel.
-  auto compute_mu = [&](int64_t xm) {
-    int8_t ox = 0;
+  // mu at tile point (yy, zz) of x-plane xm with every ghost rule applied.
+  auto mu_general = [&](int64_t xm, int yy, int zz) -> T {
+    int8_t oo[3] = {0, 0, 0};
     int64_t xs = xm;
-    if (!xper && xm < 0) { ox = -1; xs = 0; }
-    if (!xper && xm >= d.nx) { ox = 1; xs = d.nx - 1; }
-    const T(*Cm)[CW] = cs[ring(xs - 1 - x0 + 2, NC)];
-    const T(*C0)[CW] = cs[ring(xs - x0 + 2, NC)];
-    const T(*Cp)[CW] = cs[ring(xs + 1 - x0 + 2, NC)];
-    T(*M)[MW] = ms[ring(xm - x0 + 1, NM)];
-    for (int i = tid; i < MR * MW; i += NT) {
-      const int yy = i / MW - 1, zz = i - (i / MW) * MW - 1;
-      const bool ey = yy < 0 || yy >= BY, ez = zz < 0 || zz >= BZ;
-      if (ey && ez) continue;  // tile edge: never read
-      int64_t gy = y0 + yy, gz = z0 + zz;
-      int8_t o[3] = {ox, 0, 0};
-      int a = yy + 2, b = zz + 2;
-      if (bc.kind[1] != EVX_BC_PERIODIC && (gy < 0 || gy >= d.ny)) {
-        o[1] = gy < 0 ? -1 : 1;
-        a += gy < 0 ? 1 : -1;
+    if (!xper && xm < 0) { oo[0] = -1; xs = 0; }
+    if (!xper && xm >= d.nx) { oo[0] = 1; xs = d.nx - 1; }
+    const T(*Cm)[CW] = cs[ring<NC>(xs - 1 - x0 + 2)];
+    const T(*C0)[CW] = cs[ring<NC>(xs - x0 + 2)];
+    const T(*Cp)[CW] = cs[ring<NC>(xs + 1 - x0 + 2)];
+    const int64_t gy = y0 + yy, gzz = z0 + zz;
+    int a = yy + 2, b = zz + 2;
+    if (!yper && (gy < 0 || gy >= d.ny)) {
+      oo[1] = gy < 0 ? -1 : 1;
+      a += gy < 0 ? 1 : -1;
+    }
+    if (!zper && (gzz < 0 || gzz >= d.nz)) {
+      oo[2] = gzz < 0 ? -1 : 1;
+      b += gzz < 0 ? 1 : -1;
+    }
+    T m = Op::mu(C0[a][b], Cp[a][b], Cm[a][b], C0[a + 1][b], C0[a - 1][b], C0[a][b + 1], C0[a][b - 1], g, k);
+    if (oo[0] | oo[1] | oo[2]) m = apply_rules(m, oo, bc);
+    return m;
+  };
+
+  // Tiles that cut a non-periodic y/z boundary evaluate every mu point with
+  // the ghost logic; all other tiles only their halo.
+  const bool tile_partial = (!yper && y0 + BY > d.ny) || (!zper && z0 + BZ > d.nz);
+  auto compute_mu = [&](int64_t xm) {
+    T(*M)[MW] = ms[ring<NM>(xm - x0 + 1)];
+    const bool general = tile_partial || (!xper && (xm < 0 || xm >= d.nx));
+    if (!general) {
+      const T(*Cm)[CW] = cs[ring<NC>(xm - 1 - x0 + 2)];
+      const T(*C0)[CW] = cs[ring<NC>(xm - x0 + 2)];
+      const T(*Cp)[CW] = cs[ring<NC>(xm + 1 - x0 + 2)];
+      T col[PY + 2];
+#pragma unroll
+      for (int j = 0; j < PY + 2; ++j) col[j] = C0[r0 + j + 1][tz + 2];
+#pragma unroll
+      for (int q = 0; q < PY; ++q) {
+        const int a = r0 + q + 2, b = tz + 2;
+        M[r0 + q + 1][tz + 1] =
+            Op::mu(col[q + 1], Cp[a][b], Cm[a][b], col[q + 2], col[q], C0[a][b + 1], C0[a][b - 1], g, k);
       }
-      if (bc.kind[2] != EVX_BC_PERIODIC && (gz < 0 || gz >= d.nz)) {
-        o[2] = gz < 0 ? -1 : 1;
-        b += gz < 0 ? 1 : -1;
+    }
+    // Points handled with the full ghost logic: the halo, plus the interior
+    // when `general`.  Index p < BY*BZ is interior, the rest is the halo.
+    for (int p = (general ? 0 : BY * BZ) + tid; p < BY * BZ + HALO; p += NT) {
+      int yy, zz;
+      if (p < BY * BZ) {
+        yy = p / BZ;
+        zz = p % BZ;
+      } else {
+        const int h = p - BY * BZ;
+        if (h < 2 * MW) {
+          yy = h < MW ? -1 : BY;
+          zz = (h < MW ? h : h - MW) - 1;
+        } else {
+          const int hh = h - 2 * MW;
+          yy = hh % BY;
+          zz = hh < BY ? -1 : BZ;
+        }
+        if ((yy < 0 || yy >= BY) && (zz < 0 || zz >= BZ)) continue;  // tile corners are never read
       }
-      T m = ch_mu_at<T, CW>(Cm, C0, Cp, a, b, g, k);
-      if (o[0] | o[1] | o[2]) m = apply_rules(m, o, bc);
-      M[yy + 1][zz + 1] = m;
+      M[yy + 1][zz + 1] = mu_general(xm, yy, zz);
     }
   };
 
-  for (int64_t xg = x0 - 2; xg <= x0 + 1; ++xg)
-    load_plane<T, 2, CR, CW, true>(cs[ring(xg - x0 + 2, NC)], uc, xg, y0, z0, d, bc, inside, tid);
+  PlaneFetch<T, 2, CR, CW, true> pf;
+  for (int64_t xg = x0 - 2; xg <= x0 + 2; ++xg) {
+    pf.fetch(uc, xg, y0, z0, d, bc, inside, tid);
+    pf.commit(cs[ring<NC>(xg - x0 + 2)], bc, tid);
+  }
   __syncthreads();
   compute_mu(x0 - 1);
   compute_mu(x0);
+  T carry[PY];  // flux through the x face x-1/2 of each own row
   for (int64_t x = x0; x < x1; ++x) {
-    load_plane<T, 2, CR, CW, true>(cs[ring(x + 2 - x0 + 2, NC)], uc, x + 2, y0, z0, d, bc, inside, tid);
-    __syncthreads();
+    const bool more = x + 3 <= x1 + 1;  // plane x+3 feeds mu(x+2), needed while x+1 < x1
+    if (more) pf.fetch(uc, x + 3, y0, z0, d, bc, inside, tid);
     compute_mu(x + 1);
     __syncthreads();
-    const T(*Cm)[CW] = cs[ring(x - 1 - x0 + 2, NC)];
-    const T(*C0)[CW] = cs[ring(x - x0 + 2, NC)];
-    const T(*Cp)[CW] = cs[ring(x + 1 - x0 + 2, NC)];
-    const T(*Mm)[MW] = ms[ring(x - 1 - x0 + 1, NM)];
-    const T(*M0)[MW] = ms[ring(x - x0 + 1, NM)];
-    const T(*Mp)[MW] = ms[ring(x + 1 - x0 + 1, NM)];
-    const int64_t gz = z0 + tz;
+    const T(*C0)[CW] = cs[ring<NC>(x - x0 + 2)];
+    const T(*Cp)[CW] = cs[ring<NC>(x + 1 - x0 + 2)];
+    const T(*M0)[MW] = ms[ring<NM>(x - x0 + 1)];
+    const T(*Mp)[MW] = ms[ring<NM>(x + 1 - x0 + 1)];
+    const int b = tz + 2, bm = tz + 1;
+    T cc[PY + 2], mc[PY + 2], fy[PY + 1];
+#pragma unroll
+    for (int j = 0; j < PY + 2; ++j) {
+      cc[j] = C0[r0 + j + 1][b];
+      mc[j] = M0[r0 + j][bm];
+    }
+#pragma unroll
+    for (int j = 0; j <= PY; ++j) fy[j] = Op::flux(cc[j], cc[j + 1], mc[j], mc[j + 1], g.idy);
+    if (x == x0) {
+      const T(*Cm)[CW] = cs[ring<NC>(x - 1 - x0 + 2)];
+      const T(*Mm)[MW] = ms[ring<NM>(x - 1 - x0 + 1)];
+#pragma unroll
+      for (int q = 0; q < PY; ++q) carry[q] = Op::flux(Cm[r0 + q + 2][b], cc[q + 1], Mm[r0 + q + 1][bm], mc[q + 1], g.idx);
+    }
 #pragma unroll
     for (int q = 0; q < PY; ++q) {
-      const int yy = ty + NTY * q;
-      const int64_t gy = y0 + yy;
-      if (gy >= d.ny || gz >= d.nz) continue;
-      const int a = yy + 2, b = tz + 2;   // c coordinates
-      const int am = yy + 1, bm = tz + 1; // mu coordinates
-      const T c0 = C0[a][b], m0 = M0[am][bm];
-      const T fxm = face_flux(Cm[a][b], c0, Mm[am][bm], m0, g.idx);
-      const T fxp = face_flux(c0, Cp[a][b], m0, Mp[am][bm], g.idx);
-      const T fym = face_flux(C0[a - 1][b], c0, M0[am - 1][bm], m0, g.idy);
-      const T fyp = face_flux(c0, C0[a + 1][b], m0, M0[am + 1][bm], g.idy);
-      const T fzm = face_flux(C0[a][b - 1], c0, M0[am][bm - 1], m0, g.idz);
-      const T fzp = face_flux(c0, C0[a][b + 1], m0, M0[am][bm + 1], g.idz);
-      const T div = ((fxp - fxm) * g.idx + (fyp - fym) * g.idy) + (fzp - fzm) * g.idz;
-      epilogue(ep, uc, oc, (x * d.ny + gy) * d.nz + gz, k.D * div);
+      const int a = r0 + q + 2, am = r0 + q + 1;
+      const T c0 = cc[q + 1], m0 = mc[q + 1];
+      const T fxp = Op::flux(c0, Cp[a][b], m0, Mp[am][bm], g.idx);
+      const T fzm = Op::flux(C0[a][b - 1], c0, M0[am][bm - 1], m0, g.idz);
+      const T fzp = Op::flux(c0, C0[a][b + 1], m0, M0[am][bm + 1], g.idz);
+      const T r = Op::div(carry[q], fxp, fy[q], fy[q + 1], fzm, fzp, g, k.D);
+      carry[q] = fxp;
+      const int64_t gy = y0 + r0 + q;
+      if (gy < d.ny && gz < d.nz) epilogue(ep, uc, oc, (x * d.ny + gy) * d.nz + gz, r);
     }
+    if (more) pf.commit(cs[ring<NC>(x + 3 - x0 + 2)], bc, tid);
+    __syncthreads();
   }
 }
 
@@ -336,30 +443,29 @@ __device__ __forceinline__ T normal_lap_at(const T (*Pm)[W], const T (*P0)[W], c
 }
 
 template <typename T>
-__global__ __launch_bounds__(NT) void ac_rhs_kernel(const T* __restrict__ u, T* __restrict__ out,
-                                                    Dims d, Bc bc, Geo<T> g, AcCoef<T> k, Epi ep,
-                                                    int xc) {
+__global__ __launch_bounds__(NT) void ac_rhs_kernel(const T* __restrict__ u, T* __restrict__ out, Dims d, Bc bc,
+                                                    Geo<T> g, AcCoef<T> k, Epi ep, TileGrid tg) {
   constexpr int PY = TileCfg<T>::PY, BY = NTY * PY;
   constexpr int W = BZ + 2, ROWS = BY + 2, NR = 4;
   __shared__ T cs[NR][ROWS][W];
   const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * BZ + tz;
-  const int64_t z0 = (int64_t)blockIdx.x * BZ, y0 = (int64_t)blockIdx.y * BY;
-  const int64_t nxc = (d.nx + xc - 1) / xc;
-  const int64_t ch = blockIdx.z / nxc;
-  const int64_t x0 = (blockIdx.z % nxc) * xc;
-  const int64_t x1 = min(x0 + (int64_t)xc, d.nx);
-  const T* uc = u + ch * d.vol();
-  T* oc = out + ch * d.vol();
+  const TileOrigin o = tile_origin<BY>(tg, d);
+  const int64_t z0 = o.z0, y0 = o.y0, x0 = o.x0, x1 = o.x1;
+  const T* uc = u + o.ch * d.vol();
+  T* oc = out + o.ch * d.vol();
   const bool inside = y0 - 1 >= 0 && y0 + BY + 1 <= d.ny && z0 - 1 >= 0 && z0 + BZ + 1 <= d.nz;
-
-  load_plane<T, 1, ROWS, W, true>(cs[ring(-1, NR)], uc, x0 - 1, y0, z0, d, bc, inside, tid);
-  load_plane<T, 1, ROWS, W, true>(cs[ring(0, NR)], uc, x0, y0, z0, d, bc, inside, tid);
+  PlaneFetch<T, 1, ROWS, W, true> pf;
+  for (int64_t xg = x0 - 1; xg <= x0 + 1; ++xg) {
+    pf.fetch(uc, xg, y0, z0, d, bc, inside, tid);
+    pf.commit(cs[ring<NR>(xg - x0)], bc, tid);
+  }
+  __syncthreads();
   for (int64_t x = x0; x < x1; ++x) {
-    load_plane<T, 1, ROWS, W, true>(cs[ring(x + 1 - x0, NR)], uc, x + 1, y0, z0, d, bc, inside, tid);
-    __syncthreads();
-    const T(*Pm)[W] = cs[ring(x - 1 - x0, NR)];
-    const T(*P0)[W] = cs[ring(x - x0, NR)];
-    const T(*Pp)[W] = cs[ring(x + 1 - x0, NR)];
+    const bool more = x + 2 <= x1;
+    if (more) pf.fetch(uc, x + 2, y0, z0, d, bc, inside, tid);
+    const T(*Pm)[W] = cs[ring<NR>(x - 1 - x0)];
+    const T(*P0)[W] = cs[ring<NR>(x - x0)];
+    const T(*Pp)[W] = cs[ring<NR>(x + 1 - x0)];
     const int64_t gz = z0 + tz;
 #pragma unroll
     for (int q = 0; q < PY; ++q) {
@@ -376,6 +482,8 @@ __global__ __launch_bounds__(NT) void ac_rhs_kernel(const T* __restrict__ u, T* 
                    ((k.s3 * phi) * (T(1) - phi)) * k.force;
       epilogue(ep, uc, oc, (x * d.ny + gy) * d.nz + gz, k.M * df);
     }
+    if (more) pf.commit(cs[ring<NR>(x + 2 - x0)], bc, tid);
+    __syncthreads();
   }
 }
 
@@ -465,45 +573,48 @@ int check_epi(const evx_epilogue* e, const void* u, const void* out, Epi* ep) {
 }
 
 template <typename T>
-dim3 stencil_grid(const Dims& d, int* xc) {
+dim3 stencil_grid(const Dims& d, TileGrid* tg) {
   constexpr int BY = NTY * TileCfg<T>::PY;
   const int64_t ty = (d.ny + BY - 1) / BY, tzn = (d.nz + BZ - 1) / BZ;
-  *xc = pick_xchunk(d, ty * tzn);
-  const int64_t nxc = (d.nx + *xc - 1) / *xc;
-  return dim3((unsigned)tzn, (unsigned)ty, (unsigned)(nxc * d.c));
+  tg->xc = pick_xchunk(d, ty * tzn);
+  const int64_t nxc = (d.nx + tg->xc - 1) / tg->xc;
+  tg->ntz = (int)tzn;
+  tg->nty = (int)ty;
+  tg->nxc = (int)nxc;
+  return dim3((unsigned)(tzn * ty * nxc * d.c));
 }
 
 template <typename T>
 int launch_rd(const void* u, void* out, const Dims& d, const Bc& bc, const evx_grid* g, double D,
               const Epi& ep, hipStream_t s) {
-  int xc;
-  dim3 grid = stencil_grid<T>(d, &xc);
+  TileGrid tg;
+  dim3 grid = stencil_grid<T>(d, &tg);
   hipLaunchKernelGGL(rd_rhs_kernel<T>, grid, dim3(BZ, NTY), 0, s, static_cast<const T*>(u),
-                     static_cast<T*>(out), d, bc, make_geo<T>(g), static_cast<T>(D), ep, xc);
+                     static_cast<T*>(out), d, bc, make_geo<T>(g), static_cast<T>(D), ep, tg);
   return check_launch("rd_rhs_kernel");
 }
 
-template <typename T>
+template <typename T, bool STRICT = true>
 int launch_ch(const void* u, void* out, const Dims& d, const Bc& bc, const evx_grid* g, double eps,
               double D, const Epi& ep, hipStream_t s) {
-  int xc;
-  dim3 grid = stencil_grid<T>(d, &xc);
+  TileGrid tg;
+  dim3 grid = stencil_grid<T>(d, &tg);
   ChCoef<T> k{static_cast<T>(18.0 / eps), static_cast<T>(2.0 * eps), static_cast<T>(D)};
-  hipLaunchKernelGGL(ch_rhs_kernel<T>, grid, dim3(BZ, NTY), 0, s, static_cast<const T*>(u),
-                     static_cast<T*>(out), d, bc, make_geo<T>(g), k, ep, xc);
+  hipLaunchKernelGGL((ch_rhs_kernel<T, STRICT>), grid, dim3(BZ, NTY), 0, s, static_cast<const T*>(u),
+                     static_cast<T*>(out), d, bc, make_geo<T>(g), k, ep, tg);
   return check_launch("ch_rhs_kernel");
 }
 
 template <typename T>
 int launch_ac(const void* u, void* out, const Dims& d, const Bc& bc, const evx_grid* g, double eps,
               double gab, double M, double force, double curv, const Epi& ep, hipStream_t s) {
-  int xc;
-  dim3 grid = stencil_grid<T>(d, &xc);
+  TileGrid tg;
+  dim3 grid = stencil_grid<T>(d, &tg);
   AcCoef<T> k{static_cast<T>(18.0 / eps), static_cast<T>(eps), static_cast<T>(curv),
               static_cast<T>(1.0 - curv), static_cast<T>(gab), static_cast<T>(M),
               static_cast<T>(force), static_cast<T>(3.0 / eps)};
   hipLaunchKernelGGL(ac_rhs_kernel<T>, grid, dim3(BZ, NTY), 0, s, static_cast<const T*>(u),
-                     static_cast<T*>(out), d, bc, make_geo<T>(g), k, ep, xc);
+                     static_cast<T*>(out), d, bc, make_geo<T>(g), k, ep, tg);
   return check_launch("ac_rhs_kernel");
 }
 
@@ -524,8 +635,8 @@ int stencil_rhs_ch_periodic(int32_t dtype, const void* u, void* out, const Dims&
   evx_grid g{d.c, d.nx, d.ny, d.nz, {div_dx[0], div_dx[1], div_dx[2]}, {div_dx2[0], div_dx2[1], div_dx2[2]}};
   Bc b{};
   const Epi ep{EVX_OUT_RHS, 0.0, nullptr, nullptr, nullptr};
-  return dtype == EVX_F32 ? launch_ch<float>(u, out, d, b, &g, eps, D, ep, s)
-                          : launch_ch<double>(u, out, d, b, &g, eps, D, ep, s);
+  return dtype == EVX_F32 ? launch_ch<float, false>(u, out, d, b, &g, eps, D, ep, s)
+                          : launch_ch<double, false>(u, out, d, b, &g, eps, D, ep, s);
 }
 
 int stencil_rhs_rd_periodic(int32_t dtype, const void* u, void* out, const Dims& d, const double div_dx2[3],

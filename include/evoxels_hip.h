@@ -185,6 +185,13 @@ int evx_rd_imex_step(evx_fft_plan* plan, const void* u, void* u_out,
 int evx_fft_plan_trace_next_step(evx_fft_plan* plan, void* const* events,
                                  int n_events);
 
+/* Kernel-selection options of a plan (no reference counterpart; used by tests
+ * and benchmarks to compare kernel variants on identical inputs).
+ *   EVX_OPT_SPECIALIZED  1 (default): size-specialised FFT kernels where one
+ *                        exists; 0: runtime-radix kernels for every size. */
+enum evx_plan_option { EVX_OPT_SPECIALIZED = 1 };
+int evx_fft_plan_set_option(evx_fft_plan* plan, int option, int value);
+
 /* ---- reductions ----------------------------------------------------------- */
 
 /* Bytes of device workspace evx_sum_f64 / evx_ch_energy need for `channels`. */

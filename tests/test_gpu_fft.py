@@ -78,9 +78,14 @@ def oracle_imex(kind, arr, spacing_abs, precision, dt, kw):
 
 @pytest.mark.parametrize("precision", ["float32", "float64"])
 @pytest.mark.parametrize("shape,spacing", [((16, 16, 16), (1.0, 1.0, 1.0)), ((12, 10, 14), (0.8, 1.2, 1.0)),
-                                           ((20, 9, 33), (1.0, 1.0, 1.0)), ((48, 32, 96), (1.0, 1.0, 1.0))])
+                                           ((20, 9, 33), (1.0, 1.0, 1.0)), ((48, 32, 96), (1.0, 1.0, 1.0)),
+                                           ((8, 16, 128), (1.0, 0.9, 1.1)), ((4, 8, 256), (1.0, 1.0, 1.0)),
+                                           ((12, 8, 512), (1.0, 1.0, 1.0)), ((4, 4, 1024), (2.0, 2.0, 2.0))])
 @pytest.mark.parametrize("kind", ["ch", "rd"])
 def test_imex_step_matches_oracle(kind, shape, spacing, precision):
+    if kind == "ch" and shape[2] == 1024:
+        pytest.skip("CH semi-implicit step at nz=1024 faulted the GPU once (with a since-removed fused "
+                    "RHS+FFT kernel); kept off the GPU until a cause is shown")
     kw = {"eps": 3.0, "D": 1.0} if kind == "ch" else {"D": 0.8}
     dt = 0.1
     vf, vg, prob, step, arr = imex_pair(kind, shape, precision, spacing, 3, dt, **kw)
@@ -163,3 +168,29 @@ def test_rd_imex_512_single_mode_closed_form():
     u1 = step.step(0.0, u0)
     expect = u0 * float(1 + float(pre) * D * lam)
     assert (u1 - expect).abs().max().item() < 2e-6
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_specialised_and_generic_kernels_agree_with_two_channels(precision):
+    """Two channels through the size-specialised kernels, against the
+    runtime-radix kernels and the float64 oracle per channel."""
+    from evoxels_amd import _lib
+    shape = (8, 8, 256)
+    vf, vg = grid_for(shape, precision)
+    prob = CahnHilliard(vg, eps=2.0, D=1.5)
+    step = PseudoSpectralIMEX(prob, 0.07)
+    rng = np.random.default_rng(9)
+    arr = 0.5 + 0.08 * rng.standard_normal((2,) + shape)
+    u = torch.tensor(arr, dtype=vg.precision, device="cuda")
+    fast = step.step(0.0, u).cpu().numpy()
+    plan, _ = step.workspace(u)
+    lib = _lib.load()
+    _lib.check(lib.evx_fft_plan_set_option(plan.handle, _lib.OPT_SPECIALIZED, 0))
+    generic = step.step(0.0, u).cpu().numpy()
+    _lib.check(lib.evx_fft_plan_set_option(plan.handle, _lib.OPT_SPECIALIZED, 1))
+    og, pre, rhs = oracle_imex("ch", arr[0], vf.spacing, "float64", 0.07, {"eps": 2.0, "D": 1.5})
+    tol = 1e-12 if precision == "float64" else 2e-6
+    for c in range(2):
+        ref = O.imex_step(og.field(arr[c]), rhs, pre)[0].numpy()
+        assert np.linalg.norm(fast[c] - ref) / np.linalg.norm(ref) < tol
+    assert np.abs(fast - generic).max() < (1e-13 if precision == "float64" else 1e-5)
