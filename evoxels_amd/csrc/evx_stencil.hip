@@ -109,55 +109,67 @@ __host__ Geo<T> make_geo(const evx_grid* g) {
   return make_geo_from<T>(g->div_dx, g->div_dx2);
 }
 
-// Register-staged load of one x-plane tile [y0-H, y0+BY+H) x [z0-H, z0+BZ+H).
-// fetch() issues the global loads; commit() applies clip and the ghost rules
-// and writes LDS.  Kernels call fetch() for a future plane before computing
-// the current one, so load latency overlaps the stencil arithmetic.
+// Register-staged loader of one x-plane tile [y0-H, y0+BY+H) x [z0-H, z0+BZ+H).
+// init() resolves, once per block, the y/z part of every element this thread
+// loads: periodic axes wrap, non-periodic ones clamp and remember the ghost
+// rule.  fetch() then only resolves the (uniform) x index of the plane and
+// issues one load per element; commit() applies clip and the ghost rules and
+// writes the element to LDS slot tid + k*NT of the plane.
 template <typename T, int H, int ROWS, int W, bool CLIP>
-struct PlaneFetch {
+struct PlaneLoader {
   static constexpr int NPT = (ROWS * W + NT - 1) / NT;
+  int off[NPT];      // in-plane offset y*nz + z of the source point, -1 if unused
+  uint8_t yzf[NPT];  // y/z ghost flags: 4/8 = y low/high, 16/32 = z low/high
   T v[NPT];
-  uint8_t f[NPT];  // ghost-rule flags: 2 bits per axis (1 = low ghost, 2 = high ghost)
+  uint8_t xf;        // x ghost flag of the fetched plane: 1 = low, 2 = high
 
-  __device__ __forceinline__ void fetch(const T* __restrict__ uc, int64_t xg, int64_t y0, int64_t z0,
-                                        const Dims& d, const Bc& bc, bool inside, int tid) {
-    const bool x_in = xg >= 0 && xg < d.nx;
-    if (x_in && inside) {
-      const T* row0 = uc + (xg * d.ny + (y0 - H)) * d.nz + (z0 - H);
+  __device__ __forceinline__ void init(int64_t y0, int64_t z0, const Dims& d, const Bc& bc, int tid) {
 #pragma unroll
-      for (int k = 0; k < NPT; ++k) {
-        const int i = tid + k * NT;
-        f[k] = 0;
-        if (i < ROWS * W) v[k] = row0[(int64_t)(i / W) * d.nz + (i % W)];
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < NPT; ++k) {
-        const int i = tid + k * NT;
-        f[k] = 0;
-        if (i < ROWS * W) {
-          const Resolved<T> r = resolve<T>(xg, y0 + i / W - H, z0 + i % W - H, d, bc);
-          v[k] = uc[r.off];
-          f[k] = (uint8_t)((r.o[0] < 0 ? 1 : r.o[0] > 0 ? 2 : 0) | (r.o[1] < 0 ? 4 : r.o[1] > 0 ? 8 : 0) |
-                           (r.o[2] < 0 ? 16 : r.o[2] > 0 ? 32 : 0));
+    for (int k = 0; k < NPT; ++k) {
+      const int i = tid + k * NT;
+      off[k] = -1;
+      yzf[k] = 0;
+      if (i < ROWS * W) {
+        int64_t gy = y0 - H + i / W, gz = z0 - H + i % W;
+        uint8_t f = 0;
+        if (gy < 0 || gy >= d.ny) {
+          if (bc.kind[1] == EVX_BC_PERIODIC) gy = wrap_near(gy, d.ny);
+          else { f |= gy < 0 ? 4 : 8; gy = gy < 0 ? 0 : d.ny - 1; }
         }
+        if (gz < 0 || gz >= d.nz) {
+          if (bc.kind[2] == EVX_BC_PERIODIC) gz = wrap_near(gz, d.nz);
+          else { f |= gz < 0 ? 16 : 32; gz = gz < 0 ? 0 : d.nz - 1; }
+        }
+        off[k] = (int)(gy * d.nz + gz);
+        yzf[k] = f;
       }
     }
   }
 
-  __device__ __forceinline__ void commit(T (*P)[W], const Bc& bc, int tid) const {
+  __device__ __forceinline__ void fetch(const T* __restrict__ uc, int64_t xg, const Dims& d, const Bc& bc) {
+    xf = 0;
+    if (xg < 0 || xg >= d.nx) {
+      if (bc.kind[0] == EVX_BC_PERIODIC) xg = wrap_near(xg, d.nx);
+      else { xf = xg < 0 ? 1 : 2; xg = xg < 0 ? 0 : d.nx - 1; }
+    }
+    const T* pl = uc + xg * (d.ny * d.nz);
+#pragma unroll
+    for (int k = 0; k < NPT; ++k)
+      if (off[k] >= 0) v[k] = pl[off[k]];
+  }
+
+  __device__ __forceinline__ void commit(T* flat, const Bc& bc, int tid) const {
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
-      const int i = tid + k * NT;
-      if (i < ROWS * W) {
+      if (off[k] >= 0) {
         T x = CLIP ? clip01(v[k]) : v[k];
-        if (f[k]) {
-          const int8_t o[3] = {(int8_t)((f[k] & 1) ? -1 : (f[k] & 2) ? 1 : 0),
-                               (int8_t)((f[k] & 4) ? -1 : (f[k] & 8) ? 1 : 0),
-                               (int8_t)((f[k] & 16) ? -1 : (f[k] & 32) ? 1 : 0)};
+        const int f = xf | yzf[k];
+        if (f) {
+          const int8_t o[3] = {(int8_t)((f & 1) ? -1 : (f & 2) ? 1 : 0), (int8_t)((f & 4) ? -1 : (f & 8) ? 1 : 0),
+                               (int8_t)((f & 16) ? -1 : (f & 32) ? 1 : 0)};
           x = apply_rules(x, o, bc);
         }
-        P[i / W][i % W] = x;
+        flat[tid + k * NT] = x;
       }
     }
   }
@@ -219,20 +231,23 @@ __global__ __launch_bounds__(NT) void rd_rhs_kernel(const T* __restrict__ u, T* 
   const int64_t z0 = o.z0, y0 = o.y0, x0 = o.x0, x1 = o.x1;
   const T* uc = u + o.ch * d.vol();
   T* oc = out + o.ch * d.vol();
-  const bool inside = y0 - 1 >= 0 && y0 + BY + 1 <= d.ny && z0 - 1 >= 0 && z0 + BZ + 1 <= d.nz;
-  PlaneFetch<T, 1, ROWS, W, false> pf;
+  PlaneLoader<T, 1, ROWS, W, false> pl;
+  pl.init(y0, z0, d, bc, tid);
   for (int64_t xg = x0 - 1; xg <= x0 + 1; ++xg) {
-    pf.fetch(uc, xg, y0, z0, d, bc, inside, tid);
-    pf.commit(cs[ring<NR>(xg - x0)], bc, tid);
+    pl.fetch(uc, xg, d, bc);
+    pl.commit(&cs[ring<NR>(xg - x0)][0][0], bc, tid);
   }
+  if (x0 + 2 <= x1) pl.fetch(uc, x0 + 2, d, bc);
   __syncthreads();
+  const int64_t plane = d.ny * d.nz;
+  const int64_t gz = z0 + tz;
   for (int64_t x = x0; x < x1; ++x) {
-    const bool more = x + 2 <= x1;  // plane x+2 is needed by the next output plane
-    if (more) pf.fetch(uc, x + 2, y0, z0, d, bc, inside, tid);
+    // plane x+2 (fetched one iteration ago) goes to the slot of plane x-2
+    if (x + 2 <= x1) pl.commit(&cs[ring<NR>(x + 2 - x0)][0][0], bc, tid);
+    if (x + 3 <= x1) pl.fetch(uc, x + 3, d, bc);
     const T(*Cm)[W] = cs[ring<NR>(x - 1 - x0)];
     const T(*C0)[W] = cs[ring<NR>(x - x0)];
     const T(*Cp)[W] = cs[ring<NR>(x + 1 - x0)];
-    const int64_t gz = z0 + tz;
 #pragma unroll
     for (int q = 0; q < PY; ++q) {
       const int yy = ty + NTY * q;
@@ -241,9 +256,8 @@ __global__ __launch_bounds__(NT) void rd_rhs_kernel(const T* __restrict__ u, T* 
       const int a = yy + 1, b = tz + 1;
       const T c0 = C0[a][b];
       const T lap = lap7(c0, Cp[a][b], Cm[a][b], C0[a + 1][b], C0[a - 1][b], C0[a][b + 1], C0[a][b - 1], g);
-      epilogue_v(ep, c0, oc, (x * d.ny + gy) * d.nz + gz, D * lap);
+      epilogue_v(ep, c0, oc, x * plane + gy * d.nz + gz, D * lap);
     }
-    if (more) pf.commit(cs[ring<NR>(x + 2 - x0)], bc, tid);
     __syncthreads();
   }
 }
@@ -279,7 +293,6 @@ __global__ __launch_bounds__(NT) void ch_rhs_kernel(const T* __restrict__ u, T* 
   const int64_t z0 = o.z0, y0 = o.y0, x0 = o.x0, x1 = o.x1;
   const T* uc = u + o.ch * d.vol();
   T* oc = out + o.ch * d.vol();
-  const bool inside = y0 - 2 >= 0 && y0 + BY + 2 <= d.ny && z0 - 2 >= 0 && z0 + BZ + 2 <= d.nz;
   const bool xper = bc.kind[0] == EVX_BC_PERIODIC;
   const bool yper = bc.kind[1] == EVX_BC_PERIODIC, zper = bc.kind[2] == EVX_BC_PERIODIC;
   const int r0 = ty * PY;
@@ -312,6 +325,29 @@ __global__ __launch_bounds__(NT) void ch_rhs_kernel(const T* __restrict__ u, T* 
   // Tiles that cut a non-periodic y/z boundary evaluate every mu point with
   // the ghost logic; all other tiles only their halo.
   const bool tile_partial = (!yper && y0 + BY > d.ny) || (!zper && z0 + BZ > d.nz);
+  // This thread's point of the one-voxel mu halo (rows -1, BY and columns -1,
+  // BZ, corners excluded), resolved once: c coordinates (ha, hb) of the source
+  // point and its y/z ghost flags.
+  constexpr int NHALO = 2 * MW + 2 * BY - 4;
+  int hy = 0, hz = 0, ha = 0, hb = 0;
+  int8_t hoy = 0, hoz = 0;
+  if (tid < NHALO) {
+    int h = tid;
+    if (h < 2 * (MW - 2)) {
+      hy = h < MW - 2 ? -1 : BY;
+      hz = h < MW - 2 ? h : h - (MW - 2);
+    } else {
+      h -= 2 * (MW - 2);
+      hy = h % BY;
+      hz = h < BY ? -1 : BZ;
+    }
+    ha = hy + 2;
+    hb = hz + 2;
+    const int64_t gy = y0 + hy, gzz = z0 + hz;
+    if (!yper && (gy < 0 || gy >= d.ny)) { hoy = gy < 0 ? -1 : 1; ha += gy < 0 ? 1 : -1; }
+    if (!zper && (gzz < 0 || gzz >= d.nz)) { hoz = gzz < 0 ? -1 : 1; hb += gzz < 0 ? 1 : -1; }
+  }
+
   auto compute_mu = [&](int64_t xm) {
     T(*M)[MW] = ms[ring<NM>(xm - x0 + 1)];
     const bool general = tile_partial || (!xper && (xm < 0 || xm >= d.nx));
@@ -328,42 +364,55 @@ __global__ __launch_bounds__(NT) void ch_rhs_kernel(const T* __restrict__ u, T* 
         M[r0 + q + 1][tz + 1] =
             Op::mu(col[q + 1], Cp[a][b], Cm[a][b], col[q + 2], col[q], C0[a][b + 1], C0[a][b - 1], g, k);
       }
-    }
-    // Points handled with the full ghost logic: the halo, plus the interior
-    // when `general`.  Index p < BY*BZ is interior, the rest is the halo.
-    for (int p = (general ? 0 : BY * BZ) + tid; p < BY * BZ + HALO; p += NT) {
-      int yy, zz;
-      if (p < BY * BZ) {
-        yy = p / BZ;
-        zz = p % BZ;
-      } else {
-        const int h = p - BY * BZ;
-        if (h < 2 * MW) {
-          yy = h < MW ? -1 : BY;
-          zz = (h < MW ? h : h - MW) - 1;
-        } else {
-          const int hh = h - 2 * MW;
-          yy = hh % BY;
-          zz = hh < BY ? -1 : BZ;
+      if (tid < NHALO) {
+        T m = Op::mu(C0[ha][hb], Cp[ha][hb], Cm[ha][hb], C0[ha + 1][hb], C0[ha - 1][hb], C0[ha][hb + 1],
+                     C0[ha][hb - 1], g, k);
+        if (hoy | hoz) {
+          const int8_t oo[3] = {0, hoy, hoz};
+          m = apply_rules(m, oo, bc);
         }
-        if ((yy < 0 || yy >= BY) && (zz < 0 || zz >= BZ)) continue;  // tile corners are never read
+        M[hy + 1][hz + 1] = m;
       }
-      M[yy + 1][zz + 1] = mu_general(xm, yy, zz);
+    } else {
+      for (int p = tid; p < BY * BZ + NHALO; p += NT) {
+        int yy, zz;
+        if (p < BY * BZ) {
+          yy = p / BZ;
+          zz = p % BZ;
+        } else {
+          int h = p - BY * BZ;
+          if (h < 2 * (MW - 2)) {
+            yy = h < MW - 2 ? -1 : BY;
+            zz = h < MW - 2 ? h : h - (MW - 2);
+          } else {
+            h -= 2 * (MW - 2);
+            yy = h % BY;
+            zz = h < BY ? -1 : BZ;
+          }
+        }
+        M[yy + 1][zz + 1] = mu_general(xm, yy, zz);
+      }
     }
   };
 
-  PlaneFetch<T, 2, CR, CW, true> pf;
+  PlaneLoader<T, 2, CR, CW, true> pl;
+  pl.init(y0, z0, d, bc, tid);
   for (int64_t xg = x0 - 2; xg <= x0 + 2; ++xg) {
-    pf.fetch(uc, xg, y0, z0, d, bc, inside, tid);
-    pf.commit(cs[ring<NC>(xg - x0 + 2)], bc, tid);
+    pl.fetch(uc, xg, d, bc);
+    pl.commit(&cs[ring<NC>(xg - x0 + 2)][0][0], bc, tid);
   }
+  if (x0 + 3 <= x1 + 1) pl.fetch(uc, x0 + 3, d, bc);
   __syncthreads();
   compute_mu(x0 - 1);
   compute_mu(x0);
+  __syncthreads();  // the first commit below reuses the slot mu(x0-1) read
+  const int64_t plane = d.ny * d.nz;
   T carry[PY];  // flux through the x face x-1/2 of each own row
   for (int64_t x = x0; x < x1; ++x) {
-    const bool more = x + 3 <= x1 + 1;  // plane x+3 feeds mu(x+2), needed while x+1 < x1
-    if (more) pf.fetch(uc, x + 3, y0, z0, d, bc, inside, tid);
+    // plane x+3 (fetched one iteration ago) goes to the slot of plane x-2;
+    // it feeds mu(x+2), needed while x+1 < x1
+    if (x + 3 <= x1 + 1) pl.commit(&cs[ring<NC>(x + 3 - x0 + 2)][0][0], bc, tid);
+    if (x + 4 <= x1 + 1) pl.fetch(uc, x + 4, d, bc);
     compute_mu(x + 1);
     __syncthreads();
     const T(*C0)[CW] = cs[ring<NC>(x - x0 + 2)];
@@ -395,9 +444,8 @@ __global__ __launch_bounds__(NT) void ch_rhs_kernel(const T* __restrict__ u, T* 
       const T r = Op::div(carry[q], fxp, fy[q], fy[q + 1], fzm, fzp, g, k.D);
       carry[q] = fxp;
       const int64_t gy = y0 + r0 + q;
-      if (gy < d.ny && gz < d.nz) epilogue(ep, uc, oc, (x * d.ny + gy) * d.nz + gz, r);
+      if (gy < d.ny && gz < d.nz) epilogue(ep, uc, oc, x * plane + gy * d.nz + gz, r);
     }
-    if (more) pf.commit(cs[ring<NC>(x + 3 - x0 + 2)], bc, tid);
     __syncthreads();
   }
 }
@@ -453,20 +501,22 @@ __global__ __launch_bounds__(NT) void ac_rhs_kernel(const T* __restrict__ u, T* 
   const int64_t z0 = o.z0, y0 = o.y0, x0 = o.x0, x1 = o.x1;
   const T* uc = u + o.ch * d.vol();
   T* oc = out + o.ch * d.vol();
-  const bool inside = y0 - 1 >= 0 && y0 + BY + 1 <= d.ny && z0 - 1 >= 0 && z0 + BZ + 1 <= d.nz;
-  PlaneFetch<T, 1, ROWS, W, true> pf;
+  PlaneLoader<T, 1, ROWS, W, true> pl;
+  pl.init(y0, z0, d, bc, tid);
   for (int64_t xg = x0 - 1; xg <= x0 + 1; ++xg) {
-    pf.fetch(uc, xg, y0, z0, d, bc, inside, tid);
-    pf.commit(cs[ring<NR>(xg - x0)], bc, tid);
+    pl.fetch(uc, xg, d, bc);
+    pl.commit(&cs[ring<NR>(xg - x0)][0][0], bc, tid);
   }
+  if (x0 + 2 <= x1) pl.fetch(uc, x0 + 2, d, bc);
   __syncthreads();
+  const int64_t plane = d.ny * d.nz;
+  const int64_t gz = z0 + tz;
   for (int64_t x = x0; x < x1; ++x) {
-    const bool more = x + 2 <= x1;
-    if (more) pf.fetch(uc, x + 2, y0, z0, d, bc, inside, tid);
+    if (x + 2 <= x1) pl.commit(&cs[ring<NR>(x + 2 - x0)][0][0], bc, tid);
+    if (x + 3 <= x1) pl.fetch(uc, x + 3, d, bc);
     const T(*Pm)[W] = cs[ring<NR>(x - 1 - x0)];
     const T(*P0)[W] = cs[ring<NR>(x - x0)];
     const T(*Pp)[W] = cs[ring<NR>(x + 1 - x0)];
-    const int64_t gz = z0 + tz;
 #pragma unroll
     for (int q = 0; q < PY; ++q) {
       const int yy = ty + NTY * q;
@@ -480,9 +530,8 @@ __global__ __launch_bounds__(NT) void ac_rhs_kernel(const T* __restrict__ u, T* 
       const T nlap = k.one_m_curv * normal_lap_at<T, W>(Pm, P0, Pp, a, b, g);
       const T df = k.gab * ((lap + nlap) - (pot / T(2)) / k.eps) +
                    ((k.s3 * phi) * (T(1) - phi)) * k.force;
-      epilogue(ep, uc, oc, (x * d.ny + gy) * d.nz + gz, k.M * df);
+      epilogue(ep, uc, oc, x * plane + gy * d.nz + gz, k.M * df);
     }
-    if (more) pf.commit(cs[ring<NR>(x + 2 - x0)], bc, tid);
     __syncthreads();
   }
 }
