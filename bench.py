@@ -29,6 +29,7 @@ import time
 from pathlib import Path
 
 import numpy as np
+import torch
 
 METRIC = "voxel-steps/s + HBM GB/s (CH 512^3 FP32); fwd+adjoint step vs torch autograd"
 PEAK_GBPS = 8000.0
@@ -86,14 +87,208 @@ def cpu_baseline(c0, steps):
                       f"torch {torch.__version__} eager on CPU, os.cpu_count()={os.cpu_count()}"}
 
 
+def _pad_periodic(x):
+    """Periodic ghost layer on the three spatial axes of a (B, nx, ny, nz) tensor."""
+    x = torch.cat([x[:, -1:], x, x[:, :1]], 1)
+    x = torch.cat([x[:, :, -1:], x, x[:, :, :1]], 2)
+    return torch.cat([x[:, :, :, -1:], x, x[:, :, :, :1]], 3)
+
+
+def _torch_ch_rhs(c, eps, D, idx, idx2):
+    """Eager torch form of CahnHilliard.rhs (problem_definition.py:328-371) in
+    the reference's operator structure (pad, face averages on the padded
+    array, face gradients, trim), with per-trajectory eps, D of shape
+    (B, 1, 1, 1).  Used only as the torch-autograd comparator."""
+    c = torch.clip(c, 0, 1)
+    cb = _pad_periodic(c)
+    sum2 = (idx2[0] + idx2[1]) + idx2[2]
+    lap = (((cb[:, 2:, 1:-1, 1:-1] + cb[:, :-2, 1:-1, 1:-1]) * idx2[0]
+            + (cb[:, 1:-1, 2:, 1:-1] + cb[:, 1:-1, :-2, 1:-1]) * idx2[1])
+           + (cb[:, 1:-1, 1:-1, 2:] + cb[:, 1:-1, 1:-1, :-2]) * idx2[2]) - 2 * c * sum2
+    mu = 18 / eps * c * (1 - c) * (1 - 2 * c) - 2 * eps * lap
+    mb = _pad_periodic(mu)
+    div = 0
+    for a in range(3):
+        ax = a + 1
+        n = cb.shape[ax]
+        cf = 0.5 * (cb.narrow(ax, 1, n - 1) + cb.narrow(ax, 0, n - 1))
+        flux = cf * (1 - cf) * ((mb.narrow(ax, 1, n - 1) - mb.narrow(ax, 0, n - 1)) * idx[a])
+        d = (flux.narrow(ax, 1, n - 2) - flux.narrow(ax, 0, n - 2)) * idx[a]
+        sl = [slice(None)] + [slice(None) if b == a else slice(1, -1) for b in range(3)]
+        div = div + d[tuple(sl)]
+    return D * div
+
+
+def inversion_leg(steps, reps, torch_reps):
+    """Config: 16 trajectories of 128^3 float32, truth (M, kappa) ~ U[0.5, 2]
+    (numpy default_rng(4)), D = M, eps = kappa/2 (mu = f'(c) - kappa lap c with
+    kappa = 2 eps), dt = 0.02, domain 128, observations at 5 frames.  Times one
+    gradient evaluation J, dJ/d(eps, D) (forward sweep with checkpoints every
+    20 steps, segment recompute, adjoint sweep) of the HIP kernels and of
+    torch autograd with torch.utils.checkpoint every 20 steps on the same GPU,
+    and reports the relative difference of the two gradients."""
+    import torch.utils.checkpoint as tuc
+    from evoxels_amd.inversion import CHImexEngine, loss_and_grad, solve
+    from evoxels_amd.voxelgrid import VoxelGridHip
+    import evoxels_amd as evx
+    B, n, dt, A, ce = 16, 128, 0.02, 0.25, 20
+    obs_steps = [steps * (i + 1) // 5 for i in range(5)]
+    rng = np.random.default_rng(4)
+    truth_mk = rng.uniform(0.5, 2.0, (B, 2))
+    truth = np.stack([truth_mk[:, 1] / 2, truth_mk[:, 0]], 1)  # (eps, D)
+    guess = truth * rng.uniform(0.8, 1.25, (B, 2))
+    c0 = 0.5 + 0.05 * rng.standard_normal((B, n, n, n))
+    vf = evx.VoxelFields((n, n, n), (n, n, n))
+    vg = VoxelGridHip(vf.grid_info(), precision="float32", device="cuda")
+    eng = CHImexEngine(vg, B, dt, A)
+    u0 = torch.tensor(c0, dtype=torch.float32, device="cuda")
+    obs = solve(eng, u0, truth, steps, obs_steps)
+    log(f"[bench] inversion: {B}x{n}^3, {steps} steps, obs at {obs_steps}")
+    loss, grad = loss_and_grad(eng, u0, guess, steps, obs_steps, obs, ce)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss, grad = loss_and_grad(eng, u0, guess, steps, obs_steps, obs, ce)
+    torch.cuda.synchronize()
+    t_hip = (time.perf_counter() - t0) / reps
+    log(f"[bench] inversion: hip {t_hip:.3f} s/gradient")
+
+    idx = [1.0, 1.0, 1.0]
+    k = [2 * np.pi * torch.fft.fftfreq(n, d=1.0) for _ in range(2)] + [2 * np.pi * torch.fft.rfftfreq(n, d=1.0)]
+    kx, ky, kz = torch.meshgrid(*k, indexing="ij")
+    k2 = (kx ** 2 + ky ** 2 + kz ** 2).to("cuda")  # float32, as fftfreq returns
+    obs_t = [o.detach() for o in obs]
+
+    def run_torch():
+        eps = torch.tensor(guess[:, 0], dtype=torch.float32, device="cuda").reshape(B, 1, 1, 1).requires_grad_()
+        D = torch.tensor(guess[:, 1], dtype=torch.float32, device="cuda").reshape(B, 1, 1, 1).requires_grad_()
+
+        def segment(u, e, d):
+            pre = dt / (1 - dt * (-2 * e * d * A * k2 ** 2))
+            for _ in range(ce):
+                r = _torch_ch_rhs(u, e, d, idx, idx)
+                u = u + torch.fft.irfftn(pre * torch.fft.rfftn(r, dim=(1, 2, 3)), s=(n, n, n), dim=(1, 2, 3))
+            return u
+
+        u = u0
+        J = 0
+        for s in range(0, steps, ce):
+            u = tuc.checkpoint(segment, u, eps, D, use_reentrant=False)
+            if s + ce in obs_steps:
+                J = J + 0.5 * ((u - obs_t[obs_steps.index(s + ce)]) ** 2).sum(dim=(1, 2, 3), dtype=torch.float64)
+        J.sum().backward()
+        return J.detach(), torch.stack([eps.grad.reshape(B), D.grad.reshape(B)], 1).double()
+
+    Jt, gt = run_torch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(torch_reps):
+        Jt, gt = run_torch()
+    torch.cuda.synchronize()
+    t_torch = (time.perf_counter() - t0) / torch_reps
+    log(f"[bench] inversion: torch autograd {t_torch:.3f} s/gradient")
+    g_h = grad.cpu().numpy()
+    g_t = gt.cpu().numpy()
+    scale = np.abs(g_t).max(axis=1, keepdims=True)
+    return {"workload": f"ch_inversion_{B}x{n}^3_fp32_{steps}_steps", "trajectories": B, "grid": [n, n, n],
+            "steps": steps, "dt": dt, "checkpoint_every": ce, "obs_steps": obs_steps,
+            "params": "per trajectory (eps, D); truth (M, kappa) ~ U[0.5,2], D = M, eps = kappa/2, default_rng(4)",
+            "hip_s_per_gradient": round(t_hip, 4), "torch_autograd_s_per_gradient": round(t_torch, 4),
+            "speedup_vs_torch_autograd": round(t_torch / t_hip, 2),
+            "voxel_steps_per_s": B * n ** 3 * steps / t_hip,
+            "grad_rel_diff_vs_torch_autograd": float(np.abs(g_h - g_t).max() / scale.max()),
+            "loss_rel_diff_vs_torch": float(np.abs(loss.cpu().numpy() - Jt.cpu().numpy()).max()
+                                            / np.abs(Jt.cpu().numpy()).max())}
+
+
+def batched_leg(steps, seq_steps, cpu_steps):
+    """48 periodic CH grids, edges drawn from {32,48,64,96,128} per axis with
+    numpy default_rng(3), c0 = U[0.3,0.7] + 0.05*N(0,1) per grid, float32,
+    dt = 0.1, eps = 3, D = 1 (run_cahn_hilliard_solver defaults), unit spacing.
+    The autotuned batched step is captured in a graph of `steps` steps and
+    replayed; compared with the single-grid fused step called grid by grid
+    and with the CPU oracle run grid by grid."""
+    from evoxels_amd.batched import BatchedCahnHilliard
+    from evoxels_amd.problem_definition import CahnHilliard
+    from evoxels_amd.timesteppers import PseudoSpectralIMEX
+    from evoxels_amd.voxelgrid import VoxelGridHip
+    import evoxels_amd as evx
+    rng = np.random.default_rng(3)
+    shapes = [tuple(int(v) for v in s) for s in rng.choice([32, 48, 64, 96, 128], (48, 3))]
+    arrs = [rng.uniform(0.3, 0.7) + 0.05 * rng.standard_normal(s) for s in shapes]
+    vox = int(sum(np.prod(s) for s in shapes))
+    b_alg = sum(4 * (3 + 8 * (1 + 2 / s[2])) * np.prod(s) for s in shapes)  # five-sweep minimum
+    b = BatchedCahnHilliard(shapes, 0.1, eps=3.0, D=1.0)
+    state = b.pack(arrs)
+    log(f"[bench] batched: {len(shapes)} grids, {vox} voxels; autotuning")
+    sched = b.autotune(state, steps=5)
+    log(f"[bench] batched: schedule {sched}")
+    g = b.capture(state, steps)
+    g.replay()  # warm-up replay
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    assert b.error_flag() == 0
+    # grid-by-grid single-grid fused steps on the same GPU
+    singles = []
+    for s, a in zip(shapes, arrs):
+        vf = evx.VoxelFields(s, s)
+        vg = VoxelGridHip(vf.grid_info(), precision="float32", device="cuda")
+        st = PseudoSpectralIMEX(CahnHilliard(vg, eps=3.0, D=1.0), 0.1)
+        singles.append((st, vg.init_scalar_field(a).contiguous()))
+    for st, u in singles:
+        st.step_into(u, u)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(seq_steps):
+        for st, u in singles:
+            st.step_into(u, u)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_seq = e0.elapsed_time(e1) / seq_steps
+    out = {"workload": "ch_batched_48_uneven_fp32", "grids": len(shapes), "voxels": vox, "steps": steps,
+           "schedule": sched, "ms_per_step": round(ms, 4), "voxel_steps_per_s": vox / (ms * 1e-3),
+           "alg_gbps": round(b_alg / (ms * 1e-3) / 1e9, 1),
+           "note": "every grid fits the 256 MB last-level cache, so the GB/s figure is not an HBM fraction",
+           "single_grid_loop_ms_per_step": round(ms_seq, 4), "speedup_vs_single_grid_loop": round(ms_seq / ms, 2)}
+    if cpu_steps > 0:
+        sys.path.insert(0, str(ROOT))
+        from oracle import evoxels_oracle as O
+        P3 = (("periodic", None),) * 3
+        fields = []
+        for s, a in zip(shapes, arrs):
+            og = O.OracleGrid(s, (1.0, 1.0, 1.0), "float32")
+            pre = O.imex_prefac(0.1, O.ch_symbol(O.rfft_k_squared(s, og.spacing), 3.0, 1.0))
+            fields.append((og, pre, og.field(a)))
+        t0 = time.perf_counter()
+        for _ in range(cpu_steps):
+            fields = [(og, pre, O.imex_step(u, lambda t, x, og=og: O.ch_rhs(x, og, P3, 3.0, 1.0), pre))
+                      for og, pre, u in fields]
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": vox * cpu_steps / dt, "unit": "voxel-steps/s",
+                               "cores": torch.get_num_threads(), "kind": "port",
+                               "sample": f"{cpu_steps} steps of all 48 grids, grid by grid, CPU oracle"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-inversion", action="store_true", help="skip the forward+adjoint inversion leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-grid leg")
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--inv-steps", type=int, default=200)
+    ap.add_argument("--inv-reps", type=int, default=3)
+    ap.add_argument("--inv-torch-reps", type=int, default=1)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 1 GiB (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -202,6 +397,20 @@ def main():
     }
     del u
     torch.cuda.empty_cache()
+    if not args.no_batched:
+        try:
+            out["batched"] = batched_leg(args.batch_steps, 10, 0 if args.no_cpu_baseline else 2)
+        except Exception as e:  # reported in the JSON line; the headline stands on its own
+            out["batched"] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] batched leg failed: {e}")
+        torch.cuda.empty_cache()
+    if not args.no_inversion:
+        try:
+            out["inversion"] = inversion_leg(args.inv_steps, args.inv_reps, args.inv_torch_reps)
+        except Exception as e:  # reported in the JSON line; the headline stands on its own
+            out["inversion"] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] inversion leg failed: {e}")
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline: oracle on {args.cpu_steps} steps (+1 warm-up)")
         out["cpu_baseline"] = cpu_baseline(c0.astype(np.float32), args.cpu_steps)

@@ -48,7 +48,7 @@ class EvxEpilogue(C.Structure):
 
 class EvxImexParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("spacing", C.c_double * 3), ("eps", C.c_double),
-                ("D", C.c_double), ("A", C.c_double)]
+                ("D", C.c_double), ("A", C.c_double), ("chan_params", C.c_void_p)]
 
 
 _P = C.c_void_p
@@ -76,6 +76,21 @@ SIGNATURES = {
     "evx_irfftn": [_P, _P, _P, _P],
     "evx_ch_imex_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
     "evx_rd_imex_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
+    "evx_ch_rhs_vjp_workspace_bytes": [_I32, C.POINTER(EvxGrid), C.POINTER(C.c_size_t)],
+    "evx_ch_rhs_vjp": [_I32, _P, _P, _P, _P, _P, C.POINTER(EvxGrid), _D, _D, _P, _P, _P],
+    "evx_ch_imex_vjp_workspace_bytes": [_P, C.POINTER(C.c_size_t)],
+    "evx_ch_imex_step_vjp": [_P, _P, _P, _P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
+    "evx_batch_plan_create": [C.POINTER(_P), _I32, _I32, _P, _P],
+    "evx_batch_plan_destroy": [_P],
+    "evx_batch_state_layout": [_P, _P, C.POINTER(_I64)],
+    "evx_batch_workspace_bytes": [_P, C.POINTER(C.c_size_t)],
+    "evx_batch_set_params": [_P, _P, _D, _D],
+    "evx_batch_set_schedule": [_P, C.c_int, C.c_int, C.c_int],
+    "evx_batch_get_schedule": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                               C.POINTER(C.c_int)],
+    "evx_batch_autotune": [_P, _P, _P, C.c_int, _P, C.POINTER(_D)],
+    "evx_batch_ch_imex_step": [_P, _P, _P, _P],
+    "evx_batch_error_flag": [_P, C.POINTER(C.c_int)],
     "evx_fft_plan_trace_next_step": [_P, C.POINTER(_P), C.c_int],
     "evx_fft_plan_set_option": [_P, C.c_int, C.c_int],
     "evx_reduce_workspace_bytes": [_I64, C.POINTER(C.c_size_t)],

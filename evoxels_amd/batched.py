@@ -1,0 +1,142 @@
+"""Many independent periodic Cahn-Hilliard grids of different shapes, advanced
+together on one GPU.
+
+Each grid follows ``CahnHilliard`` + ``PseudoSpectralIMEX``
+(problem_definition.py:290-371, timesteppers.py:64-89) exactly as the
+single-grid fused step does (bit-identical per grid).  All grids live in one
+packed float32 state; one call advances every grid by one step, either with
+one persistent launch (work items pulled from a device queue, per-grid sweep
+dependencies) or with one launch per sweep.  ``autotune`` times the schedules
+and tile shapes and keeps the fastest; ``capture`` records a number of steps
+in a CUDA/HIP graph.
+
+The reference has no batched solver: it advances one grid per solver
+(solvers.py:80-114).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import weakref
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+
+STAGED, PERSISTENT = 0, 1
+
+
+class BatchedCahnHilliard:
+    """Semi-implicit CH steps for grids of shapes ``shapes`` (edges in
+    {16, 32, 48, 64, 96, 128}); ``eps``/``D`` are scalars or per-grid
+    sequences; ``spacing`` is None (unit spacing) or one 3-tuple per grid."""
+
+    def __init__(self, shapes: Sequence[Sequence[int]], dt: float, eps=3.0, D=1.0, A: float = 0.25,
+                 spacing=None, device="cuda"):
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        n = len(self.shapes)
+        if n == 0:
+            raise ValueError("at least one grid is required")
+        if any(len(s) != 3 for s in self.shapes):
+            raise ValueError("every shape must be (nx, ny, nz)")
+        dev = torch.device(device)
+        if dev.type != "cuda" or not torch.cuda.is_available():
+            raise RuntimeError("the 'hip' backend needs a ROCm GPU")
+        self.device = dev
+        sp = np.ones((n, 3)) if spacing is None else np.asarray(spacing, dtype=np.float64).reshape(n, 3)
+        shp = np.ascontiguousarray(np.asarray(self.shapes, dtype=np.int64))
+        sp = np.ascontiguousarray(sp)
+        lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(lib.evx_batch_plan_create(C.byref(h), _lib.EVX_F32, n, shp.ctypes.data, sp.ctypes.data),
+                   "evx_batch_plan_create")
+        self.handle = h
+        self._fin = weakref.finalize(self, lib.evx_batch_plan_destroy, h)
+        offs = np.zeros(n, dtype=np.int64)
+        total = C.c_int64()
+        _lib.check(lib.evx_batch_state_layout(h, offs.ctypes.data, C.byref(total)), "evx_batch_state_layout")
+        self.offsets = [int(v) for v in offs]
+        self.total = int(total.value)
+        nb = C.c_size_t()
+        _lib.check(lib.evx_batch_workspace_bytes(h, C.byref(nb)), "evx_batch_workspace_bytes")
+        self.ws = torch.empty(nb.value, dtype=torch.uint8, device=dev)
+        self.dt = float(dt)
+        self.A = float(A)
+        self.set_params(eps, D)
+
+    def set_params(self, eps, D):
+        n = len(self.shapes)
+        e = np.broadcast_to(np.asarray(eps, dtype=np.float64), (n,))
+        d = np.broadcast_to(np.asarray(D, dtype=np.float64), (n,))
+        ed = np.ascontiguousarray(np.stack([e, d], 1))
+        self.eps, self.D = e.copy(), d.copy()
+        _lib.check(_lib.load().evx_batch_set_params(self.handle, ed.ctypes.data, self.dt, self.A),
+                   "evx_batch_set_params")
+
+    # ------------------------------------------------------------ state
+    def new_state(self):
+        return torch.zeros(self.total, dtype=torch.float32, device=self.device)
+
+    def views(self, state):
+        """Per-grid (nx, ny, nz) views into the packed state."""
+        return [state[o:o + int(np.prod(s))].view(s) for o, s in zip(self.offsets, self.shapes)]
+
+    def pack(self, arrays):
+        state = self.new_state()
+        for v, a in zip(self.views(state), arrays):
+            v.copy_(torch.as_tensor(np.asarray(a), dtype=torch.float32))
+        return state
+
+    # ------------------------------------------------------------ steps
+    def step(self, state, n: int = 1):
+        if state.dtype != torch.float32 or state.numel() < self.total or not state.is_contiguous() \
+                or state.device != self.device:
+            raise ValueError("state must be a contiguous float32 tensor from new_state()/pack()")
+        lib = _lib.load()
+        s = _lib.stream_handle(state.device)
+        for _ in range(n):
+            _lib.check(lib.evx_batch_ch_imex_step(self.handle, _lib.ptr(state), _lib.ptr(self.ws), s),
+                       "evx_batch_ch_imex_step")
+        return state
+
+    def capture(self, state, n_steps: int):
+        """A graph whose replay advances ``state`` by ``n_steps`` steps."""
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=side):
+            self.step(state, n_steps)
+        return g
+
+    # ------------------------------------------------------------ schedule
+    @property
+    def schedule(self):
+        m, b, x, ni = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _lib.check(_lib.load().evx_batch_get_schedule(self.handle, C.byref(m), C.byref(b), C.byref(x),
+                                                      C.byref(ni)), "evx_batch_get_schedule")
+        return {"mode": "persistent" if m.value == PERSISTENT else "staged", "blocks": b.value,
+                "xchunk": x.value, "items": ni.value}
+
+    def set_schedule(self, mode="persistent", blocks=0, xchunk=8):
+        torch.cuda.synchronize(self.device)
+        code = PERSISTENT if mode == "persistent" else STAGED
+        _lib.check(_lib.load().evx_batch_set_schedule(self.handle, code, int(blocks), int(xchunk)),
+                   "evx_batch_set_schedule")
+
+    def autotune(self, state, steps: int = 5):
+        """Times the candidate schedules on a copy of ``state`` and keeps the
+        fastest; returns the schedule with its time per step."""
+        scratch = state.clone()
+        best = C.c_double()
+        _lib.check(_lib.load().evx_batch_autotune(self.handle, _lib.ptr(scratch), _lib.ptr(self.ws), int(steps),
+                                                  _lib.stream_handle(state.device), C.byref(best)),
+                   "evx_batch_autotune")
+        out = self.schedule
+        out["ms_per_step"] = best.value
+        return out
+
+    def error_flag(self) -> int:
+        f = C.c_int()
+        _lib.check(_lib.load().evx_batch_error_flag(self.handle, C.byref(f)), "evx_batch_error_flag")
+        return f.value

@@ -1,0 +1,611 @@
+// Batched semi-implicit Cahn-Hilliard steps for many independent periodic
+// grids of different shapes (float32).
+//
+// A step of one grid is six sweeps (CH right-hand side, z R2C, y forward,
+// x forward * prefac * inverse, y inverse, z C2R + add).  Each sweep is cut
+// into tiles; an item is (grid, sweep, tile).  Items are listed sweep-major
+// over all grids.  Two schedules run the same tile bodies:
+//
+//   persistent  one launch per step.  Workgroups claim items from a device
+//               queue in list order; before an item of sweep s of grid g
+//               starts, all items of sweep s-1 of grid g must be done (a
+//               per-grid counter, release/acquire at agent scope).  Every
+//               dependency of an item lies earlier in the list and was
+//               therefore claimed by a running workgroup, so the waits always
+//               end; each wait is also bounded (error flag) so a wave can
+//               never spin forever.  The last workgroup to leave resets the
+//               queue and counters for the next launch (no memset, so the
+//               launch can be captured in a graph).
+//   staged      one launch per sweep, one workgroup per item.
+//
+// The tile bodies are the ones of the single-grid kernels, so every grid
+// gets bit-identical results to evx_ch_imex_step on that grid alone.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "evx_fft_tiles.hpp"
+#include "evx_stencil_tiles.hpp"
+
+namespace evx {
+namespace {
+
+using T = float;
+using C = cpx<float>;
+
+constexpr int NSTAGE = 6;
+constexpr int BT = 256;  // threads of every batched workgroup
+
+// FFT sizes the batched kernels are built for (all give 256-thread tiles).
+#define EVX_BATCH_SIZES(X) X(16) X(32) X(48) X(64) X(96) X(128)
+
+template <int N>
+constexpr size_t fft_lds() {
+  return ZTile<T, N>::LDS_BYTES > STile<T, N>::LDS_BYTES ? ZTile<T, N>::LDS_BYTES : STile<T, N>::LDS_BYTES;
+}
+constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
+constexpr size_t BATCH_LDS =
+    cmax(ChTile<T>::LDS_BYTES,
+         cmax(fft_lds<16>(), cmax(fft_lds<32>(), cmax(fft_lds<48>(), cmax(fft_lds<64>(), cmax(fft_lds<96>(),
+                                                                                           fft_lds<128>()))))));
+static_assert(BATCH_LDS <= 160 * 1024 / 3, "three batched workgroups must fit one CU's LDS");
+
+#define X(N) static_assert(ZGeo<T, N>::THREADS == BT && SGeo<T, N>::THREADS == BT, "256-thread tiles");
+EVX_BATCH_SIZES(X)
+#undef X
+
+struct BInst {
+  int nx, ny, nz, nh;
+  int64_t u_off;  // floats: field in the packed state
+  int64_t r_off;  // floats: right-hand side staging in the workspace
+  int64_t s_off;  // complex: half spectrum in the workspace
+  int tw[3];      // complex offsets of the x, y, z twiddle tables
+  int nt[NSTAGE]; // tiles per sweep
+  Geo<T> g;
+  ChCoef<T> k;
+  Prefac pf;
+  TileGrid tg;
+};
+
+struct BArgs {
+  const BInst* inst;
+  const int4* items;  // {grid, sweep, tile, 0}
+  int n_items;
+  int n_inst;
+  T* u;
+  T* rhs;
+  C* spec;
+  const C* tw;
+  unsigned* ctr;  // [0] queue head, [1] workgroups done, [2..] per-grid sweep counters
+  int* err;
+};
+
+template <int MODE>
+__device__ __forceinline__ void strided_dispatch(const BInst& b, C* spec, const C* tw, int axis, int tile, int tid,
+                                                 unsigned char* smem) {
+  LineMap lm;
+  lm.nh = b.nh;
+  int64_t nlines;
+  int n;
+  if (axis == 1) {
+    nlines = (int64_t)b.nx * b.nh;
+    lm.G = 1;
+    lm.GM = (int64_t)b.ny * b.nh;
+    lm.OM = 0;
+    lm.es = b.nh;
+    n = b.ny;
+  } else {
+    nlines = (int64_t)b.ny * b.nh;
+    lm.G = b.ny;
+    lm.GM = (int64_t)b.nx * b.ny * b.nh;
+    lm.OM = b.nh;
+    lm.es = (int64_t)b.ny * b.nh;
+    n = b.nx;
+  }
+  C* s = spec + b.s_off;
+  const C* t = tw + b.tw[axis];
+  switch (n) {
+#define X(N)                                                            \
+  case N:                                                               \
+    strided_tile<T, N, MODE>(s, nlines, lm, t, b.pf, tile, tid, smem); \
+    break;
+    EVX_BATCH_SIZES(X)
+#undef X
+    default:
+      break;
+  }
+}
+
+// One item of the batched step.
+__device__ __forceinline__ void run_item(const BArgs& a, int4 it, int tid, unsigned char* smem) {
+  const BInst& b = a.inst[it.x];
+  const int tile = it.z;
+  switch (it.y) {
+    case 0: {
+      const Dims d{1, b.nx, b.ny, b.nz};
+      Bc bc{};
+      const Epi ep{EVX_OUT_RHS, 0.0, nullptr, nullptr, nullptr};
+      ch_rhs_tile<T, false>(a.u + b.u_off, a.rhs + b.r_off, d, bc, b.g, b.k, ep, b.tg, nullptr, tile, tid, smem);
+      break;
+    }
+    case 1: {
+      const int64_t nlines = (int64_t)b.nx * b.ny;
+      switch (b.nz) {
+#define X(N)                                                                                                   \
+  case N:                                                                                                      \
+    z_r2c_tile<T, N>(a.rhs + b.r_off, a.spec + b.s_off, nlines, a.tw + b.tw[2], tile, tid, smem); \
+    break;
+        EVX_BATCH_SIZES(X)
+#undef X
+        default:
+          break;
+      }
+      break;
+    }
+    case 2:
+      strided_dispatch<0>(b, a.spec, a.tw, 1, tile, tid, smem);
+      break;
+    case 3:
+      strided_dispatch<2>(b, a.spec, a.tw, 0, tile, tid, smem);
+      break;
+    case 4:
+      strided_dispatch<1>(b, a.spec, a.tw, 1, tile, tid, smem);
+      break;
+    default: {
+      const int64_t nlines = (int64_t)b.nx * b.ny;
+      T* u = a.u + b.u_off;
+      switch (b.nz) {
+#define X(N)                                                                                              \
+  case N:                                                                                                 \
+    z_c2r_tile<T, N>(a.spec + b.s_off, u, u, T(1), nlines, a.tw + b.tw[2], tile, tid, smem); \
+    break;
+        EVX_BATCH_SIZES(X)
+#undef X
+        default:
+          break;
+      }
+      break;
+    }
+  }
+}
+
+// Upper bound of sleeping polls of one dependency wait (about half a second,
+// far beyond any sweep).  Once one wait has hit it, the plan's error flag is
+// set and every later wait returns at once, so a launch always drains
+// quickly even if a dependency were never met.
+constexpr unsigned SPIN_LIMIT = 1u << 23;
+
+__device__ __forceinline__ void wait_for(unsigned* p, unsigned target, int* err) {
+  unsigned spins = 0;
+  while (__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    if ((spins & 1023u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+    __builtin_amdgcn_s_sleep(2);
+    if (++spins > SPIN_LIMIT) {
+      __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+  }
+}
+
+__global__ __launch_bounds__(BT) void batch_persistent(BArgs a) {
+  __shared__ __align__(16) unsigned char smem[BATCH_LDS];
+  __shared__ int s_idx;
+  const int tid = threadIdx.x;
+  unsigned* head = a.ctr;
+  unsigned* left = a.ctr + 1;
+  unsigned* done = a.ctr + 2;
+  for (;;) {
+    if (tid == 0) s_idx = (int)__hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int idx = s_idx;
+    if (idx >= a.n_items) break;
+    const int4 it = a.items[idx];
+    if (tid == 0 && it.y > 0) wait_for(done + it.x * NSTAGE + it.y - 1, (unsigned)a.inst[it.x].nt[it.y - 1], a.err);
+    __syncthreads();
+    // Re-derive the thread index each iteration so that the per-size address
+    // arithmetic of every tile body is not hoisted out of the loop (which
+    // would keep all of it live at once).
+    int t = tid;
+    asm volatile("" : "+v"(t));
+    run_item(a, it, t, smem);
+    __syncthreads();
+    if (tid == 0) __hip_atomic_fetch_add(done + it.x * NSTAGE + it.y, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (tid == 0) {
+    const unsigned e = __hip_atomic_fetch_add(left, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (e == gridDim.x - 1) {  // every other workgroup has left: reset for the next launch
+      for (int i = 0; i < a.n_inst * NSTAGE; ++i) __hip_atomic_store(done + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(head, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(left, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+__global__ __launch_bounds__(BT) void batch_stage(BArgs a, int first) {
+  __shared__ __align__(16) unsigned char smem[BATCH_LDS];
+  run_item(a, a.items[first + (int)blockIdx.x], threadIdx.x, smem);
+}
+
+size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
+
+bool batch_size_ok(int64_t n) {
+  switch (n) {
+#define X(N) case N:
+    EVX_BATCH_SIZES(X)
+#undef X
+    return true;
+    default:
+      return false;
+  }
+}
+
+template <int N>
+constexpr int zt_lp() { return ZGeo<T, N>::LP; }
+template <int N>
+constexpr int st_tl() { return SGeo<T, N>::TL; }
+
+int lines_per_z_tile(int n) {
+  switch (n) {
+#define X(N) \
+  case N:    \
+    return 2 * zt_lp<N>();
+    EVX_BATCH_SIZES(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+int lines_per_brick(int n) {
+  switch (n) {
+#define X(N) \
+  case N:    \
+    return st_tl<N>();
+    EVX_BATCH_SIZES(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+
+int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+}  // namespace evx
+
+struct evx_batch_plan {
+  int n;
+  std::vector<evx::BInst> inst;
+  std::vector<int> tw_off_of_size;  // per grid: twiddle offsets (host copy in inst)
+  int64_t state_elems = 0;
+  size_t ws_bytes = 0;
+  // schedule
+  int mode = 1;     // 1 persistent, 0 staged
+  int blocks = 0;   // persistent workgroups (0: 3 per CU)
+  int xchunk = 8;   // x planes per CH tile
+  int n_items = 0;
+  int stage_first[evx::NSTAGE + 1] = {0};
+  bool params_set = false;
+  // device
+  evx::BInst* d_inst = nullptr;
+  int4* d_items = nullptr;
+  evx::C* d_tw = nullptr;
+  unsigned* d_ctr = nullptr;
+  int* d_err = nullptr;
+  int cus = 256;
+};
+
+namespace evx {
+namespace {
+
+void free_plan(evx_batch_plan* p) {
+  if (!p) return;
+  hipFree(p->d_inst);
+  hipFree(p->d_items);
+  hipFree(p->d_tw);
+  hipFree(p->d_ctr);
+  hipFree(p->d_err);
+  delete p;
+}
+
+// Tile counts of every sweep for the current x-chunk, and the item list.
+int build_items(evx_batch_plan* p) {
+  std::vector<int4> items;
+  for (int s = 0; s < NSTAGE; ++s) {
+    p->stage_first[s] = (int)items.size();
+    for (int i = 0; i < p->n; ++i) {
+      BInst& b = p->inst[i];
+      int64_t nt = 0;
+      switch (s) {
+        case 0: {
+          constexpr int BY = NTY * TileCfg<T>::PY;
+          b.tg.ntz = (int)cdiv(b.nz, BZ);
+          b.tg.nty = (int)cdiv(b.ny, BY);
+          b.tg.xc = p->xchunk < b.nx ? p->xchunk : b.nx;
+          b.tg.nxc = (int)cdiv(b.nx, b.tg.xc);
+          nt = (int64_t)b.tg.ntz * b.tg.nty * b.tg.nxc;
+          break;
+        }
+        case 1:
+        case 5:
+          nt = cdiv((int64_t)b.nx * b.ny, lines_per_z_tile(b.nz));
+          break;
+        case 2:
+        case 4:
+          nt = cdiv((int64_t)b.nx * b.nh, lines_per_brick(b.ny));
+          break;
+        default:
+          nt = cdiv((int64_t)b.ny * b.nh, lines_per_brick(b.nx));
+          break;
+      }
+      if (nt > (1 << 30)) return fail(EVX_ERR_UNSUPPORTED, "too many tiles in one sweep");
+      b.nt[s] = (int)nt;
+      for (int t = 0; t < (int)nt; ++t) items.push_back(int4{i, s, t, 0});
+    }
+  }
+  p->stage_first[NSTAGE] = (int)items.size();
+  p->n_items = (int)items.size();
+  hipFree(p->d_items);
+  p->d_items = nullptr;
+  if (hipMalloc(&p->d_items, sizeof(int4) * items.size()) != hipSuccess)
+    return fail(EVX_ERR_NOMEM, "hipMalloc of the batched item list failed");
+  if (hipMemcpy(p->d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(EVX_ERR_HIP, "upload of the batched item list failed");
+  if (hipMemcpy(p->d_inst, p->inst.data(), sizeof(BInst) * p->n, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(EVX_ERR_HIP, "upload of the batched grid table failed");
+  return EVX_OK;
+}
+
+BArgs make_args(evx_batch_plan* p, void* state, void* ws) {
+  BArgs a;
+  a.inst = p->d_inst;
+  a.items = p->d_items;
+  a.n_items = p->n_items;
+  a.n_inst = p->n;
+  a.u = static_cast<T*>(state);
+  a.rhs = static_cast<T*>(ws);
+  a.spec = reinterpret_cast<C*>(static_cast<char*>(ws) + align_up(sizeof(T) * p->state_elems));
+  a.tw = p->d_tw;
+  a.ctr = p->d_ctr;
+  a.err = p->d_err;
+  return a;
+}
+
+int launch_step(evx_batch_plan* p, void* state, void* ws, hipStream_t s) {
+  const BArgs a = make_args(p, state, ws);
+  if (p->mode == 1) {
+    const int blocks = p->blocks > 0 ? p->blocks : 3 * p->cus;
+    hipLaunchKernelGGL(batch_persistent, dim3(blocks), dim3(BT), 0, s, a);
+    return check_launch("batch_persistent");
+  }
+  for (int st = 0; st < NSTAGE; ++st) {
+    const int n = p->stage_first[st + 1] - p->stage_first[st];
+    if (n == 0) continue;
+    hipLaunchKernelGGL(batch_stage, dim3(n), dim3(BT), 0, s, a, p->stage_first[st]);
+    EVX_TRY(check_launch("batch_stage"));
+  }
+  return EVX_OK;
+}
+
+}  // namespace
+}  // namespace evx
+
+using namespace evx;
+
+extern "C" int evx_batch_plan_create(evx_batch_plan** out, int32_t dtype, int32_t n_inst, const int64_t* shapes,
+                                     const double* spacing) {
+  if (!out || !shapes || !spacing) return fail(EVX_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (dtype != EVX_F32) return fail(EVX_ERR_UNSUPPORTED, "batched steps are built for float32");
+  if (n_inst < 1 || n_inst > 4096) return fail(EVX_ERR_INVALID, "n_inst must be in [1, 4096]");
+  auto* p = new evx_batch_plan();
+  p->n = n_inst;
+  p->inst.resize(n_inst);
+  // twiddle tables, one per distinct size
+  std::vector<int> sizes, offs;
+  std::vector<C> tw;
+  auto tw_of = [&](int n) -> int {
+    for (size_t i = 0; i < sizes.size(); ++i)
+      if (sizes[i] == n) return offs[i];
+    sizes.push_back(n);
+    offs.push_back((int)tw.size());
+    const long double tau = 6.283185307179586476925286766559005768L;
+    for (int m = 0; m < n; ++m) {
+      const long double ang = tau * (long double)m / (long double)n;
+      tw.push_back(C{static_cast<T>(cosl(ang)), static_cast<T>(-sinl(ang))});
+    }
+    return offs.back();
+  };
+  int64_t u_off = 0, s_off = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    BInst& b = p->inst[i];
+    const int64_t nx = shapes[3 * i], ny = shapes[3 * i + 1], nz = shapes[3 * i + 2];
+    if (!batch_size_ok(nx) || !batch_size_ok(ny) || !batch_size_ok(nz)) {
+      free_plan(p);
+      return fail(EVX_ERR_UNSUPPORTED, "grid %d: edges must be in {16, 32, 48, 64, 96, 128}, got (%lld, %lld, %lld)",
+                  i, (long long)nx, (long long)ny, (long long)nz);
+    }
+    for (int a = 0; a < 3; ++a)
+      if (!(spacing[3 * i + a] > 0.0)) {
+        free_plan(p);
+        return fail(EVX_ERR_INVALID, "grid %d: spacing must be positive", i);
+      }
+    b.nx = (int)nx;
+    b.ny = (int)ny;
+    b.nz = (int)nz;
+    b.nh = (int)(nz / 2 + 1);
+    b.u_off = u_off;
+    b.r_off = u_off;
+    b.s_off = s_off;
+    u_off += (int64_t)(align_up(sizeof(T) * nx * ny * nz) / sizeof(T));
+    s_off += (int64_t)(align_up(sizeof(C) * nx * ny * b.nh) / sizeof(C));
+    b.tw[0] = tw_of(b.nx);
+    b.tw[1] = tw_of(b.ny);
+    b.tw[2] = tw_of(b.nz);
+    double dd[3], dd2[3];
+    for (int a = 0; a < 3; ++a) {
+      const float sp = static_cast<float>(spacing[3 * i + a]);
+      dd[a] = 1.0f / sp;
+      dd2[a] = 1.0f / (sp * sp);
+    }
+    b.g = make_geo_from<T>(dd, dd2);
+    Prefac& pf = b.pf;
+    std::memset(&pf, 0, sizeof(pf));
+    pf.ch = 1;
+    pf.twopi = static_cast<float>(2.0 * M_PI);
+    const int64_t n3[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a) pf.inv_nd[a] = static_cast<float>(1.0 / ((double)n3[a] * spacing[3 * i + a]));
+    pf.nx = b.nx;
+    pf.ny = b.ny;
+    pf.nz = b.nz;
+    pf.scale = 1.0 / ((double)nx * (double)ny * (double)nz);
+  }
+  p->state_elems = u_off;
+  p->ws_bytes = align_up(sizeof(T) * u_off) + align_up(sizeof(C) * s_off);
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    p->cus = prop.multiProcessorCount;
+  const size_t nctr = 2 + (size_t)NSTAGE * n_inst;
+  if (hipMalloc(&p->d_inst, sizeof(BInst) * n_inst) != hipSuccess ||
+      hipMalloc(&p->d_tw, sizeof(C) * tw.size()) != hipSuccess ||
+      hipMalloc(&p->d_ctr, sizeof(unsigned) * nctr) != hipSuccess || hipMalloc(&p->d_err, sizeof(int)) != hipSuccess) {
+    free_plan(p);
+    return fail(EVX_ERR_NOMEM, "hipMalloc of batched plan tables failed");
+  }
+  if (hipMemcpy(p->d_tw, tw.data(), sizeof(C) * tw.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(p->d_ctr, 0, sizeof(unsigned) * nctr) != hipSuccess || hipMemset(p->d_err, 0, sizeof(int)) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess) {
+    free_plan(p);
+    return fail(EVX_ERR_HIP, "batched plan upload failed");
+  }
+  const int st = build_items(p);
+  if (st != EVX_OK) {
+    free_plan(p);
+    return st;
+  }
+  *out = p;
+  return EVX_OK;
+}
+
+extern "C" int evx_batch_plan_destroy(evx_batch_plan* p) {
+  free_plan(p);
+  return EVX_OK;
+}
+
+extern "C" int evx_batch_state_layout(const evx_batch_plan* p, int64_t* offsets, int64_t* total) {
+  if (!p || !total) return fail(EVX_ERR_INVALID, "null argument");
+  if (offsets)
+    for (int i = 0; i < p->n; ++i) offsets[i] = p->inst[i].u_off;
+  *total = p->state_elems;
+  return EVX_OK;
+}
+
+extern "C" int evx_batch_workspace_bytes(const evx_batch_plan* p, size_t* bytes) {
+  if (!p || !bytes) return fail(EVX_ERR_INVALID, "null argument");
+  *bytes = p->ws_bytes;
+  return EVX_OK;
+}
+
+extern "C" int evx_batch_set_params(evx_batch_plan* p, const double* eps_d, double dt, double A) {
+  if (!p || !eps_d) return fail(EVX_ERR_INVALID, "null argument");
+  for (int i = 0; i < p->n; ++i) {
+    const double eps = eps_d[2 * i], D = eps_d[2 * i + 1];
+    if (!(eps != 0.0)) return fail(EVX_ERR_INVALID, "grid %d: eps must be nonzero", i);
+    BInst& b = p->inst[i];
+    b.k = ChCoef<T>{static_cast<T>(18.0 / eps), static_cast<T>(2.0 * eps), static_cast<T>(D)};
+    b.pf.sc = static_cast<float>(((-2.0 * eps) * D) * A);  // problem_definition.py:303
+    b.pf.dt = static_cast<float>(dt);
+    b.pf.chan = nullptr;
+    b.pf.A = A;
+  }
+  if (hipMemcpy(p->d_inst, p->inst.data(), sizeof(BInst) * p->n, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(EVX_ERR_HIP, "upload of the batched grid table failed");
+  p->params_set = true;
+  return EVX_OK;
+}
+
+extern "C" int evx_batch_set_schedule(evx_batch_plan* p, int mode, int blocks, int xchunk) {
+  if (!p) return fail(EVX_ERR_INVALID, "null plan");
+  if (mode != 0 && mode != 1) return fail(EVX_ERR_INVALID, "mode must be 0 (staged) or 1 (persistent)");
+  if (blocks < 0 || blocks > 65535) return fail(EVX_ERR_INVALID, "blocks must be in [0, 65535]");
+  if (xchunk < 1 || xchunk > 4096) return fail(EVX_ERR_INVALID, "xchunk must be in [1, 4096]");
+  // The previous launch of this plan must be complete before the item list
+  // changes (the caller synchronises; set-up time only).
+  p->mode = mode;
+  p->blocks = blocks;
+  if (xchunk != p->xchunk) {
+    p->xchunk = xchunk;
+    EVX_TRY(build_items(p));
+  }
+  return EVX_OK;
+}
+
+extern "C" int evx_batch_get_schedule(const evx_batch_plan* p, int* mode, int* blocks, int* xchunk, int* n_items) {
+  if (!p || !mode || !blocks || !xchunk || !n_items) return fail(EVX_ERR_INVALID, "null argument");
+  *mode = p->mode;
+  *blocks = p->blocks > 0 ? p->blocks : 3 * p->cus;
+  *xchunk = p->xchunk;
+  *n_items = p->n_items;
+  return EVX_OK;
+}
+
+extern "C" int evx_batch_ch_imex_step(evx_batch_plan* p, void* state, void* workspace, void* stream) {
+  if (!p || !state || !workspace) return fail(EVX_ERR_INVALID, "null argument");
+  if (!p->params_set) return fail(EVX_ERR_INVALID, "evx_batch_set_params has not been called");
+  return launch_step(p, state, workspace, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int evx_batch_error_flag(evx_batch_plan* p, int* flag) {
+  if (!p || !flag) return fail(EVX_ERR_INVALID, "null argument");
+  if (hipMemcpy(flag, p->d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(EVX_ERR_HIP, "reading the batched error flag failed");
+  return EVX_OK;
+}
+
+// Autotuner: times each candidate schedule (mode, workgroups, x-chunk) over
+// `steps` steps with HIP events after one warm-up step, and keeps the
+// fastest.  The candidates run on the caller's scratch state, which is
+// overwritten.  Synchronises the stream (set-up time only).
+extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* workspace, int steps, void* stream,
+                                  double* best_ms) {
+  if (!p || !scratch_state || !workspace) return fail(EVX_ERR_INVALID, "null argument");
+  if (!p->params_set) return fail(EVX_ERR_INVALID, "evx_batch_set_params has not been called");
+  if (steps < 1 || steps > 1000) return fail(EVX_ERR_INVALID, "steps must be in [1, 1000]");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  struct Cand {
+    int mode, blocks, xchunk;
+  };
+  std::vector<Cand> cands;
+  for (int xc : {4, 8, 16, 32}) {
+    cands.push_back({0, 0, xc});
+    for (int k : {1, 2, 3}) cands.push_back({1, k * p->cus, xc});
+  }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return fail(EVX_ERR_HIP, "hipEventCreate failed");
+  double best = 1e300;
+  Cand pick = {p->mode, p->blocks, p->xchunk};
+  int st = EVX_OK;
+  for (const Cand& c : cands) {
+    if (hipStreamSynchronize(s) != hipSuccess) { st = fail(EVX_ERR_HIP, "stream synchronisation failed"); break; }
+    if ((st = evx_batch_set_schedule(p, c.mode, c.blocks, c.xchunk)) != EVX_OK) break;
+    if ((st = launch_step(p, scratch_state, workspace, s)) != EVX_OK) break;
+    hipEventRecord(e0, s);
+    for (int i = 0; i < steps && st == EVX_OK; ++i) st = launch_step(p, scratch_state, workspace, s);
+    if (st != EVX_OK) break;
+    hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess) { st = fail(EVX_ERR_HIP, "event synchronisation failed"); break; }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (ms / steps < best) {
+      best = ms / steps;
+      pick = c;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (st != EVX_OK) return st;
+  EVX_TRY(evx_batch_set_schedule(p, pick.mode, pick.blocks, pick.xchunk));
+  if (best_ms) *best_ms = best;
+  return EVX_OK;
+}

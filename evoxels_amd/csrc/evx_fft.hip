@@ -21,11 +21,16 @@
 
 #include "evx_common.hpp"
 #include "evx_fft_engine.hpp"
+#include "evx_fft_tiles.hpp"
 
 namespace evx {
 
 int stencil_rhs_ch_periodic(int32_t dtype, const void* u, void* out, const Dims& d, const double div_dx[3],
-                            const double div_dx2[3], double eps, double D, hipStream_t s);
+                            const double div_dx2[3], double eps, double D, const double* chan, hipStream_t s);
+size_t ch_rhs_vjp_ws_bytes(int32_t dtype, const Dims& d);
+int ch_rhs_vjp_launch(int32_t dtype, const void* u, const void* w, const void* base, void* grad,
+                      double* dparams, const Dims& d, const double dd[3], const double dd2[3], double eps,
+                      double D, const double* chan, const void* u1, double dt, void* ws, hipStream_t s);
 int stencil_rhs_rd_periodic(int32_t dtype, const void* u, void* out, const Dims& d, const double div_dx2[3],
                             double D, hipStream_t s);
 
@@ -219,44 +224,6 @@ __global__ void z_c2r_kernel(const cpx<T>* __restrict__ spec, T* out, const T* b
   }
 }
 
-// ---------------------------------------------------------- strided sweeps --
-// Lines along y or x of the half spectrum.  Flat line index L:
-//   outer = L / nh, k = L % nh, start = (outer / G) * GM + (outer % G) * OM + k
-// y: G = 1, GM = ny*nh            (outer = c*nx + x)
-// x: G = ny, GM = nx*ny*nh, OM = nh (outer = c*ny + y)
-struct LineMap {
-  int64_t nh, G, GM, OM, es;
-};
-
-__device__ __forceinline__ int64_t line_start(const LineMap& m, int64_t L) {
-  const int64_t outer = L / m.nh, k = L - (L / m.nh) * m.nh;
-  return (outer / m.G) * m.GM + (outer % m.G) * m.OM + k;
-}
-
-// float32 IMEX multiplier of the reference: k^2 from fftfreq (float32), the
-// symbol sc*k^2 (RD) or sc*(k^2)^2 (CH), prefac = reciprocal(1 - dt*sym)*dt.
-struct Prefac {
-  int ch;          // 1: Cahn-Hilliard k^4 symbol, 0: k^2 symbol
-  float sc;        // float32 scalar of the symbol (e.g. -2*eps*D*A)
-  float dt;        // float32(dt)
-  float twopi;     // float32(2*pi)
-  float inv_nd[3]; // float32(1/(n*d)) per axis
-  int nx, ny, nz;
-  double scale;    // 1/(nx*ny*nz) of the inverse transform
-};
-
-__device__ __forceinline__ float prefac_f32(int jx, int jy, int jz, const Prefac& p) {
-#pragma clang fp contract(off)
-  const int sx = jx < (p.nx + 1) / 2 ? jx : jx - p.nx;
-  const int sy = jy < (p.ny + 1) / 2 ? jy : jy - p.ny;
-  const float kx = p.twopi * (static_cast<float>(sx) * p.inv_nd[0]);
-  const float ky = p.twopi * (static_cast<float>(sy) * p.inv_nd[1]);
-  const float kz = p.twopi * (static_cast<float>(jz) * p.inv_nd[2]);
-  const float k2 = (kx * kx + ky * ky) + kz * kz;
-  const float sym = p.ch ? p.sc * (k2 * k2) : p.sc * k2;
-  const float den = 1.0f - p.dt * sym;
-  return (1.0f / den) * p.dt;
-}
 
 // MODE 0: forward, 1: inverse, 2: forward * prefac * inverse (x axis only)
 template <typename T, int MODE>
@@ -287,8 +254,9 @@ __global__ void strided_kernel(cpx<T>* spec, int64_t nlines, LineMap lm, Axis<T>
       const int64_t outer = L / lm.nh;
       const int jz = (int)(L - outer * lm.nh);
       const int jy = (int)(outer % pf.ny);
+      const float ssc = symbol_scalar(pf, outer / pf.ny);
       for (int j = tl; j < n; j += jstep) {
-        const T f = static_cast<T>(prefac_f32(j, jy, jz, pf)) * sc;
+        const T f = static_cast<T>(prefac_f32(j, jy, jz, pf, ssc)) * sc;
         buf[j * lpw + line] = cscale(buf[j * lpw + line], f);
       }
     }
@@ -300,183 +268,27 @@ __global__ void strided_kernel(cpx<T>* spec, int64_t nlines, LineMap lm, Axis<T>
 }
 
 
-// --------------------------------------------------------- specialised sizes --
-// Compile-time radix plans: L = lcm of the radices (minimum values per thread).
-template <int N> struct FftSpec;
-template <> struct FftSpec<8>    { using R = Rad<8>;          static constexpr int L = 8; };
-template <> struct FftSpec<16>   { using R = Rad<4, 4>;       static constexpr int L = 4; };
-template <> struct FftSpec<32>   { using R = Rad<4, 8>;       static constexpr int L = 8; };
-template <> struct FftSpec<48>   { using R = Rad<3, 4, 4>;    static constexpr int L = 12; };
-template <> struct FftSpec<64>   { using R = Rad<8, 8>;       static constexpr int L = 8; };
-template <> struct FftSpec<96>   { using R = Rad<3, 4, 8>;    static constexpr int L = 24; };
-template <> struct FftSpec<100>  { using R = Rad<4, 5, 5>;    static constexpr int L = 20; };
-template <> struct FftSpec<128>  { using R = Rad<2, 8, 8>;    static constexpr int L = 8; };
-template <> struct FftSpec<256>  { using R = Rad<4, 8, 8>;    static constexpr int L = 8; };
-template <> struct FftSpec<512>  { using R = Rad<8, 8, 8>;    static constexpr int L = 8; };
-template <> struct FftSpec<1024> { using R = Rad<2, 8, 8, 8>; static constexpr int L = 8; };
-#define EVX_FAST_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(100) X(128) X(256) X(512) X(1024)
-
-// z sweeps: one line pair per TPL threads, TPL = N/L (each thread owns L values).
-template <typename T, int N>
-struct ZGeo {
-  static constexpr int TPL = N / FftSpec<N>::L;
-  static constexpr int LP = TPL >= 256 ? 1 : 256 / TPL;  // line pairs per workgroup
-  static constexpr int THREADS = LP * TPL;
-  static constexpr int LS = N + N / 16;                   // padded LDS line
-};
-
-template <typename T, int LS>
-struct LineLay {
-  cpx<T>* base;
-  __device__ __forceinline__ cpx<T>* at(int e) const { return base + e + (e >> 4); }
-};
-
-template <typename T, int TL>
-struct BrickLay {
-  cpx<T>* base;
-  __device__ __forceinline__ cpx<T>* at(int e) const { return base + e * TL; }
-};
-
 template <typename T, int N>
 __global__ __launch_bounds__((ZGeo<T, N>::THREADS)) void z_r2c_fast(const T* __restrict__ x, cpx<T>* __restrict__ spec,
                                                                   int64_t nlines, const cpx<T>* __restrict__ twg) {
-  using G = ZGeo<T, N>;
-  constexpr int TPL = G::TPL, E = N / TPL, NH = N / 2 + 1;
-  __shared__ cpx<T> tw[N];
-  __shared__ cpx<T> lds[G::LP * G::LS];
-  const int tid = threadIdx.x, pr = tid / TPL, tl = tid - (tid / TPL) * TPL;
-  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
-  const int64_t la = ((int64_t)blockIdx.x * G::LP + pr) * 2;
-  const bool has_a = la < nlines, has_b = la + 1 < nlines;
-  const T* xa = x + la * N;
-  cpx<T> v[E];
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int e = tl + TPL * q;
-    v[q].x = has_a ? xa[e] : T(0);
-    v[q].y = has_b ? xa[N + e] : T(0);
-  }
-  __syncthreads();
-  const LineLay<T, G::LS> lay{lds + pr * G::LS};
-  fft_regs<T, N, TPL, false>(v, lay, tl, tw, typename FftSpec<N>::R{});
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < E; ++q) *lay.at(tl + TPL * q) = v[q];
-  __syncthreads();
-  cpx<T>* sa = spec + la * NH;
-#pragma unroll
-  for (int q = 0; q < (NH + TPL - 1) / TPL; ++q) {
-    const int k = tl + TPL * q;
-    if (k < NH && has_a) {
-      const cpx<T> Z = v[q];
-      const cpx<T> Zc = *lay.at(k == 0 ? 0 : N - k);
-      sa[k] = cpx<T>{(Z.x + Zc.x) * T(0.5), (Z.y - Zc.y) * T(0.5)};
-      if (has_b) sa[NH + k] = cpx<T>{(Z.y + Zc.y) * T(0.5), (Zc.x - Z.x) * T(0.5)};
-    }
-  }
-}
-
-template <typename T, int N>
-__device__ __forceinline__ cpx<T> herm_at(const cpx<T>* __restrict__ line, int k) {
-  constexpr int NH = N / 2 + 1;
-  cpx<T> v = k < NH ? line[k] : cpx<T>{line[N - k].x, -line[N - k].y};
-  if (k == 0 || 2 * k == N) v.y = T(0);
-  return v;
+  __shared__ __align__(16) unsigned char smem[ZTile<T, N>::LDS_BYTES];
+  z_r2c_tile<T, N>(x, spec, nlines, twg, blockIdx.x, threadIdx.x, smem);
 }
 
 template <typename T, int N>
 __global__ __launch_bounds__((ZGeo<T, N>::THREADS)) void z_c2r_fast(const cpx<T>* __restrict__ spec, T* out,
                                                                   const T* base, T scale, int64_t nlines,
                                                                   const cpx<T>* __restrict__ twg) {
-  using G = ZGeo<T, N>;
-  constexpr int TPL = G::TPL, E = N / TPL, NH = N / 2 + 1;
-  __shared__ cpx<T> tw[N];
-  __shared__ cpx<T> lds[G::LP * G::LS];
-  const int tid = threadIdx.x, pr = tid / TPL, tl = tid - (tid / TPL) * TPL;
-  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
-  const int64_t la = ((int64_t)blockIdx.x * G::LP + pr) * 2;
-  const bool has_a = la < nlines, has_b = la + 1 < nlines;
-  const cpx<T>* sa = spec + la * NH;
-  cpx<T> v[E];
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int k = tl + TPL * q;
-    const cpx<T> A = has_a ? herm_at<T, N>(sa, k) : cpx<T>{T(0), T(0)};
-    const cpx<T> B = has_b ? herm_at<T, N>(sa + NH, k) : cpx<T>{T(0), T(0)};
-    v[q] = cpx<T>{A.x - B.y, A.y + B.x};
-  }
-  // The state added to the update is loaded now so its latency overlaps the FFT.
-  const T* ba = base ? base + la * N : nullptr;
-  cpx<T> bv[E];
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int e = tl + TPL * q;
-    bv[q].x = (ba && has_a) ? ba[e] : T(0);
-    bv[q].y = (ba && has_b) ? ba[N + e] : T(0);
-  }
-  __syncthreads();
-  const LineLay<T, G::LS> lay{lds + pr * G::LS};
-  fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
-  T* oa = out + la * N;
-#pragma unroll
-  for (int q = 0; q < E; ++q) {
-    const int e = tl + TPL * q;
-    if (has_a) oa[e] = bv[q].x + scale * v[q].x;
-    if (has_b) oa[N + e] = bv[q].y + scale * v[q].y;
-  }
+  __shared__ __align__(16) unsigned char smem[ZTile<T, N>::LDS_BYTES];
+  z_c2r_tile<T, N>(spec, out, base, scale, nlines, twg, blockIdx.x, threadIdx.x, smem);
 }
-
-// Strided sweeps: TL adjacent lines (128-byte runs per element row), TPL
-// threads per line with E values each.
-constexpr int pick_strided_e(int n, int l, int tl0) {
-  int e = l;
-  while (n / e > 256 / tl0 && n % (e * 2) == 0) e *= 2;
-  return e;
-}
-
-template <typename T, int N>
-struct SGeo {
-  static constexpr int TL0 = 128 / (int)sizeof(cpx<T>);
-  static constexpr int E = pick_strided_e(N, FftSpec<N>::L, TL0);
-  static constexpr int TPL = N / E;
-  static constexpr int TL = TL0 > 256 / TPL ? TL0 : 256 / TPL;
-  static constexpr int THREADS = TL * TPL;
-};
 
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__((SGeo<T, N>::THREADS)) void strided_fast(cpx<T>* __restrict__ spec, int64_t nlines,
                                                                     LineMap lm, const cpx<T>* __restrict__ twg,
                                                                     Prefac pf) {
-  using G = SGeo<T, N>;
-  constexpr int TPL = G::TPL, TL = G::TL, E = N / TPL;
-  __shared__ cpx<T> tw[N];
-  __shared__ cpx<T> lds[TL * N];
-  const int tid = threadIdx.x, t = tid % TL, tl = tid / TL;
-  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
-  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int64_t L = (int64_t)tile * TL + t;
-  const bool active = L < nlines;
-  const int64_t start = active ? line_start(lm, L) : 0;
-  cpx<T> v[E];
-#pragma unroll
-  for (int q = 0; q < E; ++q)
-    v[q] = active ? spec[start + (int64_t)(tl + TPL * q) * lm.es] : cpx<T>{T(0), T(0)};
-  __syncthreads();
-  const BrickLay<T, TL> lay{lds + t};
-  fft_regs<T, N, TPL, MODE == 1>(v, lay, tl, tw, typename FftSpec<N>::R{});
-  if constexpr (MODE == 2) {
-    const int64_t outer = L / lm.nh;
-    const int jz = (int)(L - outer * lm.nh);
-    const int jy = (int)(outer % pf.ny);
-    const T sc = static_cast<T>(pf.scale);
-#pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = cscale(v[q], static_cast<T>(prefac_f32(tl + TPL * q, jy, jz, pf)) * sc);
-    fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
-  }
-  if (active) {
-#pragma unroll
-    for (int q = 0; q < E; ++q) spec[start + (int64_t)(tl + TPL * q) * lm.es] = v[q];
-  }
+  __shared__ __align__(16) unsigned char smem[STile<T, N>::LDS_BYTES];
+  strided_tile<T, N, MODE>(spec, nlines, lm, twg, pf, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, smem);
 }
 
 template <typename T>
@@ -735,6 +547,8 @@ Prefac make_prefac(const evx_fft_plan* p, int ch, double sym_scalar, double dt, 
   pf.ny = (int)p->d.ny;
   pf.nz = (int)p->d.nz;
   pf.scale = 1.0 / ((double)p->d.nx * (double)p->d.ny * (double)p->d.nz);
+  pf.chan = nullptr;
+  pf.A = 0.0;
   return pf;
 }
 
@@ -907,16 +721,18 @@ extern "C" int evx_irfftn(evx_fft_plan* p, void* spec, void* y, void* stream) {
 extern "C" int evx_ch_imex_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
                                 void* stream) {
   EVX_TRY(imex_common(p, u, u_out, ws, prm));
-  if (!(prm->eps != 0.0)) return fail(EVX_ERR_INVALID, "eps must be nonzero");
+  if (!prm->chan_params && !(prm->eps != 0.0)) return fail(EVX_ERR_INVALID, "eps must be nonzero");
   hipStream_t s = static_cast<hipStream_t>(stream);
   double dd[3], dd2[3];
   geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
   void* rhs = ws;
   void* spec = static_cast<char*>(ws) + real_bytes(p);
   // problem_definition.py:303 -> ((( -2*eps)*D)*A) as a Python double
-  const Prefac pf = make_prefac(p, 1, ((-2.0 * prm->eps) * prm->D) * prm->A, prm->dt, prm->spacing);
+  Prefac pf = make_prefac(p, 1, ((-2.0 * prm->eps) * prm->D) * prm->A, prm->dt, prm->spacing);
+  pf.chan = prm->chan_params;
+  pf.A = prm->A;
   EVX_TRY(mark(p, 0, s));
-  EVX_TRY(stencil_rhs_ch_periodic(p->dtype, u, rhs, p->d, dd, dd2, prm->eps, prm->D, s));
+  EVX_TRY(stencil_rhs_ch_periodic(p->dtype, u, rhs, p->d, dd, dd2, prm->eps, prm->D, prm->chan_params, s));
   EVX_TRY(mark(p, 1, s));
   return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
                              : spectral_update<double>(p, rhs, spec, u, u_out, pf, s);
@@ -925,6 +741,7 @@ extern "C" int evx_ch_imex_step(evx_fft_plan* p, const void* u, void* u_out, voi
 extern "C" int evx_rd_imex_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
                                 void* stream) {
   EVX_TRY(imex_common(p, u, u_out, ws, prm));
+  if (prm->chan_params) return fail(EVX_ERR_UNSUPPORTED, "per-channel parameters are implemented for CH only");
   hipStream_t s = static_cast<hipStream_t>(stream);
   double dd[3], dd2[3];
   geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
@@ -937,6 +754,39 @@ extern "C" int evx_rd_imex_step(evx_fft_plan* p, const void* u, void* u_out, voi
   const Prefac pf = make_prefac(p, 0, (-prm->D) * prm->A, prm->dt, prm->spacing);
   return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
                              : spectral_update<double>(p, rhs, spec, u, u_out, pf, s);
+}
+
+extern "C" int evx_ch_imex_vjp_workspace_bytes(const evx_fft_plan* p, size_t* bytes) {
+  EVX_TRY(check_plan(p));
+  if (!bytes) return fail(EVX_ERR_INVALID, "null argument");
+  *bytes = real_bytes(p) + spec_bytes(p) + ch_rhs_vjp_ws_bytes(p->dtype, p->d);
+  return EVX_OK;
+}
+
+// lam0 = lam1 + J^T L(lam1) and the parameter gradients; w = L(lam1) goes
+// through the same five spectral sweeps as the forward step with no base.
+extern "C" int evx_ch_imex_step_vjp(evx_fft_plan* p, const void* u0, const void* u1, const void* lam1, void* lam0,
+                                    double* dparams, void* ws, const evx_imex_params* prm, void* stream) {
+  EVX_TRY(imex_common(p, u0, lam0, ws, prm));
+  if (!u1 || !lam1 || !dparams) return fail(EVX_ERR_INVALID, "null argument to imex step vjp");
+  if (!prm->chan_params && !(prm->eps != 0.0 && prm->D != 0.0))
+    return fail(EVX_ERR_INVALID, "eps and D must be nonzero");
+  if (!(prm->dt != 0.0)) return fail(EVX_ERR_INVALID, "dt must be nonzero");
+  if (lam0 == u0 || lam0 == u1) return fail(EVX_ERR_INVALID, "lam0 must not alias u0 or u1");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  double dd[3], dd2[3];
+  geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
+  void* w = ws;
+  void* spec = static_cast<char*>(ws) + real_bytes(p);
+  void* adj = static_cast<char*>(spec) + spec_bytes(p);
+  Prefac pf = make_prefac(p, 1, ((-2.0 * prm->eps) * prm->D) * prm->A, prm->dt, prm->spacing);
+  pf.chan = prm->chan_params;
+  pf.A = prm->A;
+  p->ntrace = 0;
+  EVX_TRY(p->dtype == EVX_F32 ? spectral_update<float>(p, lam1, spec, nullptr, w, pf, s)
+                              : spectral_update<double>(p, lam1, spec, nullptr, w, pf, s));
+  return ch_rhs_vjp_launch(p->dtype, u0, w, lam1, lam0, dparams, p->d, dd, dd2, prm->eps, prm->D,
+                           prm->chan_params, u1, prm->dt, adj, s);
 }
 
 extern "C" int evx_fft_plan_trace_next_step(evx_fft_plan* p, void* const* events, int n_events) {

@@ -1,0 +1,256 @@
+// Size-specialised FFT tile bodies (z R2C / C2R line pairs, strided y / x
+// bricks) and the float32 IMEX multiplier, shared by the one-grid kernels and
+// the batched (many-grid) kernels.  Each body serves one tile with a
+// caller-provided LDS buffer and a flat thread index.
+#pragma once
+#include "evx_common.hpp"
+#include "evx_fft_engine.hpp"
+
+namespace evx {
+
+// ---------------------------------------------------------- strided sweeps --
+// Lines along y or x of the half spectrum.  Flat line index L:
+//   outer = L / nh, k = L % nh, start = (outer / G) * GM + (outer % G) * OM + k
+// y: G = 1, GM = ny*nh            (outer = c*nx + x)
+// x: G = ny, GM = nx*ny*nh, OM = nh (outer = c*ny + y)
+struct LineMap {
+  int64_t nh, G, GM, OM, es;
+};
+
+__device__ __forceinline__ int64_t line_start(const LineMap& m, int64_t L) {
+  const int64_t outer = L / m.nh, k = L - (L / m.nh) * m.nh;
+  return (outer / m.G) * m.GM + (outer % m.G) * m.OM + k;
+}
+
+// float32 IMEX multiplier of the reference: k^2 from fftfreq (float32), the
+// symbol sc*k^2 (RD) or sc*(k^2)^2 (CH), prefac = reciprocal(1 - dt*sym)*dt.
+struct Prefac {
+  int ch;          // 1: Cahn-Hilliard k^4 symbol, 0: k^2 symbol
+  float sc;        // float32 scalar of the symbol (e.g. -2*eps*D*A)
+  float dt;        // float32(dt)
+  float twopi;     // float32(2*pi)
+  float inv_nd[3]; // float32(1/(n*d)) per axis
+  int nx, ny, nz;
+  double scale;    // 1/(nx*ny*nz) of the inverse transform
+  const double* chan;  // optional per-channel {eps, D}; sc is then made per channel
+  double A;
+};
+
+// float32 symbol scalar of channel c: the same double expression the host
+// evaluates for scalar parameters (problem_definition.py:206,303).
+__device__ __forceinline__ float symbol_scalar(const Prefac& p, int64_t c) {
+  if (!p.chan) return p.sc;
+  const double e = p.chan[2 * c], D = p.chan[2 * c + 1];
+  return p.ch ? static_cast<float>(((-2.0 * e) * D) * p.A) : static_cast<float>((-D) * p.A);
+}
+
+__device__ __forceinline__ float prefac_f32(int jx, int jy, int jz, const Prefac& p, float sc) {
+#pragma clang fp contract(off)
+  const int sx = jx < (p.nx + 1) / 2 ? jx : jx - p.nx;
+  const int sy = jy < (p.ny + 1) / 2 ? jy : jy - p.ny;
+  const float kx = p.twopi * (static_cast<float>(sx) * p.inv_nd[0]);
+  const float ky = p.twopi * (static_cast<float>(sy) * p.inv_nd[1]);
+  const float kz = p.twopi * (static_cast<float>(jz) * p.inv_nd[2]);
+  const float k2 = (kx * kx + ky * ky) + kz * kz;
+  const float sym = p.ch ? sc * (k2 * k2) : sc * k2;
+  const float den = 1.0f - p.dt * sym;
+  return (1.0f / den) * p.dt;
+}
+
+// --------------------------------------------------------- specialised sizes --
+// Compile-time radix plans: L = lcm of the radices (minimum values per thread).
+template <int N> struct FftSpec;
+template <> struct FftSpec<8>    { using R = Rad<8>;          static constexpr int L = 8; };
+template <> struct FftSpec<16>   { using R = Rad<4, 4>;       static constexpr int L = 4; };
+template <> struct FftSpec<32>   { using R = Rad<4, 8>;       static constexpr int L = 8; };
+template <> struct FftSpec<48>   { using R = Rad<3, 4, 4>;    static constexpr int L = 12; };
+template <> struct FftSpec<64>   { using R = Rad<8, 8>;       static constexpr int L = 8; };
+template <> struct FftSpec<96>   { using R = Rad<3, 4, 4, 2>; static constexpr int L = 12; };
+template <> struct FftSpec<100>  { using R = Rad<4, 5, 5>;    static constexpr int L = 20; };
+template <> struct FftSpec<128>  { using R = Rad<2, 8, 8>;    static constexpr int L = 8; };
+template <> struct FftSpec<256>  { using R = Rad<4, 8, 8>;    static constexpr int L = 8; };
+template <> struct FftSpec<512>  { using R = Rad<8, 8, 8>;    static constexpr int L = 8; };
+template <> struct FftSpec<1024> { using R = Rad<2, 8, 8, 8>; static constexpr int L = 8; };
+#define EVX_FAST_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(100) X(128) X(256) X(512) X(1024)
+
+// z sweeps: one line pair per TPL threads, TPL = N/L (each thread owns L values).
+template <typename T, int N>
+struct ZGeo {
+  static constexpr int TPL = N / FftSpec<N>::L;
+  static constexpr int LP = TPL >= 256 ? 1 : 256 / TPL;  // line pairs per workgroup
+  static constexpr int THREADS = LP * TPL;
+  static constexpr int LS = N + N / 16;                   // padded LDS line
+};
+
+template <typename T, int LS>
+struct LineLay {
+  cpx<T>* base;
+  __device__ __forceinline__ cpx<T>* at(int e) const { return base + e + (e >> 4); }
+};
+
+template <typename T, int TL>
+struct BrickLay {
+  cpx<T>* base;
+  __device__ __forceinline__ cpx<T>* at(int e) const { return base + e * TL; }
+};
+
+// LDS of one z tile: twiddles plus LP padded lines.
+template <typename T, int N>
+struct ZTile {
+  static constexpr size_t LDS_BYTES = sizeof(cpx<T>) * (N + ZGeo<T, N>::LP * ZGeo<T, N>::LS);
+};
+
+// z R2C of tile `tile` (LP line pairs): two real lines per complex FFT.
+template <typename T, int N>
+__device__ __forceinline__ void z_r2c_tile(const T* __restrict__ x, cpx<T>* __restrict__ spec, int64_t nlines,
+                                           const cpx<T>* __restrict__ twg, int64_t tile, int tid,
+                                           unsigned char* smem) {
+  using G = ZGeo<T, N>;
+  constexpr int TPL = G::TPL, E = N / TPL, NH = N / 2 + 1;
+  cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* lds = tw + N;
+  const int pr = tid / TPL, tl = tid - (tid / TPL) * TPL;
+  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
+  const int64_t la = (tile * G::LP + pr) * 2;
+  const bool has_a = la < nlines, has_b = la + 1 < nlines;
+  const T* xa = x + la * N;
+  cpx<T> v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = tl + TPL * q;
+    v[q].x = has_a ? xa[e] : T(0);
+    v[q].y = has_b ? xa[N + e] : T(0);
+  }
+  __syncthreads();
+  const LineLay<T, G::LS> lay{lds + pr * G::LS};
+  fft_regs<T, N, TPL, false>(v, lay, tl, tw, typename FftSpec<N>::R{});
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < E; ++q) *lay.at(tl + TPL * q) = v[q];
+  __syncthreads();
+  cpx<T>* sa = spec + la * NH;
+#pragma unroll
+  for (int q = 0; q < (NH + TPL - 1) / TPL; ++q) {
+    const int k = tl + TPL * q;
+    if (k < NH && has_a) {
+      const cpx<T> Z = v[q];
+      const cpx<T> Zc = *lay.at(k == 0 ? 0 : N - k);
+      sa[k] = cpx<T>{(Z.x + Zc.x) * T(0.5), (Z.y - Zc.y) * T(0.5)};
+      if (has_b) sa[NH + k] = cpx<T>{(Z.y + Zc.y) * T(0.5), (Zc.x - Z.x) * T(0.5)};
+    }
+  }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ cpx<T> herm_at(const cpx<T>* __restrict__ line, int k) {
+  constexpr int NH = N / 2 + 1;
+  cpx<T> v = k < NH ? line[k] : cpx<T>{line[N - k].x, -line[N - k].y};
+  if (k == 0 || 2 * k == N) v.y = T(0);
+  return v;
+}
+
+// z C2R of tile `tile`: out = base + scale * irfft(spec) (base may be NULL,
+// out may alias base).
+template <typename T, int N>
+__device__ __forceinline__ void z_c2r_tile(const cpx<T>* __restrict__ spec, T* out, const T* base, T scale,
+                                           int64_t nlines, const cpx<T>* __restrict__ twg, int64_t tile, int tid,
+                                           unsigned char* smem) {
+  using G = ZGeo<T, N>;
+  constexpr int TPL = G::TPL, E = N / TPL, NH = N / 2 + 1;
+  cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* lds = tw + N;
+  const int pr = tid / TPL, tl = tid - (tid / TPL) * TPL;
+  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
+  const int64_t la = (tile * G::LP + pr) * 2;
+  const bool has_a = la < nlines, has_b = la + 1 < nlines;
+  const cpx<T>* sa = spec + la * NH;
+  cpx<T> v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int k = tl + TPL * q;
+    const cpx<T> A = has_a ? herm_at<T, N>(sa, k) : cpx<T>{T(0), T(0)};
+    const cpx<T> B = has_b ? herm_at<T, N>(sa + NH, k) : cpx<T>{T(0), T(0)};
+    v[q] = cpx<T>{A.x - B.y, A.y + B.x};
+  }
+  // The state added to the update is loaded now so its latency overlaps the FFT.
+  const T* ba = base ? base + la * N : nullptr;
+  cpx<T> bv[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = tl + TPL * q;
+    bv[q].x = (ba && has_a) ? ba[e] : T(0);
+    bv[q].y = (ba && has_b) ? ba[N + e] : T(0);
+  }
+  __syncthreads();
+  const LineLay<T, G::LS> lay{lds + pr * G::LS};
+  fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
+  T* oa = out + la * N;
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int e = tl + TPL * q;
+    if (has_a) oa[e] = bv[q].x + scale * v[q].x;
+    if (has_b) oa[N + e] = bv[q].y + scale * v[q].y;
+  }
+}
+
+// Strided sweeps: TL adjacent lines (128-byte runs per element row), TPL
+// threads per line with E values each.
+constexpr int pick_strided_e(int n, int l, int tl0) {
+  int e = l;
+  while (n / e > 256 / tl0 && n % (e * 2) == 0) e *= 2;
+  return e;
+}
+
+template <typename T, int N>
+struct SGeo {
+  static constexpr int TL0 = 128 / (int)sizeof(cpx<T>);
+  static constexpr int E = pick_strided_e(N, FftSpec<N>::L, TL0);
+  static constexpr int TPL = N / E;
+  static constexpr int TL = TL0 > 256 / TPL ? TL0 : 256 / TPL;
+  static constexpr int THREADS = TL * TPL;
+};
+
+template <typename T, int N>
+struct STile {
+  static constexpr size_t LDS_BYTES = sizeof(cpx<T>) * (N + SGeo<T, N>::TL * N);
+};
+
+// Strided sweep of brick `tile` (TL adjacent lines).  MODE 0: forward,
+// 1: inverse, 2: forward * prefac * inverse (x axis only).
+template <typename T, int N, int MODE>
+__device__ __forceinline__ void strided_tile(cpx<T>* __restrict__ spec, int64_t nlines, const LineMap& lm,
+                                             const cpx<T>* __restrict__ twg, const Prefac& pf, int64_t tile,
+                                             int tid, unsigned char* smem) {
+  using G = SGeo<T, N>;
+  constexpr int TPL = G::TPL, TL = G::TL, E = N / TPL;
+  cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* lds = tw + N;
+  const int t = tid % TL, tl = tid / TL;
+  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
+  const int64_t L = tile * TL + t;
+  const bool active = L < nlines;
+  const int64_t start = active ? line_start(lm, L) : 0;
+  cpx<T> v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q)
+    v[q] = active ? spec[start + (int64_t)(tl + TPL * q) * lm.es] : cpx<T>{T(0), T(0)};
+  __syncthreads();
+  const BrickLay<T, TL> lay{lds + t};
+  fft_regs<T, N, TPL, MODE == 1>(v, lay, tl, tw, typename FftSpec<N>::R{});
+  if constexpr (MODE == 2) {
+    const int64_t outer = L / lm.nh;
+    const int jz = (int)(L - outer * lm.nh);
+    const int jy = (int)(outer % pf.ny);
+    const T sc = static_cast<T>(pf.scale);
+    const float ssc = symbol_scalar(pf, active ? outer / pf.ny : 0);
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cscale(v[q], static_cast<T>(prefac_f32(tl + TPL * q, jy, jz, pf, ssc)) * sc);
+    fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
+  }
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) spec[start + (int64_t)(tl + TPL * q) * lm.es] = v[q];
+  }
+}
+
+}  // namespace evx

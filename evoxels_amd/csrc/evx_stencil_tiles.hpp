@@ -1,0 +1,417 @@
+// Tile machinery of the 2.5-D streaming stencils and the Cahn-Hilliard
+// right-hand-side tile body, shared by the one-grid kernels and the batched
+// (many-grid) kernels.
+#pragma once
+#include "evx_ch_math.hpp"
+#include "evx_common.hpp"
+#include "evx_fft_engine.hpp"
+
+namespace evx {
+
+constexpr int BZ = 64;   // threads (and tile width) along z
+constexpr int NTY = 4;   // thread rows along y
+constexpr int NT = BZ * NTY;
+
+template <typename T>
+struct TileCfg;
+template <>
+struct TileCfg<float> { static constexpr int PY = 4; };   // BY = 16
+template <>
+struct TileCfg<double> { static constexpr int PY = 2; };  // BY = 8
+
+struct Epi {
+  int mode;
+  double coef;
+  const void* base;
+  const void* acc_in;
+  void* acc_out;
+};
+
+// Writes the fused output for right-hand side value r at flat index i.
+template <typename T>
+__device__ __forceinline__ void epilogue(const Epi& ep, const T* __restrict__ u,
+                                         T* __restrict__ out, int64_t i, T r) {
+#pragma clang fp contract(off)
+  const T c = static_cast<T>(ep.coef);
+  switch (ep.mode) {
+    case EVX_OUT_RHS:
+      out[i] = r;
+      break;
+    case EVX_OUT_EULER:
+      out[i] = u[i] + c * r;
+      break;
+    case EVX_OUT_RK4_FIRST: {
+      const T* b = ep.base ? static_cast<const T*>(ep.base) : u;
+      out[i] = b[i] + c * r;
+      static_cast<T*>(ep.acc_out)[i] = r;
+      break;
+    }
+    case EVX_OUT_RK4_MID: {
+      const T* b = ep.base ? static_cast<const T*>(ep.base) : u;
+      const T a = static_cast<const T*>(ep.acc_in)[i];
+      out[i] = b[i] + c * r;
+      static_cast<T*>(ep.acc_out)[i] = a + T(2) * r;
+      break;
+    }
+    default: {  // EVX_OUT_RK4_LAST
+      const T* b = ep.base ? static_cast<const T*>(ep.base) : u;
+      const T a = static_cast<const T*>(ep.acc_in)[i];
+      out[i] = b[i] + c * (a + r);
+      break;
+    }
+  }
+}
+
+// As epilogue(), with the state value at i supplied by the caller (no reload).
+template <typename T>
+__device__ __forceinline__ void epilogue_v(const Epi& ep, T u_i, T* __restrict__ out, int64_t i, T r) {
+#pragma clang fp contract(off)
+  const T c = static_cast<T>(ep.coef);
+  switch (ep.mode) {
+    case EVX_OUT_RHS:
+      out[i] = r;
+      break;
+    case EVX_OUT_EULER:
+      out[i] = u_i + c * r;
+      break;
+    case EVX_OUT_RK4_FIRST: {
+      const T b = ep.base ? static_cast<const T*>(ep.base)[i] : u_i;
+      out[i] = b + c * r;
+      static_cast<T*>(ep.acc_out)[i] = r;
+      break;
+    }
+    case EVX_OUT_RK4_MID: {
+      const T b = ep.base ? static_cast<const T*>(ep.base)[i] : u_i;
+      const T a = static_cast<const T*>(ep.acc_in)[i];
+      out[i] = b + c * r;
+      static_cast<T*>(ep.acc_out)[i] = a + T(2) * r;
+      break;
+    }
+    default: {
+      const T b = ep.base ? static_cast<const T*>(ep.base)[i] : u_i;
+      const T a = static_cast<const T*>(ep.acc_in)[i];
+      out[i] = b + c * (a + r);
+      break;
+    }
+  }
+}
+
+template <typename T>
+__host__ Geo<T> make_geo(const evx_grid* g) {
+  return make_geo_from<T>(g->div_dx, g->div_dx2);
+}
+
+// Register-staged loader of one x-plane tile [y0-H, y0+BY+H) x [z0-H, z0+BZ+H).
+// init() resolves, once per block, the y/z part of every element this thread
+// loads: periodic axes wrap, non-periodic ones clamp and remember the ghost
+// rule.  fetch() then only resolves the (uniform) x index of the plane and
+// issues one load per element; commit() applies clip and the ghost rules and
+// writes the element to LDS slot tid + k*NT of the plane.
+template <typename T, int H, int ROWS, int W, bool CLIP>
+struct PlaneLoader {
+  static constexpr int NPT = (ROWS * W + NT - 1) / NT;
+  int off[NPT];      // in-plane offset y*nz + z of the source point, -1 if unused
+  uint8_t yzf[NPT];  // y/z ghost flags: 4/8 = y low/high, 16/32 = z low/high
+  T v[NPT];
+  uint8_t xf;        // x ghost flag of the fetched plane: 1 = low, 2 = high
+
+  __device__ __forceinline__ void init(int64_t y0, int64_t z0, const Dims& d, const Bc& bc, int tid) {
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int i = tid + k * NT;
+      off[k] = -1;
+      yzf[k] = 0;
+      if (i < ROWS * W) {
+        int64_t gy = y0 - H + i / W, gz = z0 - H + i % W;
+        uint8_t f = 0;
+        if (gy < 0 || gy >= d.ny) {
+          if (bc.kind[1] == EVX_BC_PERIODIC) gy = wrap_near(gy, d.ny);
+          else { f |= gy < 0 ? 4 : 8; gy = gy < 0 ? 0 : d.ny - 1; }
+        }
+        if (gz < 0 || gz >= d.nz) {
+          if (bc.kind[2] == EVX_BC_PERIODIC) gz = wrap_near(gz, d.nz);
+          else { f |= gz < 0 ? 16 : 32; gz = gz < 0 ? 0 : d.nz - 1; }
+        }
+        off[k] = (int)(gy * d.nz + gz);
+        yzf[k] = f;
+      }
+    }
+  }
+
+  __device__ __forceinline__ void fetch(const T* __restrict__ uc, int64_t xg, const Dims& d, const Bc& bc) {
+    xf = 0;
+    if (xg < 0 || xg >= d.nx) {
+      if (bc.kind[0] == EVX_BC_PERIODIC) xg = wrap_near(xg, d.nx);
+      else { xf = xg < 0 ? 1 : 2; xg = xg < 0 ? 0 : d.nx - 1; }
+    }
+    const T* pl = uc + xg * (d.ny * d.nz);
+#pragma unroll
+    for (int k = 0; k < NPT; ++k)
+      if (off[k] >= 0) v[k] = pl[off[k]];
+  }
+
+  __device__ __forceinline__ void commit(T* flat, const Bc& bc, int tid) const {
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      if (off[k] >= 0) {
+        T x = CLIP ? clip01(v[k]) : v[k];
+        const int f = xf | yzf[k];
+        if (f) {
+          const int8_t o[3] = {(int8_t)((f & 1) ? -1 : (f & 2) ? 1 : 0), (int8_t)((f & 4) ? -1 : (f & 8) ? 1 : 0),
+                               (int8_t)((f & 16) ? -1 : (f & 32) ? 1 : 0)};
+          x = apply_rules(x, o, bc);
+        }
+        flat[tid + k * NT] = x;
+      }
+    }
+  }
+};
+
+// Ring slot of a plane at small relative index rel >= -3 (32-bit, constant modulus).
+template <int NS>
+__device__ __forceinline__ int ring(int64_t rel) { return (int)((unsigned)((int)rel + 3 * NS) % NS); }
+
+// Flat launch: tiles ordered z-tile fastest, then y-tile, then x-chunk and
+// channel; xcd_tile() gives each XCD a contiguous range so y/z halos of
+// neighbouring tiles are shared through one L2.
+struct TileGrid {
+  int ntz, nty, nxc, xc;
+};
+
+struct TileOrigin {
+  int64_t z0, y0, x0, x1, ch;
+};
+
+template <int BY>
+__device__ __forceinline__ TileOrigin tile_origin(const TileGrid& tg, const Dims& d, unsigned t) {
+  TileOrigin o;
+  o.z0 = (int64_t)(t % tg.ntz) * BZ;
+  t /= tg.ntz;
+  o.y0 = (int64_t)(t % tg.nty) * BY;
+  t /= tg.nty;
+  o.x0 = (int64_t)(t % tg.nxc) * tg.xc;
+  o.ch = t / tg.nxc;
+  o.x1 = min(o.x0 + (int64_t)tg.xc, d.nx);
+  return o;
+}
+
+__host__ inline int pick_xchunk(const Dims& d, int64_t tiles_yz) {
+  // Enough blocks to fill 256 CUs a few times over, but at least 8 planes
+  // per block so the warm-up planes stay a small fraction of the traffic.
+  const int64_t target = 2048;
+  int64_t chunks = (target + tiles_yz * d.c - 1) / (tiles_yz * d.c);
+  if (chunks < 1) chunks = 1;
+  int64_t xc = (d.nx + chunks - 1) / chunks;
+  if (xc < 8) xc = 8;
+  if (xc > d.nx) xc = d.nx;
+  return (int)xc;
+}
+
+// ---------------------------------------------------------------------------
+// Cahn-Hilliard (problem_definition.py:328-371):
+//   c = clip(u,0,1); mu = ((18/eps*c)*(1-c))*(1-2c) - (2eps)*lap(cB)
+//   r = D * sum_axes grad_face( (cf*(1-cf)) * grad_face(muB) )
+// mu lives in a 4-plane LDS ring with a one-voxel (y,z) halo; c in a 6-plane
+// ring with a two-voxel halo.  mu ghosts (non-periodic faces) are the BC rule
+// applied to mu at the adjacent interior voxel, as pad_bc(mu) does.
+// ---------------------------------------------------------------------------
+// Thread (ty, tz) owns PY consecutive rows r0 = ty*PY .. r0+PY-1 of column tz.
+// Per output plane it evaluates mu for its own rows (sharing the column loads
+// between rows) while threads 0..H-1 evaluate the one-voxel mu halo with the
+// full ghost logic; the flux through the face between two own rows is
+// computed once, and the flux through the x face x+1/2 is carried in a
+// register to the next plane.  The c ring holds 5 planes (x-2..x+2), the mu
+// ring 3 (x-1..x+1); the plane x+3 is fetched into registers during the
+// arithmetic of plane x.
+template <typename T>
+struct ChTile {
+  static constexpr int PY = TileCfg<T>::PY, BY = NTY * PY;
+  static constexpr int CW = BZ + 4, CR = BY + 4, NC = 5;
+  static constexpr int MW = BZ + 2, MR = BY + 2, NM = 3;
+  static constexpr size_t LDS_BYTES = sizeof(T) * (NC * CR * CW + NM * MR * MW);
+};
+
+// One (y,z) tile x x-chunk of the CH right-hand side.  tid = flat thread index
+// in a 256-thread block, tile = flat tile index (see tile_origin), smem =
+// ChTile<T>::LDS_BYTES of LDS.
+template <typename T, bool STRICT>
+__device__ __forceinline__ void ch_rhs_tile(const T* __restrict__ u, T* __restrict__ out, const Dims& d,
+                                            const Bc& bc, const Geo<T>& g, ChCoef<T> k, const Epi& ep,
+                                            const TileGrid& tg, const double* __restrict__ chan, unsigned tile,
+                                            int tid, unsigned char* smem) {
+#pragma clang fp contract(off)
+  using Op = Ops<STRICT>;
+  using TC = ChTile<T>;
+  constexpr int PY = TC::PY, BY = TC::BY;
+  constexpr int CW = TC::CW, CR = TC::CR, NC = TC::NC;
+  constexpr int MW = TC::MW, MR = TC::MR, NM = TC::NM;
+  constexpr int HALO = 2 * MW + 2 * BY;  // mu halo points: rows -1, BY and columns -1, BZ
+  T(*cs)[CR][CW] = reinterpret_cast<T(*)[CR][CW]>(smem);
+  T(*ms)[MR][MW] = reinterpret_cast<T(*)[MR][MW]>(smem + sizeof(T) * NC * CR * CW);
+  const int tz = tid % BZ, ty = tid / BZ;
+  const TileOrigin o = tile_origin<BY>(tg, d, tile);
+  const int64_t z0 = o.z0, y0 = o.y0, x0 = o.x0, x1 = o.x1;
+  const T* uc = u + o.ch * d.vol();
+  T* oc = out + o.ch * d.vol();
+  if (chan) {  // per-channel (eps, D), rounded as the host rounds the scalars
+    const double e = chan[2 * o.ch], dd = chan[2 * o.ch + 1];
+    k = ChCoef<T>{static_cast<T>(18.0 / e), static_cast<T>(2.0 * e), static_cast<T>(dd)};
+  }
+  const bool xper = bc.kind[0] == EVX_BC_PERIODIC;
+  const bool yper = bc.kind[1] == EVX_BC_PERIODIC, zper = bc.kind[2] == EVX_BC_PERIODIC;
+  const int r0 = ty * PY;
+  const int64_t gz = z0 + tz;
+
+  // mu at tile point (yy, zz) of x-plane xm with every ghost rule applied.
+  auto mu_general = [&](int64_t xm, int yy, int zz) -> T {
+    int8_t oo[3] = {0, 0, 0};
+    int64_t xs = xm;
+    if (!xper && xm < 0) { oo[0] = -1; xs = 0; }
+    if (!xper && xm >= d.nx) { oo[0] = 1; xs = d.nx - 1; }
+    const T(*Cm)[CW] = cs[ring<NC>(xs - 1 - x0 + 2)];
+    const T(*C0)[CW] = cs[ring<NC>(xs - x0 + 2)];
+    const T(*Cp)[CW] = cs[ring<NC>(xs + 1 - x0 + 2)];
+    const int64_t gy = y0 + yy, gzz = z0 + zz;
+    int a = yy + 2, b = zz + 2;
+    if (!yper && (gy < 0 || gy >= d.ny)) {
+      oo[1] = gy < 0 ? -1 : 1;
+      a += gy < 0 ? 1 : -1;
+    }
+    if (!zper && (gzz < 0 || gzz >= d.nz)) {
+      oo[2] = gzz < 0 ? -1 : 1;
+      b += gzz < 0 ? 1 : -1;
+    }
+    T m = Op::mu(C0[a][b], Cp[a][b], Cm[a][b], C0[a + 1][b], C0[a - 1][b], C0[a][b + 1], C0[a][b - 1], g, k);
+    if (oo[0] | oo[1] | oo[2]) m = apply_rules(m, oo, bc);
+    return m;
+  };
+
+  // Tiles that cut a non-periodic y/z boundary evaluate every mu point with
+  // the ghost logic; all other tiles only their halo.
+  const bool tile_partial = (!yper && y0 + BY > d.ny) || (!zper && z0 + BZ > d.nz);
+  // This thread's point of the one-voxel mu halo (rows -1, BY and columns -1,
+  // BZ, corners excluded), resolved once: c coordinates (ha, hb) of the source
+  // point and its y/z ghost flags.
+  constexpr int NHALO = 2 * MW + 2 * BY - 4;
+  int hy = 0, hz = 0, ha = 0, hb = 0;
+  int8_t hoy = 0, hoz = 0;
+  if (tid < NHALO) {
+    int h = tid;
+    if (h < 2 * (MW - 2)) {
+      hy = h < MW - 2 ? -1 : BY;
+      hz = h < MW - 2 ? h : h - (MW - 2);
+    } else {
+      h -= 2 * (MW - 2);
+      hy = h % BY;
+      hz = h < BY ? -1 : BZ;
+    }
+    ha = hy + 2;
+    hb = hz + 2;
+    const int64_t gy = y0 + hy, gzz = z0 + hz;
+    if (!yper && (gy < 0 || gy >= d.ny)) { hoy = gy < 0 ? -1 : 1; ha += gy < 0 ? 1 : -1; }
+    if (!zper && (gzz < 0 || gzz >= d.nz)) { hoz = gzz < 0 ? -1 : 1; hb += gzz < 0 ? 1 : -1; }
+  }
+
+  auto compute_mu = [&](int64_t xm) {
+    T(*M)[MW] = ms[ring<NM>(xm - x0 + 1)];
+    const bool general = tile_partial || (!xper && (xm < 0 || xm >= d.nx));
+    if (!general) {
+      const T(*Cm)[CW] = cs[ring<NC>(xm - 1 - x0 + 2)];
+      const T(*C0)[CW] = cs[ring<NC>(xm - x0 + 2)];
+      const T(*Cp)[CW] = cs[ring<NC>(xm + 1 - x0 + 2)];
+      T col[PY + 2];
+#pragma unroll
+      for (int j = 0; j < PY + 2; ++j) col[j] = C0[r0 + j + 1][tz + 2];
+#pragma unroll
+      for (int q = 0; q < PY; ++q) {
+        const int a = r0 + q + 2, b = tz + 2;
+        M[r0 + q + 1][tz + 1] =
+            Op::mu(col[q + 1], Cp[a][b], Cm[a][b], col[q + 2], col[q], C0[a][b + 1], C0[a][b - 1], g, k);
+      }
+      if (tid < NHALO) {
+        T m = Op::mu(C0[ha][hb], Cp[ha][hb], Cm[ha][hb], C0[ha + 1][hb], C0[ha - 1][hb], C0[ha][hb + 1],
+                     C0[ha][hb - 1], g, k);
+        if (hoy | hoz) {
+          const int8_t oo[3] = {0, hoy, hoz};
+          m = apply_rules(m, oo, bc);
+        }
+        M[hy + 1][hz + 1] = m;
+      }
+    } else {
+      for (int p = tid; p < BY * BZ + NHALO; p += NT) {
+        int yy, zz;
+        if (p < BY * BZ) {
+          yy = p / BZ;
+          zz = p % BZ;
+        } else {
+          int h = p - BY * BZ;
+          if (h < 2 * (MW - 2)) {
+            yy = h < MW - 2 ? -1 : BY;
+            zz = h < MW - 2 ? h : h - (MW - 2);
+          } else {
+            h -= 2 * (MW - 2);
+            yy = h % BY;
+            zz = h < BY ? -1 : BZ;
+          }
+        }
+        M[yy + 1][zz + 1] = mu_general(xm, yy, zz);
+      }
+    }
+  };
+
+  PlaneLoader<T, 2, CR, CW, true> pl;
+  pl.init(y0, z0, d, bc, tid);
+  for (int64_t xg = x0 - 2; xg <= x0 + 2; ++xg) {
+    pl.fetch(uc, xg, d, bc);
+    pl.commit(&cs[ring<NC>(xg - x0 + 2)][0][0], bc, tid);
+  }
+  if (x0 + 3 <= x1 + 1) pl.fetch(uc, x0 + 3, d, bc);
+  __syncthreads();
+  compute_mu(x0 - 1);
+  compute_mu(x0);
+  __syncthreads();  // the first commit below reuses the slot mu(x0-1) read
+  const int64_t plane = d.ny * d.nz;
+  T carry[PY];  // flux through the x face x-1/2 of each own row
+  for (int64_t x = x0; x < x1; ++x) {
+    // plane x+3 (fetched one iteration ago) goes to the slot of plane x-2;
+    // it feeds mu(x+2), needed while x+1 < x1
+    if (x + 3 <= x1 + 1) pl.commit(&cs[ring<NC>(x + 3 - x0 + 2)][0][0], bc, tid);
+    if (x + 4 <= x1 + 1) pl.fetch(uc, x + 4, d, bc);
+    compute_mu(x + 1);
+    __syncthreads();
+    const T(*C0)[CW] = cs[ring<NC>(x - x0 + 2)];
+    const T(*Cp)[CW] = cs[ring<NC>(x + 1 - x0 + 2)];
+    const T(*M0)[MW] = ms[ring<NM>(x - x0 + 1)];
+    const T(*Mp)[MW] = ms[ring<NM>(x + 1 - x0 + 1)];
+    const int b = tz + 2, bm = tz + 1;
+    T cc[PY + 2], mc[PY + 2], fy[PY + 1];
+#pragma unroll
+    for (int j = 0; j < PY + 2; ++j) {
+      cc[j] = C0[r0 + j + 1][b];
+      mc[j] = M0[r0 + j][bm];
+    }
+#pragma unroll
+    for (int j = 0; j <= PY; ++j) fy[j] = Op::flux(cc[j], cc[j + 1], mc[j], mc[j + 1], g.idy);
+    if (x == x0) {
+      const T(*Cm)[CW] = cs[ring<NC>(x - 1 - x0 + 2)];
+      const T(*Mm)[MW] = ms[ring<NM>(x - 1 - x0 + 1)];
+#pragma unroll
+      for (int q = 0; q < PY; ++q) carry[q] = Op::flux(Cm[r0 + q + 2][b], cc[q + 1], Mm[r0 + q + 1][bm], mc[q + 1], g.idx);
+    }
+#pragma unroll
+    for (int q = 0; q < PY; ++q) {
+      const int a = r0 + q + 2, am = r0 + q + 1;
+      const T c0 = cc[q + 1], m0 = mc[q + 1];
+      const T fxp = Op::flux(c0, Cp[a][b], m0, Mp[am][bm], g.idx);
+      const T fzm = Op::flux(C0[a][b - 1], c0, M0[am][bm - 1], m0, g.idz);
+      const T fzp = Op::flux(c0, C0[a][b + 1], m0, M0[am][bm + 1], g.idz);
+      const T r = Op::div(carry[q], fxp, fy[q], fy[q + 1], fzm, fzp, g, k.D);
+      carry[q] = fxp;
+      const int64_t gy = y0 + r0 + q;
+      if (gy < d.ny && gz < d.nz) epilogue(ep, uc, oc, x * plane + gy * d.nz + gz, r);
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace evx

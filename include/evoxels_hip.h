@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define EVX_ABI_VERSION 1
+#define EVX_ABI_VERSION 2
 
 enum evx_status {
   EVX_OK = 0,
@@ -163,6 +163,11 @@ typedef struct evx_imex_params {
   double eps;         /* CahnHilliard eps  (problem_definition.py:293)    */
   double D;           /* diffusivity / mobility prefactor                 */
   double A;           /* stabilisation constant (problem_definition.py:296) */
+  /* Optional device array of 2*channels doubles {eps_c, D_c}: per-channel
+   * parameters for batched independent trajectories (each channel then
+   * follows the reference's scalar-parameter step with its own eps, D).
+   * NULL: every channel uses eps and D above. */
+  const double* chan_params;
 } evx_imex_params;
 
 /* u_out = u + irfftn(prefac * rfftn(ch_rhs(u)))  for a fully periodic
@@ -174,6 +179,88 @@ int evx_ch_imex_step(evx_fft_plan* plan, const void* u, void* u_out,
 /* Same for ReactionDiffusion (f == 0; symbol -D*A*k^2, problem_definition.py:206). */
 int evx_rd_imex_step(evx_fft_plan* plan, const void* u, void* u_out,
                      void* workspace, const evx_imex_params* p, void* stream);
+
+/* ---- adjoints (reverse mode) ----------------------------------------------
+ * The reference differentiates its solvers with JAX (inversion.py:20-233);
+ * these entry points give the vector-Jacobian products of the torch path's
+ * CahnHilliard step for gradient-based parameter estimation.  Periodic
+ * boundaries only.  Parameter gradients are accumulated (+=) into a device
+ * double array dparams[2*channels] = {dJ/deps_c, dJ/dD_c}, in a fixed
+ * reduction order (deterministic).  The clip of CahnHilliard.rhs passes
+ * gradient where 0 <= u <= 1, as torch.clip's backward does. */
+
+/* Device workspace bytes of evx_ch_rhs_vjp for a (channels,nx,ny,nz) field. */
+int evx_ch_rhs_vjp_workspace_bytes(int32_t dtype, const evx_grid* g,
+                                   size_t* bytes);
+
+/* grad_u = base + (d rhs/d u)^T w for r = ch_rhs(u) (problem_definition.py:
+ * 328-371, periodic); base may be NULL (zero) and may alias grad_u.
+ * dparams += (w . dr/deps, w . dr/dD) per channel.  chan_params as in
+ * evx_imex_params (NULL: eps, D for every channel). */
+int evx_ch_rhs_vjp(int32_t dtype, const void* u, const void* w,
+                   const void* base, void* grad_u, double* dparams,
+                   const evx_grid* g, double eps, double D,
+                   const double* chan_params, void* workspace, void* stream);
+
+/* Device workspace bytes of evx_ch_imex_step_vjp for this plan. */
+int evx_ch_imex_vjp_workspace_bytes(const evx_fft_plan* plan, size_t* bytes);
+
+/* Reverse step of evx_ch_imex_step, u1 = u0 + L(rhs(u0)) with
+ * L = irfftn(prefac * rfftn(.)) (timesteppers.py:85-89):
+ *   lam0 = lam1 + (d rhs/d u0)^T L(lam1)
+ *   dparams += the eps / D gradients through rhs and through prefac.
+ * L is self-adjoint (real, even multiplier).  The prefac term uses
+ * d prefac/d theta = prefac*(prefac-dt)/(dt*theta) for theta in {eps, D}
+ * (sigma is proportional to eps*D), so it needs u1 - u0 and no extra
+ * transform.  u0 and u1 are the forward step's input and output; lam0 may
+ * alias lam1. */
+int evx_ch_imex_step_vjp(evx_fft_plan* plan, const void* u0, const void* u1,
+                         const void* lam1, void* lam0, double* dparams,
+                         void* workspace, const evx_imex_params* p,
+                         void* stream);
+
+/* ---- batched grids --------------------------------------------------------
+ * Many independent periodic Cahn-Hilliard grids of different shapes, each
+ * advanced by the semi-implicit step of evx_ch_imex_step (bit-identical per
+ * grid), float32.  No reference counterpart: the reference advances one
+ * grid per solver (solvers.py:80-114).  Edges must be in
+ * {16, 32, 48, 64, 96, 128}.  The state is one packed float array: grid i
+ * starts at element offsets[i] of evx_batch_state_layout. */
+typedef struct evx_batch_plan evx_batch_plan;
+int evx_batch_plan_create(evx_batch_plan** plan, int32_t dtype, int32_t n_grids,
+                          const int64_t* shapes /* n_grids x 3 */,
+                          const double* spacing /* n_grids x 3 */);
+int evx_batch_plan_destroy(evx_batch_plan* plan);
+int evx_batch_state_layout(const evx_batch_plan* plan, int64_t* offsets,
+                           int64_t* total_elems);
+int evx_batch_workspace_bytes(const evx_batch_plan* plan, size_t* bytes);
+/* Per-grid (eps, D) as a host array eps_D[2*i], eps_D[2*i+1]; dt and A as in
+ * evx_imex_params.  Set-up call (synchronous upload). */
+int evx_batch_set_params(evx_batch_plan* plan, const double* eps_D, double dt,
+                         double A);
+/* Schedules: EVX_BATCH_PERSISTENT (one launch per step; workgroups pull
+ * (grid, sweep, tile) items from a device queue with per-grid sweep
+ * dependencies) or EVX_BATCH_STAGED (one launch per sweep).  blocks = 0
+ * means three workgroups per CU; xchunk = x planes per CH tile.  Set-up
+ * call: no step of this plan may be in flight. */
+enum evx_batch_mode { EVX_BATCH_STAGED = 0, EVX_BATCH_PERSISTENT = 1 };
+int evx_batch_set_schedule(evx_batch_plan* plan, int mode, int blocks,
+                           int xchunk);
+int evx_batch_get_schedule(const evx_batch_plan* plan, int* mode, int* blocks,
+                           int* xchunk, int* n_items);
+/* Times every candidate schedule on scratch_state (overwritten) and keeps
+ * the fastest; best_ms (may be NULL) receives its time per step.
+ * Synchronises the stream. */
+int evx_batch_autotune(evx_batch_plan* plan, void* scratch_state,
+                       void* workspace, int steps, void* stream,
+                       double* best_ms);
+/* One semi-implicit step of every grid, in place on the packed state.
+ * Asynchronous, allocation-free, capturable in a graph. */
+int evx_batch_ch_imex_step(evx_batch_plan* plan, void* state, void* workspace,
+                           void* stream);
+/* Reads (synchronously) the plan's fault flag: nonzero if a dependency wait
+ * of the persistent schedule ever hit its bound. */
+int evx_batch_error_flag(evx_batch_plan* plan, int* flag);
 
 /* Stage timing for benchmarks (no reference counterpart; the reference times
  * whole runs with a wall clock, solvers.py:108-114).  Arms the next

@@ -33,7 +33,7 @@ def test_binding_table_matches_header():
 
 def test_version_and_error_paths_without_gpu():
     lib = _lib.load()
-    assert lib.evx_version() == 1
+    assert lib.evx_version() == 2
     g = _lib.make_grid((1, 4, 4, 4), (1, 1, 1), (1, 1, 1))
     b = _lib.make_bc((("periodic", None),) * 3)
     e = _lib.make_epilogue()

@@ -1,0 +1,239 @@
+"""Reverse mode of the semi-implicit Cahn-Hilliard step: the HIP adjoint
+kernels against torch autograd through the CPU oracle (float64), against
+central finite differences, and per-channel parameters of the batched forward
+step against the oracle run channel by channel."""
+import numpy as np
+import pytest
+import torch
+
+import evoxels_amd as evx
+from evoxels_amd.inversion import (CHImexEngine, CHImexStep, CHRhs, InversionModel, ch_rhs_vjp,
+                                   loss_and_grad, solve)
+from evoxels_amd.voxelgrid import VoxelGridHip
+from oracle import evoxels_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+P3 = (("periodic", None),) * 3
+
+
+def grid_for(shape, precision, spacing=(1.0, 1.0, 1.0)):
+    vf = evx.VoxelFields(shape, tuple(s * n for s, n in zip(spacing, shape)))
+    vf.precision = precision
+    return vf, VoxelGridHip(vf.grid_info(), precision=precision, device="cuda")
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def field_with_clip_cases(rng, shape):
+    u = 0.5 + 0.15 * rng.standard_normal(shape)
+    flat = u.reshape(-1)
+    # values on and beyond the clip bounds: torch.clip passes the gradient on
+    # the closed interval [0, 1]
+    flat[::97] = 0.0
+    flat[5::89] = 1.0
+    flat[11::83] = -0.05
+    flat[17::79] = 1.04
+    return u
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+@pytest.mark.parametrize("shape,spacing", [((16, 16, 16), (1.0, 1.0, 1.0)), ((12, 10, 14), (0.8, 1.2, 1.0)),
+                                           ((8, 6, 70), (1.0, 1.0, 1.0))])
+def test_ch_rhs_vjp_matches_oracle_autograd(shape, spacing, precision):
+    vf, vg = grid_for(shape, precision, spacing)
+    rng = np.random.default_rng(7)
+    arr = np.stack([field_with_clip_cases(rng, shape) for _ in range(2)])
+    w = rng.standard_normal((2,) + shape)
+    eps, D = 2.5, 1.3
+    ut = torch.tensor(arr, dtype=vg.precision, device="cuda")
+    wt = torch.tensor(w, dtype=vg.precision, device="cuda")
+    base = torch.tensor(rng.standard_normal((2,) + shape), dtype=vg.precision, device="cuda")
+    grad, dp = ch_rhs_vjp(vg, ut, wt, eps, D, base=base)
+    og = O.OracleGrid(shape, vf.spacing, "float64")
+    u64 = torch.tensor(arr, dtype=torch.float64, requires_grad=True)
+    e64 = torch.tensor(eps, dtype=torch.float64, requires_grad=True)
+    d64 = torch.tensor(D, dtype=torch.float64, requires_grad=True)
+    dpe, dpd = [], []
+    gu = torch.zeros_like(u64)
+    for c in range(2):
+        r = O.ch_rhs(u64[c:c + 1], og, P3, e64, d64)
+        ge, gd, gc = torch.autograd.grad((r * torch.tensor(w[c:c + 1])).sum(), (e64, d64, u64))
+        dpe.append(float(ge))
+        dpd.append(float(gd))
+        gu += gc
+    want_u = gu.numpy() + base.cpu().double().numpy()
+    tol_u, tol_p = (1e-12, 1e-11) if precision == "float64" else (2e-5, 2e-5)
+    assert rel(grad.cpu(), want_u) < tol_u
+    assert rel(dp[:, 0].cpu(), dpe) < tol_p
+    assert rel(dp[:, 1].cpu(), dpd) < tol_p
+
+
+def test_ch_rhs_autograd_function_scalar_params():
+    shape = (10, 12, 16)
+    vf, vg = grid_for(shape, "float64")
+    rng = np.random.default_rng(3)
+    u = torch.tensor(0.5 + 0.1 * rng.standard_normal((1,) + shape), device="cuda", requires_grad=True)
+    eps = torch.tensor(3.0, dtype=torch.float64, device="cuda", requires_grad=True)
+    D = torch.tensor(0.7, dtype=torch.float64, device="cuda", requires_grad=True)
+    r = CHRhs.apply(u, eps, D, vg)
+    w = torch.tensor(rng.standard_normal((1,) + shape), device="cuda")
+    (r * w).sum().backward()
+    og = O.OracleGrid(shape, vf.spacing, "float64")
+    u64 = u.detach().cpu().requires_grad_()
+    e64 = torch.tensor(3.0, dtype=torch.float64, requires_grad=True)
+    d64 = torch.tensor(0.7, dtype=torch.float64, requires_grad=True)
+    r64 = O.ch_rhs(u64, og, P3, e64, d64)
+    assert rel(r.detach().cpu(), r64.detach()) < 1e-13
+    (r64 * w.cpu()).sum().backward()
+    assert rel(u.grad.cpu(), u64.grad) < 1e-12
+    assert abs(float(eps.grad) - float(e64.grad)) <= 1e-11 * abs(float(e64.grad))
+    assert abs(float(D.grad) - float(d64.grad)) <= 1e-11 * abs(float(d64.grad))
+
+
+def oracle_step_autograd(arr_c, shape, spacing, dt, e64, d64):
+    og = O.OracleGrid(shape, spacing, "float64")
+    k2 = O.rfft_k_squared(shape, spacing)
+    pre = O.imex_prefac(dt, O.ch_symbol(k2, e64, d64))
+    return O.imex_step(arr_c, lambda t, v: O.ch_rhs(v, og, P3, e64, d64), pre)
+
+
+def test_batched_forward_per_channel_params_matches_oracle():
+    shape, dt = (12, 16, 20), 0.1
+    vf, vg = grid_for(shape, "float64", (1.0, 0.9, 1.1))
+    rng = np.random.default_rng(11)
+    arr = 0.5 + 0.05 * rng.standard_normal((3,) + shape)
+    prm = np.array([[3.0, 1.0], [2.2, 0.6], [4.1, 1.7]])
+    eng = CHImexEngine(vg, 3, dt)
+    u = torch.tensor(arr, device="cuda")
+    got = eng.step(u, eng.params(prm)).cpu().numpy()
+    for c in range(3):
+        ref = oracle_step_autograd(torch.tensor(arr[c:c + 1]), shape, vf.spacing, dt,
+                                   torch.tensor(prm[c, 0]), torch.tensor(prm[c, 1])).numpy()
+        upd = got[c] - arr[c]
+        upd_ref = ref[0] - arr[c]
+        assert np.linalg.norm(upd - upd_ref) / np.linalg.norm(upd_ref) < 1e-10
+
+
+@pytest.mark.parametrize("shape,spacing", [((16, 16, 16), (1.0, 1.0, 1.0)), ((12, 10, 18), (0.9, 1.1, 1.0))])
+def test_imex_step_vjp_matches_oracle_autograd(shape, spacing):
+    dt = 0.1
+    vf, vg = grid_for(shape, "float64", spacing)
+    rng = np.random.default_rng(5)
+    arr = np.stack([field_with_clip_cases(rng, shape) for _ in range(2)])
+    lam1 = rng.standard_normal((2,) + shape)
+    prm = np.array([[3.0, 1.0], [2.0, 1.4]])
+    eng = CHImexEngine(vg, 2, dt)
+    p = eng.params(prm)
+    u0 = torch.tensor(arr, device="cuda")
+    u1 = eng.step(u0, p)
+    dparams = torch.zeros_like(p)
+    lam0 = eng.step_vjp(u0, u1, torch.tensor(lam1, device="cuda"), p, dparams, lam0=torch.empty_like(u0))
+    for c in range(2):
+        u64 = torch.tensor(arr[c:c + 1], requires_grad=True)
+        e64 = torch.tensor(prm[c, 0], requires_grad=True)
+        d64 = torch.tensor(prm[c, 1], requires_grad=True)
+        out = oracle_step_autograd(u64, shape, vf.spacing, dt, e64, d64)
+        ge, gd, gu = torch.autograd.grad((out * torch.tensor(lam1[c:c + 1])).sum(), (e64, d64, u64))
+        assert rel(lam0[c].cpu(), gu[0]) < 1e-12
+        # autograd differentiates the float32 prefactor in float32 arithmetic
+        assert rel(dparams[c, 0].cpu(), ge) < 2e-5
+        assert rel(dparams[c, 1].cpu(), gd) < 2e-5
+
+
+def make_inverse_problem(shape, precision, n_steps, obs_steps, dt=0.1, seed=1):
+    vf, vg = grid_for(shape, precision)
+    rng = np.random.default_rng(seed)
+    u0 = torch.tensor(0.5 + 0.05 * rng.standard_normal((2,) + shape), dtype=vg.precision, device="cuda")
+    eng = CHImexEngine(vg, 2, dt)
+    truth = np.array([[3.0, 1.0], [2.4, 1.3]])
+    obs = solve(eng, u0, truth, n_steps, obs_steps)
+    return vg, eng, u0, truth, obs
+
+
+def test_trajectory_gradient_matches_central_differences():
+    """float64, 2 trajectories, 12 steps, observations at 4, 8, 12, checkpoints
+    every 5 steps (a short last segment): dJ/d(eps, D) against central
+    differences with a relative step of 1e-4."""
+    n_steps, obs_steps = 12, [4, 8, 12]
+    vg, eng, u0, truth, obs = make_inverse_problem((16, 16, 16), "float64", n_steps, obs_steps)
+    prm = truth * np.array([[1.1, 0.92], [0.9, 1.07]])
+    loss, grad = loss_and_grad(eng, u0, prm, n_steps, obs_steps, obs, checkpoint_every=5)
+    grad = grad.cpu().numpy()
+    fd = np.zeros_like(grad)
+    for c in range(2):
+        for j in range(2):
+            h = 1e-4 * prm[c, j]
+            pp, pm = prm.copy(), prm.copy()
+            pp[c, j] += h
+            pm[c, j] -= h
+            lp, _ = loss_and_grad(eng, u0, pp, n_steps, obs_steps, obs, checkpoint_every=5)
+            lm, _ = loss_and_grad(eng, u0, pm, n_steps, obs_steps, obs, checkpoint_every=5)
+            fd[c, j] = (float(lp[c]) - float(lm[c])) / (2 * h)
+    print("adjoint", grad, "fd", fd)
+    assert np.all(np.abs(grad - fd) <= 1e-4 * np.abs(fd).max(axis=1, keepdims=True))
+    # the checkpoint interval does not change the gradient
+    _, g1 = loss_and_grad(eng, u0, prm, n_steps, obs_steps, obs, checkpoint_every=1)
+    _, g20 = loss_and_grad(eng, u0, prm, n_steps, obs_steps, obs, checkpoint_every=20)
+    assert rel(g1.cpu(), grad) < 1e-12 and rel(g20.cpu(), grad) < 1e-12
+
+
+def test_trajectory_gradient_matches_oracle_autograd_and_autograd_function():
+    n_steps, obs_steps = 6, [3, 6]
+    shape = (12, 12, 16)
+    vg, eng, u0, truth, obs = make_inverse_problem(shape, "float64", n_steps, obs_steps)
+    prm = truth * 0.95
+    loss, grad = loss_and_grad(eng, u0, prm, n_steps, obs_steps, obs, checkpoint_every=4)
+    # CHImexStep chained under torch autograd
+    p = torch.tensor(prm, device="cuda", requires_grad=True)
+    u = u0.clone()
+    J = 0
+    for k in range(n_steps):
+        u = CHImexStep.apply(u, p, eng)
+        if k + 1 in obs_steps:
+            J = J + 0.5 * ((u - obs[obs_steps.index(k + 1)]) ** 2).sum()
+    J.backward()
+    assert rel(p.grad.cpu(), grad.cpu()) < 1e-12
+    assert abs(float(J) - float(loss.sum())) < 1e-12 * float(loss.sum())
+    # the oracle differentiated by torch autograd on the CPU, channel by channel
+    spacing = (1.0, 1.0, 1.0)
+    for c in range(2):
+        e64 = torch.tensor(prm[c, 0], requires_grad=True)
+        d64 = torch.tensor(prm[c, 1], requires_grad=True)
+        v = u0[c:c + 1].cpu()
+        Jc = 0
+        for k in range(n_steps):
+            v = oracle_step_autograd(v, shape, spacing, 0.1, e64, d64)
+            if k + 1 in obs_steps:
+                Jc = Jc + 0.5 * ((v - obs[obs_steps.index(k + 1)][c:c + 1].cpu()) ** 2).sum()
+        ge, gd = torch.autograd.grad(Jc, (e64, d64))
+        assert rel(float(loss[c]), float(Jc)) < 1e-10
+        assert rel(grad[c, 0].cpu(), ge) < 2e-5
+        assert rel(grad[c, 1].cpu(), gd) < 2e-5
+
+
+def test_float32_trajectory_gradient_close_to_float64():
+    n_steps, obs_steps = 10, [5, 10]
+    shape = (16, 16, 16)
+    vg64, e64, u64, truth, obs64 = make_inverse_problem(shape, "float64", n_steps, obs_steps)
+    vg32, e32, u32, _, obs32 = make_inverse_problem(shape, "float32", n_steps, obs_steps)
+    prm = truth * 1.05
+    _, g64 = loss_and_grad(e64, u64, prm, n_steps, obs_steps, obs64)
+    _, g32 = loss_and_grad(e32, u32, prm, n_steps, obs_steps, obs32)
+    assert rel(g32.cpu(), g64.cpu()) < 2e-3
+
+
+def test_inversion_model_reduces_misfit():
+    n_steps, obs_steps = 10, [5, 10]
+    vg, eng, u0, truth, obs = make_inverse_problem((16, 16, 16), "float64", n_steps, obs_steps)
+    model = InversionModel(vg, channels=2, dt=0.1)
+    start = {"eps": torch.tensor([3.6, 2.0]), "D": torch.tensor([0.8, 1.6])}
+    p, hist = model.train(start, u0, obs_steps, obs, max_steps=40, lr=0.05)
+    assert hist[-1].sum() < 0.05 * hist[0].sum()
+    err0 = np.abs(np.log(np.array([[3.6, 0.8], [2.0, 1.6]]) / truth))
+    err1 = np.abs(np.log(np.stack([p["eps"].cpu().numpy(), p["D"].cpu().numpy()], 1) / truth))
+    assert err1.sum() < 0.5 * err0.sum()

@@ -106,3 +106,38 @@ def test_hip_grid_refuses_cpu_device():
     if not torch.cuda.is_available():
         with pytest.raises(RuntimeError, match="ROCm GPU"):
             VoxelGridHip(vf.grid_info(), device="cuda")
+
+
+def test_inversion_checkpoint_schedule_and_argument_checks():
+    from evoxels_amd.inversion import _obs_map, _params_tensor, checkpoint_schedule
+    assert checkpoint_schedule(12, 5) == [(10, 12), (5, 10), (0, 5)]
+    assert checkpoint_schedule(40, 20) == [(20, 40), (0, 20)]
+    assert checkpoint_schedule(0, 20) == []
+    with pytest.raises(ValueError):
+        checkpoint_schedule(10, 0)
+    assert _obs_map([4, 8, 12], 12) == {4: 0, 8: 1, 12: 2}
+    with pytest.raises(ValueError):
+        _obs_map([4, 13], 12)
+    with pytest.raises(ValueError):
+        _obs_map([4, 4], 12)
+    p = _params_tensor({"eps": [3.0, 2.0], "D": 1.5}, 2, "cpu")
+    assert p.dtype == torch.float64 and p.tolist() == [[3.0, 1.5], [2.0, 1.5]]
+    with pytest.raises(ValueError):
+        _params_tensor([[1.0, 2.0, 3.0]], 1, "cpu")
+
+
+def test_batched_plan_rejects_unsupported_shapes_and_dtype_before_any_gpu_work():
+    import ctypes as C
+    from evoxels_amd import _lib
+    lib = _lib.load()
+    h = C.c_void_p()
+    shp = np.array([[32, 32, 100]], dtype=np.int64)
+    sp = np.ones((1, 3))
+    assert lib.evx_batch_plan_create(C.byref(h), _lib.EVX_F32, 1, shp.ctypes.data, sp.ctypes.data) \
+        == _lib.EVX_ERR_UNSUPPORTED
+    assert "edges" in _lib.last_error()
+    shp = np.array([[32, 32, 32]], dtype=np.int64)
+    assert lib.evx_batch_plan_create(C.byref(h), _lib.EVX_F64, 1, shp.ctypes.data, sp.ctypes.data) \
+        == _lib.EVX_ERR_UNSUPPORTED
+    assert lib.evx_batch_plan_create(C.byref(h), _lib.EVX_F32, 0, shp.ctypes.data, sp.ctypes.data) \
+        == _lib.EVX_ERR_INVALID
