@@ -30,6 +30,7 @@ BC_KIND = {"periodic": 0, "neumann": 1, "dirichlet": 2}
 OUT_RHS, OUT_EULER, OUT_RK4_FIRST, OUT_RK4_MID, OUT_RK4_LAST = range(5)
 
 OPT_SPECIALIZED = 1
+OPT_EXACT_SYMBOL = 2
 
 
 class EvxBC(C.Structure):
@@ -85,12 +86,10 @@ SIGNATURES = {
     "evx_batch_state_layout": [_P, _P, C.POINTER(_I64)],
     "evx_batch_workspace_bytes": [_P, C.POINTER(C.c_size_t)],
     "evx_batch_set_params": [_P, _P, _D, _D],
-    "evx_batch_set_schedule": [_P, C.c_int, C.c_int, C.c_int],
-    "evx_batch_get_schedule": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
-                               C.POINTER(C.c_int)],
+    "evx_batch_set_tiles": [_P, C.c_int],
+    "evx_batch_get_tiles": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "evx_batch_autotune": [_P, _P, _P, C.c_int, _P, C.POINTER(_D)],
     "evx_batch_ch_imex_step": [_P, _P, _P, _P],
-    "evx_batch_error_flag": [_P, C.POINTER(C.c_int)],
     "evx_fft_plan_trace_next_step": [_P, C.POINTER(_P), C.c_int],
     "evx_fft_plan_set_option": [_P, C.c_int, C.c_int],
     "evx_reduce_workspace_bytes": [_I64, C.POINTER(C.c_size_t)],

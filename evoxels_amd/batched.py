@@ -2,13 +2,12 @@
 together on one GPU.
 
 Each grid follows ``CahnHilliard`` + ``PseudoSpectralIMEX``
-(problem_definition.py:290-371, timesteppers.py:64-89) exactly as the
-single-grid fused step does (bit-identical per grid).  All grids live in one
-packed float32 state; one call advances every grid by one step, either with
-one persistent launch (work items pulled from a device queue, per-grid sweep
-dependencies) or with one launch per sweep.  ``autotune`` times the schedules
-and tile shapes and keeps the fastest; ``capture`` records a number of steps
-in a CUDA/HIP graph.
+(problem_definition.py:290-371, timesteppers.py:64-89) with the tile bodies
+of the single-grid fused step.  All grids live in one packed float32 state;
+one call advances every grid by one step with six launches (one per sweep),
+each covering the tiles of every grid.  ``autotune`` times the tile shapes
+and keeps the fastest; ``capture`` records a number of steps in a HIP graph,
+so a replay advances all grids by many steps with one host call.
 
 The reference has no batched solver: it advances one grid per solver
 (solvers.py:80-114).
@@ -23,8 +22,6 @@ import numpy as np
 import torch
 
 from . import _lib
-
-STAGED, PERSISTENT = 0, 1
 
 
 class BatchedCahnHilliard:
@@ -43,6 +40,8 @@ class BatchedCahnHilliard:
         dev = torch.device(device)
         if dev.type != "cuda" or not torch.cuda.is_available():
             raise RuntimeError("the 'hip' backend needs a ROCm GPU")
+        if dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
         self.device = dev
         sp = np.ones((n, 3)) if spacing is None else np.asarray(spacing, dtype=np.float64).reshape(n, 3)
         shp = np.ascontiguousarray(np.asarray(self.shapes, dtype=np.int64))
@@ -109,24 +108,20 @@ class BatchedCahnHilliard:
             self.step(state, n_steps)
         return g
 
-    # ------------------------------------------------------------ schedule
+    # ------------------------------------------------------------ tiles
     @property
     def schedule(self):
-        m, b, x, ni = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        _lib.check(_lib.load().evx_batch_get_schedule(self.handle, C.byref(m), C.byref(b), C.byref(x),
-                                                      C.byref(ni)), "evx_batch_get_schedule")
-        return {"mode": "persistent" if m.value == PERSISTENT else "staged", "blocks": b.value,
-                "xchunk": x.value, "items": ni.value}
+        x, ni = C.c_int(), C.c_int()
+        _lib.check(_lib.load().evx_batch_get_tiles(self.handle, C.byref(x), C.byref(ni)), "evx_batch_get_tiles")
+        return {"launches_per_step": 6, "xchunk": x.value, "items": ni.value}
 
-    def set_schedule(self, mode="persistent", blocks=0, xchunk=8):
+    def set_tiles(self, xchunk=8):
         torch.cuda.synchronize(self.device)
-        code = PERSISTENT if mode == "persistent" else STAGED
-        _lib.check(_lib.load().evx_batch_set_schedule(self.handle, code, int(blocks), int(xchunk)),
-                   "evx_batch_set_schedule")
+        _lib.check(_lib.load().evx_batch_set_tiles(self.handle, int(xchunk)), "evx_batch_set_tiles")
 
     def autotune(self, state, steps: int = 5):
-        """Times the candidate schedules on a copy of ``state`` and keeps the
-        fastest; returns the schedule with its time per step."""
+        """Times the candidate tile shapes on a copy of ``state`` and keeps the
+        fastest; returns the choice with its time per step."""
         scratch = state.clone()
         best = C.c_double()
         _lib.check(_lib.load().evx_batch_autotune(self.handle, _lib.ptr(scratch), _lib.ptr(self.ws), int(steps),
@@ -135,8 +130,3 @@ class BatchedCahnHilliard:
         out = self.schedule
         out["ms_per_step"] = best.value
         return out
-
-    def error_flag(self) -> int:
-        f = C.c_int()
-        _lib.check(_lib.load().evx_batch_error_flag(self.handle, C.byref(f)), "evx_batch_error_flag")
-        return f.value

@@ -3,23 +3,13 @@
 //
 // A step of one grid is six sweeps (CH right-hand side, z R2C, y forward,
 // x forward * prefac * inverse, y inverse, z C2R + add).  Each sweep is cut
-// into tiles; an item is (grid, sweep, tile).  Items are listed sweep-major
-// over all grids.  Two schedules run the same tile bodies:
+// into tiles; an item is (grid, sweep, tile).  A step is six launches, one
+// per sweep, each covering the items of every grid (one workgroup per item,
+// whatever the grid's shape), so small grids share the machine instead of
+// each paying its own launches.  The host keeps the item list sweep-major;
+// the x-chunk of the CH tiles is chosen by the autotuner.
 //
-//   persistent  one launch per step.  Workgroups claim items from a device
-//               queue in list order; before an item of sweep s of grid g
-//               starts, all items of sweep s-1 of grid g must be done (a
-//               per-grid counter, release/acquire at agent scope).  Every
-//               dependency of an item lies earlier in the list and was
-//               therefore claimed by a running workgroup, so the waits always
-//               end; each wait is also bounded (error flag) so a wave can
-//               never spin forever.  The last workgroup to leave resets the
-//               queue and counters for the next launch (no memset, so the
-//               launch can be captured in a graph).
-//   staged      one launch per sweep, one workgroup per item.
-//
-// The tile bodies are the ones of the single-grid kernels, so every grid
-// gets bit-identical results to evx_ch_imex_step on that grid alone.
+// The tile bodies are the ones of the single-grid kernels.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -76,8 +66,6 @@ struct BArgs {
   T* rhs;
   C* spec;
   const C* tw;
-  unsigned* ctr;  // [0] queue head, [1] workgroups done, [2..] per-grid sweep counters
-  int* err;
 };
 
 template <int MODE>
@@ -169,58 +157,6 @@ __device__ __forceinline__ void run_item(const BArgs& a, int4 it, int tid, unsig
   }
 }
 
-// Upper bound of sleeping polls of one dependency wait (about half a second,
-// far beyond any sweep).  Once one wait has hit it, the plan's error flag is
-// set and every later wait returns at once, so a launch always drains
-// quickly even if a dependency were never met.
-constexpr unsigned SPIN_LIMIT = 1u << 23;
-
-__device__ __forceinline__ void wait_for(unsigned* p, unsigned target, int* err) {
-  unsigned spins = 0;
-  while (__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-    if ((spins & 1023u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-    __builtin_amdgcn_s_sleep(2);
-    if (++spins > SPIN_LIMIT) {
-      __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      break;
-    }
-  }
-}
-
-__global__ __launch_bounds__(BT) void batch_persistent(BArgs a) {
-  __shared__ __align__(16) unsigned char smem[BATCH_LDS];
-  __shared__ int s_idx;
-  const int tid = threadIdx.x;
-  unsigned* head = a.ctr;
-  unsigned* left = a.ctr + 1;
-  unsigned* done = a.ctr + 2;
-  for (;;) {
-    if (tid == 0) s_idx = (int)__hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int idx = s_idx;
-    if (idx >= a.n_items) break;
-    const int4 it = a.items[idx];
-    if (tid == 0 && it.y > 0) wait_for(done + it.x * NSTAGE + it.y - 1, (unsigned)a.inst[it.x].nt[it.y - 1], a.err);
-    __syncthreads();
-    // Re-derive the thread index each iteration so that the per-size address
-    // arithmetic of every tile body is not hoisted out of the loop (which
-    // would keep all of it live at once).
-    int t = tid;
-    asm volatile("" : "+v"(t));
-    run_item(a, it, t, smem);
-    __syncthreads();
-    if (tid == 0) __hip_atomic_fetch_add(done + it.x * NSTAGE + it.y, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  if (tid == 0) {
-    const unsigned e = __hip_atomic_fetch_add(left, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (e == gridDim.x - 1) {  // every other workgroup has left: reset for the next launch
-      for (int i = 0; i < a.n_inst * NSTAGE; ++i) __hip_atomic_store(done + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(head, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(left, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
-
 __global__ __launch_bounds__(BT) void batch_stage(BArgs a, int first) {
   __shared__ __align__(16) unsigned char smem[BATCH_LDS];
   run_item(a, a.items[first + (int)blockIdx.x], threadIdx.x, smem);
@@ -279,8 +215,6 @@ struct evx_batch_plan {
   int64_t state_elems = 0;
   size_t ws_bytes = 0;
   // schedule
-  int mode = 1;     // 1 persistent, 0 staged
-  int blocks = 0;   // persistent workgroups (0: 3 per CU)
   int xchunk = 8;   // x planes per CH tile
   int n_items = 0;
   int stage_first[evx::NSTAGE + 1] = {0};
@@ -289,9 +223,6 @@ struct evx_batch_plan {
   evx::BInst* d_inst = nullptr;
   int4* d_items = nullptr;
   evx::C* d_tw = nullptr;
-  unsigned* d_ctr = nullptr;
-  int* d_err = nullptr;
-  int cus = 256;
 };
 
 namespace evx {
@@ -302,8 +233,6 @@ void free_plan(evx_batch_plan* p) {
   hipFree(p->d_inst);
   hipFree(p->d_items);
   hipFree(p->d_tw);
-  hipFree(p->d_ctr);
-  hipFree(p->d_err);
   delete p;
 }
 
@@ -365,18 +294,11 @@ BArgs make_args(evx_batch_plan* p, void* state, void* ws) {
   a.rhs = static_cast<T*>(ws);
   a.spec = reinterpret_cast<C*>(static_cast<char*>(ws) + align_up(sizeof(T) * p->state_elems));
   a.tw = p->d_tw;
-  a.ctr = p->d_ctr;
-  a.err = p->d_err;
   return a;
 }
 
 int launch_step(evx_batch_plan* p, void* state, void* ws, hipStream_t s) {
   const BArgs a = make_args(p, state, ws);
-  if (p->mode == 1) {
-    const int blocks = p->blocks > 0 ? p->blocks : 3 * p->cus;
-    hipLaunchKernelGGL(batch_persistent, dim3(blocks), dim3(BT), 0, s, a);
-    return check_launch("batch_persistent");
-  }
   for (int st = 0; st < NSTAGE; ++st) {
     const int n = p->stage_first[st + 1] - p->stage_first[st];
     if (n == 0) continue;
@@ -461,20 +383,12 @@ extern "C" int evx_batch_plan_create(evx_batch_plan** out, int32_t dtype, int32_
   }
   p->state_elems = u_off;
   p->ws_bytes = align_up(sizeof(T) * u_off) + align_up(sizeof(C) * s_off);
-  int dev = 0;
-  hipDeviceProp_t prop;
-  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-    p->cus = prop.multiProcessorCount;
-  const size_t nctr = 2 + (size_t)NSTAGE * n_inst;
   if (hipMalloc(&p->d_inst, sizeof(BInst) * n_inst) != hipSuccess ||
-      hipMalloc(&p->d_tw, sizeof(C) * tw.size()) != hipSuccess ||
-      hipMalloc(&p->d_ctr, sizeof(unsigned) * nctr) != hipSuccess || hipMalloc(&p->d_err, sizeof(int)) != hipSuccess) {
+      hipMalloc(&p->d_tw, sizeof(C) * tw.size()) != hipSuccess) {
     free_plan(p);
     return fail(EVX_ERR_NOMEM, "hipMalloc of batched plan tables failed");
   }
-  if (hipMemcpy(p->d_tw, tw.data(), sizeof(C) * tw.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(p->d_ctr, 0, sizeof(unsigned) * nctr) != hipSuccess || hipMemset(p->d_err, 0, sizeof(int)) != hipSuccess ||
-      hipDeviceSynchronize() != hipSuccess) {
+  if (hipMemcpy(p->d_tw, tw.data(), sizeof(C) * tw.size(), hipMemcpyHostToDevice) != hipSuccess) {
     free_plan(p);
     return fail(EVX_ERR_HIP, "batched plan upload failed");
   }
@@ -524,15 +438,11 @@ extern "C" int evx_batch_set_params(evx_batch_plan* p, const double* eps_d, doub
   return EVX_OK;
 }
 
-extern "C" int evx_batch_set_schedule(evx_batch_plan* p, int mode, int blocks, int xchunk) {
+extern "C" int evx_batch_set_tiles(evx_batch_plan* p, int xchunk) {
   if (!p) return fail(EVX_ERR_INVALID, "null plan");
-  if (mode != 0 && mode != 1) return fail(EVX_ERR_INVALID, "mode must be 0 (staged) or 1 (persistent)");
-  if (blocks < 0 || blocks > 65535) return fail(EVX_ERR_INVALID, "blocks must be in [0, 65535]");
   if (xchunk < 1 || xchunk > 4096) return fail(EVX_ERR_INVALID, "xchunk must be in [1, 4096]");
   // The previous launch of this plan must be complete before the item list
   // changes (the caller synchronises; set-up time only).
-  p->mode = mode;
-  p->blocks = blocks;
   if (xchunk != p->xchunk) {
     p->xchunk = xchunk;
     EVX_TRY(build_items(p));
@@ -540,10 +450,8 @@ extern "C" int evx_batch_set_schedule(evx_batch_plan* p, int mode, int blocks, i
   return EVX_OK;
 }
 
-extern "C" int evx_batch_get_schedule(const evx_batch_plan* p, int* mode, int* blocks, int* xchunk, int* n_items) {
-  if (!p || !mode || !blocks || !xchunk || !n_items) return fail(EVX_ERR_INVALID, "null argument");
-  *mode = p->mode;
-  *blocks = p->blocks > 0 ? p->blocks : 3 * p->cus;
+extern "C" int evx_batch_get_tiles(const evx_batch_plan* p, int* xchunk, int* n_items) {
+  if (!p || !xchunk || !n_items) return fail(EVX_ERR_INVALID, "null argument");
   *xchunk = p->xchunk;
   *n_items = p->n_items;
   return EVX_OK;
@@ -555,16 +463,9 @@ extern "C" int evx_batch_ch_imex_step(evx_batch_plan* p, void* state, void* work
   return launch_step(p, state, workspace, static_cast<hipStream_t>(stream));
 }
 
-extern "C" int evx_batch_error_flag(evx_batch_plan* p, int* flag) {
-  if (!p || !flag) return fail(EVX_ERR_INVALID, "null argument");
-  if (hipMemcpy(flag, p->d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
-    return fail(EVX_ERR_HIP, "reading the batched error flag failed");
-  return EVX_OK;
-}
-
-// Autotuner: times each candidate schedule (mode, workgroups, x-chunk) over
-// `steps` steps with HIP events after one warm-up step, and keeps the
-// fastest.  The candidates run on the caller's scratch state, which is
+// Autotuner: times each candidate tile shape (x planes per CH tile, which
+// sets the number of CH items and the halo overhead of each) over `steps`
+// steps with HIP events after one warm-up step, and keeps the fastest.  The candidates run on the caller's scratch state, which is
 // overwritten.  Synchronises the stream (set-up time only).
 extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* workspace, int steps, void* stream,
                                   double* best_ms) {
@@ -572,23 +473,16 @@ extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* 
   if (!p->params_set) return fail(EVX_ERR_INVALID, "evx_batch_set_params has not been called");
   if (steps < 1 || steps > 1000) return fail(EVX_ERR_INVALID, "steps must be in [1, 1000]");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  struct Cand {
-    int mode, blocks, xchunk;
-  };
-  std::vector<Cand> cands;
-  for (int xc : {4, 8, 16, 32}) {
-    cands.push_back({0, 0, xc});
-    for (int k : {1, 2, 3}) cands.push_back({1, k * p->cus, xc});
-  }
+  const int cands[] = {4, 8, 16, 32, 64};
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return fail(EVX_ERR_HIP, "hipEventCreate failed");
   double best = 1e300;
-  Cand pick = {p->mode, p->blocks, p->xchunk};
+  int pick = p->xchunk;
   int st = EVX_OK;
-  for (const Cand& c : cands) {
+  for (const int c : cands) {
     if (hipStreamSynchronize(s) != hipSuccess) { st = fail(EVX_ERR_HIP, "stream synchronisation failed"); break; }
-    if ((st = evx_batch_set_schedule(p, c.mode, c.blocks, c.xchunk)) != EVX_OK) break;
+    if ((st = evx_batch_set_tiles(p, c)) != EVX_OK) break;
     if ((st = launch_step(p, scratch_state, workspace, s)) != EVX_OK) break;
     hipEventRecord(e0, s);
     for (int i = 0; i < steps && st == EVX_OK; ++i) st = launch_step(p, scratch_state, workspace, s);
@@ -605,7 +499,7 @@ extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* 
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (st != EVX_OK) return st;
-  EVX_TRY(evx_batch_set_schedule(p, pick.mode, pick.blocks, pick.xchunk));
+  EVX_TRY(evx_batch_set_tiles(p, pick));
   if (best_ms) *best_ms = best;
   return EVX_OK;
 }

@@ -256,7 +256,8 @@ __global__ void strided_kernel(cpx<T>* spec, int64_t nlines, LineMap lm, Axis<T>
       const int jy = (int)(outer % pf.ny);
       const float ssc = symbol_scalar(pf, outer / pf.ny);
       for (int j = tl; j < n; j += jstep) {
-        const T f = static_cast<T>(prefac_f32(j, jy, jz, pf, ssc)) * sc;
+        const T f = pf.exact ? static_cast<T>(prefac_f64(j, jy, jz, pf, outer / pf.ny) * pf.scale)
+                             : static_cast<T>(prefac_f32(j, jy, jz, pf, ssc)) * sc;
         buf[j * lpw + line] = cscale(buf[j * lpw + line], f);
       }
     }
@@ -407,6 +408,7 @@ struct evx_fft_plan {
   hipEvent_t trace[8] = {};
   int ntrace = 0;
   bool fast = true;  // use the size-specialised kernels where one exists
+  bool exact_symbol = false;  // float64 multiplier from exact wavenumbers
 };
 
 namespace evx {
@@ -549,6 +551,10 @@ Prefac make_prefac(const evx_fft_plan* p, int ch, double sym_scalar, double dt, 
   pf.scale = 1.0 / ((double)p->d.nx * (double)p->d.ny * (double)p->d.nz);
   pf.chan = nullptr;
   pf.A = 0.0;
+  pf.exact = p->exact_symbol ? 1 : 0;
+  pf.scd = sym_scalar;
+  pf.dtd = dt;
+  for (int a = 0; a < 3; ++a) pf.inv_ndd[a] = 1.0 / ((double)n[a] * spacing[a]);
   return pf;
 }
 
@@ -801,6 +807,7 @@ extern "C" int evx_fft_plan_set_option(evx_fft_plan* p, int option, int value) {
   EVX_TRY(check_plan(p));
   switch (option) {
     case EVX_OPT_SPECIALIZED: p->fast = value != 0; return EVX_OK;
+    case EVX_OPT_EXACT_SYMBOL: p->exact_symbol = value != 0; return EVX_OK;
     default: return fail(EVX_ERR_INVALID, "unknown plan option %d", option);
   }
 }

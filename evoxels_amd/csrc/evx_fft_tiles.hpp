@@ -34,6 +34,10 @@ struct Prefac {
   double scale;    // 1/(nx*ny*nz) of the inverse transform
   const double* chan;  // optional per-channel {eps, D}; sc is then made per channel
   double A;
+  // Exact-symbol mode (plan option): the multiplier in float64 from the
+  // exact wavenumbers instead of the reference's float32 table.
+  int exact;
+  double scd, dtd, inv_ndd[3];
 };
 
 // float32 symbol scalar of channel c: the same double expression the host
@@ -42,6 +46,22 @@ __device__ __forceinline__ float symbol_scalar(const Prefac& p, int64_t c) {
   if (!p.chan) return p.sc;
   const double e = p.chan[2 * c], D = p.chan[2 * c + 1];
   return p.ch ? static_cast<float>(((-2.0 * e) * D) * p.A) : static_cast<float>((-D) * p.A);
+}
+
+// Exact-symbol multiplier dt/(1 - dt*sigma(k)) in float64, channel c.
+__device__ __forceinline__ double prefac_f64(int jx, int jy, int jz, const Prefac& p, int64_t c) {
+  double sc = p.scd;
+  if (p.chan) {
+    const double e = p.chan[2 * c], D = p.chan[2 * c + 1];
+    sc = p.ch ? ((-2.0 * e) * D) * p.A : (-D) * p.A;
+  }
+  const int sx = jx < (p.nx + 1) / 2 ? jx : jx - p.nx;
+  const int sy = jy < (p.ny + 1) / 2 ? jy : jy - p.ny;
+  const double tau = 6.283185307179586476925286766559;
+  const double kx = tau * (sx * p.inv_ndd[0]), ky = tau * (sy * p.inv_ndd[1]), kz = tau * (jz * p.inv_ndd[2]);
+  const double k2 = kx * kx + ky * ky + kz * kz;
+  const double sym = p.ch ? sc * (k2 * k2) : sc * k2;
+  return p.dtd / (1.0 - p.dtd * sym);
 }
 
 __device__ __forceinline__ float prefac_f32(int jx, int jy, int jz, const Prefac& p, float sc) {
@@ -242,9 +262,16 @@ __device__ __forceinline__ void strided_tile(cpx<T>* __restrict__ spec, int64_t 
     const int jz = (int)(L - outer * lm.nh);
     const int jy = (int)(outer % pf.ny);
     const T sc = static_cast<T>(pf.scale);
-    const float ssc = symbol_scalar(pf, active ? outer / pf.ny : 0);
+    const int64_t ch = active ? outer / pf.ny : 0;
+    if (pf.exact) {
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = cscale(v[q], static_cast<T>(prefac_f32(tl + TPL * q, jy, jz, pf, ssc)) * sc);
+      for (int q = 0; q < E; ++q)
+        v[q] = cscale(v[q], static_cast<T>(prefac_f64(tl + TPL * q, jy, jz, pf, ch) * pf.scale));
+    } else {
+      const float ssc = symbol_scalar(pf, ch);
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = cscale(v[q], static_cast<T>(prefac_f32(tl + TPL * q, jy, jz, pf, ssc)) * sc);
+    }
     fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
   }
   if (active) {

@@ -53,17 +53,27 @@ def _params_tensor(params, channels, device) -> torch.Tensor:
 
 class CHImexEngine:
     """Forward and reverse semi-implicit CH steps for a batch of trajectories
-    of shape (channels, nx, ny, nz) on one grid."""
+    of shape (channels, nx, ny, nz) on one grid.
 
-    def __init__(self, vg, channels: int, dt: float, A: float = 0.25):
+    ``exact_symbol=False`` keeps the reference's float32 k-table multiplier
+    (timesteppers.py:77, voxelgrid.py:84-114), so forward states equal the
+    solver's.  ``exact_symbol=True`` evaluates the multiplier in float64 from
+    the exact wavenumbers: the step is then a smooth function of (eps, D) in
+    float64, which finite-difference checks and float64 estimation need."""
+
+    def __init__(self, vg, channels: int, dt: float, A: float = 0.25, exact_symbol: bool = False):
+        from .voxelgrid import _FFTPlan
         self.vg = vg
         self.dt = float(dt)
         self.A = float(A)
         self.shape = (int(channels),) + tuple(vg.shape)
         self.dtype = vg.precision
         self.device = vg.device
-        self.plan = vg.plan(self.shape, self.dtype)
+        self.plan = _FFTPlan(_lib.dtype_code(self.dtype), *self.shape)  # own plan: options are per plan
         lib = _lib.load()
+        _lib.check(lib.evx_fft_plan_set_option(self.plan.handle, _lib.OPT_EXACT_SYMBOL, int(bool(exact_symbol))),
+                   "evx_fft_plan_set_option")
+        self.exact_symbol = bool(exact_symbol)
         n = C.c_size_t()
         _lib.check(lib.evx_ch_imex_vjp_workspace_bytes(self.plan.handle, C.byref(n)),
                    "evx_ch_imex_vjp_workspace_bytes")
@@ -263,10 +273,11 @@ class InversionModel:
     dt: float
     A: float = 0.25
     pos_params: Sequence[str] = ("eps", "D")
+    exact_symbol: bool = False
     engine: CHImexEngine = field(init=False, repr=False)
 
     def __post_init__(self):
-        self.engine = CHImexEngine(self.vg, self.channels, self.dt, self.A)
+        self.engine = CHImexEngine(self.vg, self.channels, self.dt, self.A, self.exact_symbol)
 
     def residuals(self, params, u0, obs_steps, values):
         """values - predictions at obs_steps (inversion.py:121-124)."""

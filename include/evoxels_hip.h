@@ -238,30 +238,22 @@ int evx_batch_workspace_bytes(const evx_batch_plan* plan, size_t* bytes);
  * evx_imex_params.  Set-up call (synchronous upload). */
 int evx_batch_set_params(evx_batch_plan* plan, const double* eps_D, double dt,
                          double A);
-/* Schedules: EVX_BATCH_PERSISTENT (one launch per step; workgroups pull
- * (grid, sweep, tile) items from a device queue with per-grid sweep
- * dependencies) or EVX_BATCH_STAGED (one launch per sweep).  blocks = 0
- * means three workgroups per CU; xchunk = x planes per CH tile.  Set-up
- * call: no step of this plan may be in flight. */
-enum evx_batch_mode { EVX_BATCH_STAGED = 0, EVX_BATCH_PERSISTENT = 1 };
-int evx_batch_set_schedule(evx_batch_plan* plan, int mode, int blocks,
-                           int xchunk);
-int evx_batch_get_schedule(const evx_batch_plan* plan, int* mode, int* blocks,
-                           int* xchunk, int* n_items);
-/* Times every candidate schedule on scratch_state (overwritten) and keeps
+/* x planes per Cahn-Hilliard tile (default 8); the other sweeps' tiles
+ * follow from the FFT sizes.  Set-up call: no step of this plan may be in
+ * flight. */
+int evx_batch_set_tiles(evx_batch_plan* plan, int xchunk);
+int evx_batch_get_tiles(const evx_batch_plan* plan, int* xchunk, int* n_items);
+/* Times every candidate tile shape on scratch_state (overwritten) and keeps
  * the fastest; best_ms (may be NULL) receives its time per step.
  * Synchronises the stream. */
 int evx_batch_autotune(evx_batch_plan* plan, void* scratch_state,
                        void* workspace, int steps, void* stream,
                        double* best_ms);
-/* One semi-implicit step of every grid, in place on the packed state.
- * Asynchronous, allocation-free, capturable in a graph. */
+/* One semi-implicit step of every grid, in place on the packed state: six
+ * launches (one per sweep), each covering every grid.  Asynchronous,
+ * allocation-free, capturable in a graph. */
 int evx_batch_ch_imex_step(evx_batch_plan* plan, void* state, void* workspace,
                            void* stream);
-/* Reads (synchronously) the plan's fault flag: nonzero if a dependency wait
- * of the persistent schedule ever hit its bound. */
-int evx_batch_error_flag(evx_batch_plan* plan, int* flag);
-
 /* Stage timing for benchmarks (no reference counterpart; the reference times
  * whole runs with a wall clock, solvers.py:108-114).  Arms the next
  * evx_ch_imex_step / evx_rd_imex_step on this plan to record the caller's
@@ -272,11 +264,16 @@ int evx_batch_error_flag(evx_batch_plan* plan, int* flag);
 int evx_fft_plan_trace_next_step(evx_fft_plan* plan, void* const* events,
                                  int n_events);
 
-/* Kernel-selection options of a plan (no reference counterpart; used by tests
- * and benchmarks to compare kernel variants on identical inputs).
- *   EVX_OPT_SPECIALIZED  1 (default): size-specialised FFT kernels where one
- *                        exists; 0: runtime-radix kernels for every size. */
-enum evx_plan_option { EVX_OPT_SPECIALIZED = 1 };
+/* Options of a plan (no reference counterpart).
+ *   EVX_OPT_SPECIALIZED   1 (default): size-specialised FFT kernels where one
+ *                         exists; 0: runtime-radix kernels for every size
+ *                         (tests and benchmarks compare the two).
+ *   EVX_OPT_EXACT_SYMBOL  0 (default): the semi-implicit multiplier follows
+ *                         the reference's float32 k-table; 1: it is computed
+ *                         in float64 from the exact wavenumbers, a smooth
+ *                         function of eps and D for finite-difference checks
+ *                         and float64 parameter estimation. */
+enum evx_plan_option { EVX_OPT_SPECIALIZED = 1, EVX_OPT_EXACT_SYMBOL = 2 };
 int evx_fft_plan_set_option(evx_fft_plan* plan, int option, int value);
 
 /* ---- reductions ----------------------------------------------------------- */

@@ -145,41 +145,69 @@ def test_imex_step_vjp_matches_oracle_autograd(shape, spacing):
         assert rel(dparams[c, 1].cpu(), gd) < 2e-5
 
 
-def make_inverse_problem(shape, precision, n_steps, obs_steps, dt=0.1, seed=1):
+def make_inverse_problem(shape, precision, n_steps, obs_steps, dt=0.1, seed=1, exact=False):
     vf, vg = grid_for(shape, precision)
     rng = np.random.default_rng(seed)
     u0 = torch.tensor(0.5 + 0.05 * rng.standard_normal((2,) + shape), dtype=vg.precision, device="cuda")
-    eng = CHImexEngine(vg, 2, dt)
+    eng = CHImexEngine(vg, 2, dt, exact_symbol=exact)
     truth = np.array([[3.0, 1.0], [2.4, 1.3]])
     obs = solve(eng, u0, truth, n_steps, obs_steps)
     return vg, eng, u0, truth, obs
 
 
-def test_trajectory_gradient_matches_central_differences():
-    """float64, 2 trajectories, 12 steps, observations at 4, 8, 12, checkpoints
-    every 5 steps (a short last segment): dJ/d(eps, D) against central
-    differences with a relative step of 1e-4."""
-    n_steps, obs_steps = 12, [4, 8, 12]
-    vg, eng, u0, truth, obs = make_inverse_problem((16, 16, 16), "float64", n_steps, obs_steps)
-    prm = truth * np.array([[1.1, 0.92], [0.9, 1.07]])
-    loss, grad = loss_and_grad(eng, u0, prm, n_steps, obs_steps, obs, checkpoint_every=5)
-    grad = grad.cpu().numpy()
-    fd = np.zeros_like(grad)
-    for c in range(2):
+def central_differences(eng, u0, prm, n_steps, obs_steps, obs, rel_h):
+    fd = np.zeros_like(prm)
+    for c in range(prm.shape[0]):
         for j in range(2):
-            h = 1e-4 * prm[c, j]
+            h = rel_h * prm[c, j]
             pp, pm = prm.copy(), prm.copy()
             pp[c, j] += h
             pm[c, j] -= h
-            lp, _ = loss_and_grad(eng, u0, pp, n_steps, obs_steps, obs, checkpoint_every=5)
-            lm, _ = loss_and_grad(eng, u0, pm, n_steps, obs_steps, obs, checkpoint_every=5)
-            fd[c, j] = (float(lp[c]) - float(lm[c])) / (2 * h)
-    print("adjoint", grad, "fd", fd)
-    assert np.all(np.abs(grad - fd) <= 1e-4 * np.abs(fd).max(axis=1, keepdims=True))
+            lp = solve_loss(eng, u0, pp, n_steps, obs_steps, obs)
+            lm = solve_loss(eng, u0, pm, n_steps, obs_steps, obs)
+            fd[c, j] = (lp[c] - lm[c]) / (2 * h)
+    return fd
+
+
+def solve_loss(eng, u0, prm, n_steps, obs_steps, obs):
+    states = solve(eng, u0, prm, n_steps, obs_steps)
+    return [sum(0.5 * float(((s[c].double() - o[c].double()) ** 2).sum()) for s, o in zip(states, obs))
+            for c in range(u0.shape[0])]
+
+
+def test_trajectory_gradient_matches_central_differences():
+    """float64 with the exact-symbol multiplier, 2 trajectories, 12 steps,
+    observations at 4, 8, 12, checkpoints every 5 steps (a short last
+    segment): dJ/d(eps, D) against central differences with a relative step
+    of 1e-4, to 1e-4 of the largest gradient component of the trajectory."""
+    n_steps, obs_steps = 12, [4, 8, 12]
+    vg, eng, u0, truth, obs = make_inverse_problem((16, 16, 16), "float64", n_steps, obs_steps, exact=True)
+    prm = truth * np.array([[1.1, 0.92], [0.9, 1.07]])
+    loss, grad = loss_and_grad(eng, u0, prm, n_steps, obs_steps, obs, checkpoint_every=5)
+    grad = grad.cpu().numpy()
+    fd = central_differences(eng, u0, prm, n_steps, obs_steps, obs, 1e-4)
+    err = np.abs(grad - fd) / np.abs(fd).max(axis=1, keepdims=True)
+    print("adjoint", grad, "fd", fd, "rel err", err)
+    assert np.all(err <= 1e-4)
     # the checkpoint interval does not change the gradient
     _, g1 = loss_and_grad(eng, u0, prm, n_steps, obs_steps, obs, checkpoint_every=1)
     _, g20 = loss_and_grad(eng, u0, prm, n_steps, obs_steps, obs, checkpoint_every=20)
     assert rel(g1.cpu(), grad) < 1e-12 and rel(g20.cpu(), grad) < 1e-12
+
+
+def test_trajectory_gradient_reference_multiplier_central_differences():
+    """The reference's float32 multiplier is piecewise constant in (eps, D) at
+    the 1e-7 level, so differences need a larger step: 1e-3 relative, to 1e-3
+    of the largest gradient component."""
+    n_steps, obs_steps = 12, [4, 8, 12]
+    vg, eng, u0, truth, obs = make_inverse_problem((16, 16, 16), "float64", n_steps, obs_steps)
+    prm = truth * np.array([[1.1, 0.92], [0.9, 1.07]])
+    _, grad = loss_and_grad(eng, u0, prm, n_steps, obs_steps, obs, checkpoint_every=5)
+    grad = grad.cpu().numpy()
+    fd = central_differences(eng, u0, prm, n_steps, obs_steps, obs, 1e-3)
+    err = np.abs(grad - fd) / np.abs(fd).max(axis=1, keepdims=True)
+    print("adjoint", grad, "fd", fd, "rel err", err)
+    assert np.all(err <= 1e-3)
 
 
 def test_trajectory_gradient_matches_oracle_autograd_and_autograd_function():

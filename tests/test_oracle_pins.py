@@ -1,6 +1,8 @@
 """Pins of the CPU oracle (oracle/evoxels_oracle.py) against the reference's own
 known answers and against what the mathematics fixes.  CPU only."""
+import json
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -12,6 +14,7 @@ from tests import mms
 
 CS = mms.CS
 P3 = (("periodic", None),) * 3
+KAT = json.loads((Path(__file__).parent / "golden" / "reference_kats.json").read_text())
 
 
 def grid(shape, domain=None, precision="float64"):
@@ -24,14 +27,10 @@ def test_pad_kat_reference_mixed_bc():
     """test_solvers.py:105-126, the reference's exact padded array."""
     with pytest.warns(UserWarning, match="order of convergence to 0.5"):
         bc = O.normalize_bc((("dirichlet", (10.0, 20.0)), "neumann", "periodic"))
-    f = torch.tensor(np.arange(1, 9, dtype=np.float32).reshape(1, 2, 2, 2))
+    k = KAT["pad_mixed_bc"]
+    f = torch.tensor(np.array(k["input"], dtype=np.float32).reshape([1] + k["input_shape"]))
     padded = O.pad_bc(f, bc)[0].numpy()
-    expected = np.pad(np.arange(1, 9, dtype=np.float32).reshape(2, 2, 2), 1, mode="wrap")
-    expected[0, :, :] = 2.0 * 10.0 - expected[1, :, :]
-    expected[-1, :, :] = 2.0 * 20.0 - expected[-2, :, :]
-    expected[:, 0, :] = expected[:, 1, :]
-    expected[:, -1, :] = expected[:, -2, :]
-    assert np.array_equal(padded, expected)
+    assert np.array_equal(padded, np.array(k["expected_padded"], dtype=np.float32))
 
 
 def test_bc_normalisation_and_errors():
@@ -112,7 +111,7 @@ def test_mms_neumann_laplace_matches_notebook_slope():
     e = sp.cos(sp.pi * CS.x) ** 3
     bc = (("neumann", None), ("periodic", None), ("periodic", None))
     _, _, slope = mms.convergence(e, mms.rd_analytic(e), _oracle_rhs("rd", bc, "float32"), "float32")
-    assert abs(slope - 1.9533270430146406) < 2e-3
+    assert abs(slope - KAT["neumann_laplace_slope"]["value"]) < 2e-3
 
 
 def test_mms_dirichlet_laplace_matches_notebook_slope():
@@ -120,7 +119,7 @@ def test_mms_dirichlet_laplace_matches_notebook_slope():
     e = (CS.x * (1 - CS.x)) ** 2
     bc = (("dirichlet", (0, 0)), ("periodic", None), ("periodic", None))
     _, _, slope = mms.convergence(e, mms.rd_analytic(e), _oracle_rhs("rd", bc, "float32"), "float32")
-    assert abs(slope - 0.4954937374912848) < 2e-3
+    assert abs(slope - KAT["dirichlet_laplace_slope"]["value"]) < 2e-3
 
 
 def test_mms_cahn_hilliard_slope():
@@ -137,8 +136,8 @@ def test_mms_allen_cahn_matches_notebook_errors():
     bc = (("neumann", None),) * 3
     _, errs, slope = mms.convergence(e, mms.ac_analytic(e, eps=3.0, curvature=0.5),
                                      _oracle_rhs("ac", bc, "float64", eps=3.0, curvature=0.5))
-    np.testing.assert_allclose(errs, [0.19028211, 0.05396869, 0.01346211, 0.00334846, 0.00102839], rtol=5e-6)
-    assert abs(slope - 1.907376693930398) < 1e-5
+    np.testing.assert_allclose(errs, KAT["allen_cahn_errors"]["errors"], rtol=5e-6)
+    assert abs(slope - KAT["allen_cahn_errors"]["slope"]) < 1e-5
     _, _, slope_f = mms.convergence(e, mms.ac_analytic(e, eps=3.0, curvature=0.5, force=1.0),
                                     _oracle_rhs("ac", bc, "float64", eps=3.0, curvature=0.5, force=1.0))
     assert abs(slope_f - 2) < 0.1
