@@ -29,18 +29,22 @@ constexpr int BT = 256;  // threads of every batched workgroup
 // FFT sizes the batched kernels are built for (all give 256-thread tiles).
 #define EVX_BATCH_SIZES(X) X(16) X(32) X(48) X(64) X(96) X(128)
 
+constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
+
 template <int N>
 constexpr size_t fft_lds() {
-  return ZTile<T, N>::LDS_BYTES > STile<T, N>::LDS_BYTES ? ZTile<T, N>::LDS_BYTES : STile<T, N>::LDS_BYTES;
+  return cmax(ZTile<T, N>::LDS_BYTES, cmax(STile<T, N, 0>::LDS_BYTES, STile<T, N, 2>::LDS_BYTES));
 }
-constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
 constexpr size_t BATCH_LDS =
     cmax(ChTile<T>::LDS_BYTES,
          cmax(fft_lds<16>(), cmax(fft_lds<32>(), cmax(fft_lds<48>(), cmax(fft_lds<64>(), cmax(fft_lds<96>(),
                                                                                            fft_lds<128>()))))));
 static_assert(BATCH_LDS <= 160 * 1024 / 3, "three batched workgroups must fit one CU's LDS");
 
-#define X(N) static_assert(ZGeo<T, N>::THREADS == BT && SGeo<T, N>::THREADS == BT, "256-thread tiles");
+#define X(N)                                                                                 \
+  static_assert(ZGeo<T, N>::THREADS == BT && SGeo<T, N, 0>::THREADS == BT && SGeo<T, N, 2>::THREADS == BT && \
+                    SGeo<T, N, 0>::TL == SGeo<T, N, 2>::TL,                                       \
+                "256-thread tiles");
 EVX_BATCH_SIZES(X)
 #undef X
 
@@ -121,7 +125,7 @@ __device__ __forceinline__ void run_item(const BArgs& a, int4 it, int tid, unsig
       switch (b.nz) {
 #define X(N)                                                                                                   \
   case N:                                                                                                      \
-    z_r2c_tile<T, N>(a.rhs + b.r_off, a.spec + b.s_off, nlines, a.tw + b.tw[2], tile, tid, smem); \
+    z_r2c_tile<T, N>(a.rhs + b.r_off, a.spec + b.s_off, nlines, b.nh, a.tw + b.tw[2], tile, tid, smem); \
     break;
         EVX_BATCH_SIZES(X)
 #undef X
@@ -145,7 +149,7 @@ __device__ __forceinline__ void run_item(const BArgs& a, int4 it, int tid, unsig
       switch (b.nz) {
 #define X(N)                                                                                              \
   case N:                                                                                                 \
-    z_c2r_tile<T, N>(a.spec + b.s_off, u, u, T(1), nlines, a.tw + b.tw[2], tile, tid, smem); \
+    z_c2r_tile<T, N>(a.spec + b.s_off, u, u, T(1), nlines, b.nh, a.tw + b.tw[2], tile, tid, smem); \
     break;
         EVX_BATCH_SIZES(X)
 #undef X

@@ -2,6 +2,8 @@
 //
 // Spectrum layout: (channels, nx, ny, nh) complex, nh = nz/2 + 1, kz fastest,
 // the same index order as torch.fft.rfftn's output (voxelgrid.py:203-204).
+// Kernels take the row pitch of the spectrum separately from the row length
+// (nh); the steps' workspace spectrum uses pitch = nh.
 //
 // Transforms are mixed-radix Stockham (self-sorting, decimation in time)
 // passes on lines staged in LDS.  A workgroup owns `lpw` lines laid out
@@ -154,7 +156,7 @@ __device__ __forceinline__ void load_twiddles(cpx<T>* tw, const Axis<T>& a) {
 // Real lines of length n along z; line pairs (2q, 2q+1) form one complex line.
 template <typename T>
 __global__ void z_r2c_kernel(const T* __restrict__ x, cpx<T>* __restrict__ spec, int64_t nlines, int nh,
-                             Axis<T> a, int lpw) {
+                             int64_t pitch, Axis<T> a, int lpw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* buf = tw + a.n;
@@ -173,7 +175,7 @@ __global__ void z_r2c_kernel(const T* __restrict__ x, cpx<T>* __restrict__ spec,
   __syncthreads();
   const int line = tid % lpw, tl = tid / lpw;
   fft_lines<T, false>(buf, lpw, line, tl, a, tw, line < np && tl < a.tpl);
-  cpx<T>* dst = spec + l0 * nh;
+  cpx<T>* dst = spec + l0 * pitch;
   for (int q = tid; q < nl * nh; q += blockDim.x) {
     const int l = q / nh, k = q - (q / nh) * nh;
     const cpx<T> Z = buf[k * lpw + (l >> 1)];
@@ -181,7 +183,7 @@ __global__ void z_r2c_kernel(const T* __restrict__ x, cpx<T>* __restrict__ spec,
     cpx<T> o;
     if ((l & 1) == 0) o = {(Z.x + Zc.x) * T(0.5), (Z.y - Zc.y) * T(0.5)};
     else o = {(Z.y + Zc.y) * T(0.5), (Zc.x - Z.x) * T(0.5)};
-    dst[q] = o;
+    dst[l * pitch + k] = o;
   }
 }
 
@@ -195,7 +197,7 @@ __device__ __forceinline__ cpx<T> herm_load(const cpx<T>* line, int k, int n, in
 // out = base + scale * irfft(spec) per line (base may be NULL; out may alias base).
 template <typename T>
 __global__ void z_c2r_kernel(const cpx<T>* __restrict__ spec, T* out, const T* base, T scale, int64_t nlines,
-                             int nh, Axis<T> a, int lpw) {
+                             int nh, int64_t pitch, Axis<T> a, int lpw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* buf = tw + a.n;
@@ -206,9 +208,9 @@ __global__ void z_c2r_kernel(const cpx<T>* __restrict__ spec, T* out, const T* b
   const int np = (nl + 1) / 2;
   for (int q = tid; q < np * n; q += blockDim.x) {
     const int pr = q / n, k = q - (q / n) * n;
-    const cpx<T>* la = spec + (l0 + 2 * pr) * nh;
+    const cpx<T>* la = spec + (l0 + 2 * pr) * pitch;
     const cpx<T> A = herm_load(la, k, n, nh);
-    const cpx<T> B = (2 * pr + 1 < nl) ? herm_load(la + nh, k, n, nh) : cpx<T>{T(0), T(0)};
+    const cpx<T> B = (2 * pr + 1 < nl) ? herm_load(la + pitch, k, n, nh) : cpx<T>{T(0), T(0)};
     buf[k * lpw + pr] = {A.x - B.y, A.y + B.x};
   }
   __syncthreads();
@@ -225,7 +227,8 @@ __global__ void z_c2r_kernel(const cpx<T>* __restrict__ spec, T* out, const T* b
 }
 
 
-// MODE 0: forward, 1: inverse, 2: forward * prefac * inverse (x axis only)
+// MODE 0: forward, 1: inverse, 2/3: forward * prefac * inverse (x axis only;
+// 3 with the exact-symbol multiplier)
 template <typename T, int MODE>
 __global__ void strided_kernel(cpx<T>* spec, int64_t nlines, LineMap lm, Axis<T> a, int lpw, Prefac pf) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -247,7 +250,7 @@ __global__ void strided_kernel(cpx<T>* spec, int64_t nlines, LineMap lm, Axis<T>
   } else {
     fft_lines<T, false>(buf, lpw, line, tl, a, tw, active);
   }
-  if (MODE == 2) {
+  if (MODE >= 2) {
     const T sc = static_cast<T>(pf.scale);
     if (line < nl) {
       const int64_t L = L0 + line;
@@ -271,36 +274,37 @@ __global__ void strided_kernel(cpx<T>* spec, int64_t nlines, LineMap lm, Axis<T>
 
 template <typename T, int N>
 __global__ __launch_bounds__((ZGeo<T, N>::THREADS)) void z_r2c_fast(const T* __restrict__ x, cpx<T>* __restrict__ spec,
-                                                                  int64_t nlines, const cpx<T>* __restrict__ twg) {
+                                                                  int64_t nlines, int64_t pitch,
+                                                                  const cpx<T>* __restrict__ twg) {
   __shared__ __align__(16) unsigned char smem[ZTile<T, N>::LDS_BYTES];
-  z_r2c_tile<T, N>(x, spec, nlines, twg, blockIdx.x, threadIdx.x, smem);
+  z_r2c_tile<T, N>(x, spec, nlines, pitch, twg, blockIdx.x, threadIdx.x, smem);
 }
 
 template <typename T, int N>
 __global__ __launch_bounds__((ZGeo<T, N>::THREADS)) void z_c2r_fast(const cpx<T>* __restrict__ spec, T* out,
                                                                   const T* base, T scale, int64_t nlines,
-                                                                  const cpx<T>* __restrict__ twg) {
+                                                                  int64_t pitch, const cpx<T>* __restrict__ twg) {
   __shared__ __align__(16) unsigned char smem[ZTile<T, N>::LDS_BYTES];
-  z_c2r_tile<T, N>(spec, out, base, scale, nlines, twg, blockIdx.x, threadIdx.x, smem);
+  z_c2r_tile<T, N>(spec, out, base, scale, nlines, pitch, twg, blockIdx.x, threadIdx.x, smem);
 }
 
 template <typename T, int N, int MODE>
-__global__ __launch_bounds__((SGeo<T, N>::THREADS)) void strided_fast(cpx<T>* __restrict__ spec, int64_t nlines,
+__global__ __launch_bounds__((SGeo<T, N, MODE>::THREADS)) void strided_fast(cpx<T>* __restrict__ spec, int64_t nlines,
                                                                     LineMap lm, const cpx<T>* __restrict__ twg,
                                                                     Prefac pf) {
-  __shared__ __align__(16) unsigned char smem[STile<T, N>::LDS_BYTES];
+  __shared__ __align__(16) unsigned char smem[STile<T, N, MODE>::LDS_BYTES];
   strided_tile<T, N, MODE>(spec, nlines, lm, twg, pf, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, smem);
 }
 
 template <typename T>
-bool fast_z_r2c(int n, const void* x, void* spec, int64_t nlines, const void* tw, hipStream_t s) {
+bool fast_z_r2c(int n, const void* x, void* spec, int64_t nlines, int64_t pitch, const void* tw, hipStream_t s) {
   switch (n) {
 #define X(NN)                                                                                               \
   case NN: {                                                                                                \
     using G = ZGeo<T, NN>;                                                                                  \
     const int64_t blocks = ((nlines + 1) / 2 + G::LP - 1) / G::LP;                                          \
     hipLaunchKernelGGL((z_r2c_fast<T, NN>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,                 \
-                       static_cast<const T*>(x), static_cast<cpx<T>*>(spec), nlines,                        \
+                       static_cast<const T*>(x), static_cast<cpx<T>*>(spec), nlines, pitch,                 \
                        static_cast<const cpx<T>*>(tw));                                                     \
     return true;                                                                                            \
   }
@@ -313,7 +317,7 @@ bool fast_z_r2c(int n, const void* x, void* spec, int64_t nlines, const void* tw
 
 template <typename T>
 bool fast_z_c2r(int n, const void* spec, void* out, const void* base, double scale, int64_t nlines,
-                const void* tw, hipStream_t s) {
+                int64_t pitch, const void* tw, hipStream_t s) {
   switch (n) {
 #define X(NN)                                                                                               \
   case NN: {                                                                                                \
@@ -321,7 +325,7 @@ bool fast_z_c2r(int n, const void* spec, void* out, const void* base, double sca
     const int64_t blocks = ((nlines + 1) / 2 + G::LP - 1) / G::LP;                                          \
     hipLaunchKernelGGL((z_c2r_fast<T, NN>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,                 \
                        static_cast<const cpx<T>*>(spec), static_cast<T*>(out), static_cast<const T*>(base), \
-                       static_cast<T>(scale), nlines, static_cast<const cpx<T>*>(tw));                      \
+                       static_cast<T>(scale), nlines, pitch, static_cast<const cpx<T>*>(tw));               \
     return true;                                                                                            \
   }
     EVX_FAST_SIZES(X)
@@ -337,7 +341,7 @@ bool fast_strided(int n, void* spec, int64_t nlines, const LineMap& lm, const vo
   switch (n) {
 #define X(NN)                                                                                               \
   case NN: {                                                                                                \
-    using G = SGeo<T, NN>;                                                                                  \
+    using G = SGeo<T, NN, MODE>;                                                                            \
     const int64_t blocks = (nlines + G::TL - 1) / G::TL;                                                    \
     hipLaunchKernelGGL((strided_fast<T, NN, MODE>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,         \
                        static_cast<cpx<T>*>(spec), nlines, lm, static_cast<const cpx<T>*>(tw), pf);          \
@@ -403,6 +407,7 @@ struct evx_fft_plan {
   int dtype;
   evx::Dims d;
   int64_t nh;
+  int64_t pitch;  // row pitch (complex values) of the workspace spectrum, >= nh
   evx::AxisHost ax[3];  // x, y, z (z: complex length nz)
   // One-shot stage trace: events recorded around the kernels of the next step.
   hipEvent_t trace[8] = {};
@@ -484,32 +489,34 @@ Geom strided_geom(const AxisHost& h, int64_t nlines) {
 }
 
 template <typename T>
-int run_z_r2c(evx_fft_plan* p, const void* x, void* spec, hipStream_t s) {
+int run_z_r2c(evx_fft_plan* p, const void* x, void* spec, int64_t pitch, hipStream_t s) {
   const AxisHost& h = p->ax[2];
   const int64_t nlines = p->d.c * p->d.nx * p->d.ny;
-  if (p->fast && fast_z_r2c<T>(h.n, x, spec, nlines, h.tw, s)) return check_launch("z_r2c_fast");
+  if (p->fast && fast_z_r2c<T>(h.n, x, spec, nlines, pitch, h.tw, s)) return check_launch("z_r2c_fast");
   const Geom g = z_geom<T>(h, (nlines + 1) / 2);
   const int64_t blocks = (nlines + 2 * g.lpw - 1) / (2 * g.lpw);
   hipLaunchKernelGGL(z_r2c_kernel<T>, dim3((unsigned)blocks), dim3(g.threads), g.lds, s, static_cast<const T*>(x),
-                     static_cast<cpx<T>*>(spec), nlines, (int)p->nh, axis_dev<T>(h, g.tpl), g.lpw);
+                     static_cast<cpx<T>*>(spec), nlines, (int)p->nh, pitch, axis_dev<T>(h, g.tpl), g.lpw);
   return check_launch("z_r2c_kernel");
 }
 
 template <typename T>
-int run_z_c2r(evx_fft_plan* p, const void* spec, void* out, const void* base, double scale, hipStream_t s) {
+int run_z_c2r(evx_fft_plan* p, const void* spec, void* out, const void* base, double scale, int64_t pitch,
+              hipStream_t s) {
   const AxisHost& h = p->ax[2];
   const int64_t nlines = p->d.c * p->d.nx * p->d.ny;
-  if (p->fast && fast_z_c2r<T>(h.n, spec, out, base, scale, nlines, h.tw, s)) return check_launch("z_c2r_fast");
+  if (p->fast && fast_z_c2r<T>(h.n, spec, out, base, scale, nlines, pitch, h.tw, s))
+    return check_launch("z_c2r_fast");
   const Geom g = z_geom<T>(h, (nlines + 1) / 2);
   const int64_t blocks = (nlines + 2 * g.lpw - 1) / (2 * g.lpw);
   hipLaunchKernelGGL(z_c2r_kernel<T>, dim3((unsigned)blocks), dim3(g.threads), g.lds, s,
                      static_cast<const cpx<T>*>(spec), static_cast<T*>(out), static_cast<const T*>(base),
-                     static_cast<T>(scale), nlines, (int)p->nh, axis_dev<T>(h, g.tpl), g.lpw);
+                     static_cast<T>(scale), nlines, (int)p->nh, pitch, axis_dev<T>(h, g.tpl), g.lpw);
   return check_launch("z_c2r_kernel");
 }
 
 template <typename T, int MODE>
-int run_strided(evx_fft_plan* p, int axis, void* spec, const Prefac& pf, hipStream_t s) {
+int run_strided(evx_fft_plan* p, int axis, void* spec, const Prefac& pf, int64_t pitch, hipStream_t s) {
   const AxisHost& h = p->ax[axis];
   const Dims& d = p->d;
   LineMap lm;
@@ -518,17 +525,17 @@ int run_strided(evx_fft_plan* p, int axis, void* spec, const Prefac& pf, hipStre
   if (axis == 1) {
     nlines = d.c * d.nx * p->nh;
     lm.G = 1;
-    lm.GM = d.ny * p->nh;
+    lm.GM = d.ny * pitch;
     lm.OM = 0;
-    lm.es = p->nh;
+    lm.es = pitch;
   } else {
     nlines = d.c * d.ny * p->nh;
     lm.G = d.ny;
-    lm.GM = d.nx * d.ny * p->nh;
-    lm.OM = p->nh;
-    lm.es = d.ny * p->nh;
+    lm.GM = d.nx * d.ny * pitch;
+    lm.OM = pitch;
+    lm.es = d.ny * pitch;
   }
-  if (h.n == 1 && MODE != 2) return EVX_OK;  // length-1 transform is the identity
+  if (h.n == 1 && MODE < 2) return EVX_OK;  // length-1 transform is the identity
   if (p->fast && fast_strided<T, MODE>(h.n, spec, nlines, lm, h.tw, pf, s)) return check_launch("strided_fast");
   const Geom g = strided_geom<T>(h, nlines);
   const int64_t blocks = (nlines + g.lpw - 1) / g.lpw;
@@ -563,8 +570,9 @@ size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
 size_t real_bytes(const evx_fft_plan* p) {
   return align_up((size_t)p->d.c * p->d.vol() * (p->dtype == EVX_F32 ? 4 : 8));
 }
+// Bytes of the workspace spectrum (rows of p->pitch complex values).
 size_t spec_bytes(const evx_fft_plan* p) {
-  return align_up((size_t)p->d.c * p->d.nx * p->d.ny * p->nh * (p->dtype == EVX_F32 ? 8 : 16));
+  return align_up((size_t)p->d.c * p->d.nx * p->d.ny * p->pitch * (p->dtype == EVX_F32 ? 8 : 16));
 }
 
 // Records trace event k (if the caller armed one) on the step's stream.
@@ -578,15 +586,16 @@ int mark(evx_fft_plan* p, int k, hipStream_t s) {
 template <typename T>
 int spectral_update(evx_fft_plan* p, const void* rhs, void* spec, const void* u, void* u_out, const Prefac& pf,
                     hipStream_t s) {
-  EVX_TRY(run_z_r2c<T>(p, rhs, spec, s));
+  const int64_t pt = p->pitch;
+  EVX_TRY(run_z_r2c<T>(p, rhs, spec, pt, s));
   EVX_TRY(mark(p, 2, s));
-  EVX_TRY((run_strided<T, 0>(p, 1, spec, pf, s)));
+  EVX_TRY((run_strided<T, 0>(p, 1, spec, pf, pt, s)));
   EVX_TRY(mark(p, 3, s));
-  EVX_TRY((run_strided<T, 2>(p, 0, spec, pf, s)));
+  EVX_TRY((pf.exact ? run_strided<T, 3>(p, 0, spec, pf, pt, s) : run_strided<T, 2>(p, 0, spec, pf, pt, s)));
   EVX_TRY(mark(p, 4, s));
-  EVX_TRY((run_strided<T, 1>(p, 1, spec, pf, s)));
+  EVX_TRY((run_strided<T, 1>(p, 1, spec, pf, pt, s)));
   EVX_TRY(mark(p, 5, s));
-  EVX_TRY(run_z_c2r<T>(p, spec, u_out, u, 1.0, s));
+  EVX_TRY(run_z_c2r<T>(p, spec, u_out, u, 1.0, pt, s));
   EVX_TRY(mark(p, 6, s));
   p->ntrace = 0;
   return EVX_OK;
@@ -636,6 +645,10 @@ extern "C" int evx_fft_plan_create(evx_fft_plan** out, int32_t dtype, int64_t ch
   p->dtype = dtype;
   p->d = Dims{channels, nx, ny, nz};
   p->nh = nz / 2 + 1;
+  // Workspace spectrum rows are unpadded: padding them to 128-byte multiples
+  // measured slower on MI355X for every sweep (the odd row stride spreads a
+  // brick's rows over more memory channels than aligned ones).
+  p->pitch = p->nh;
   const int64_t n[3] = {nx, ny, nz};
   for (int a = 0; a < 3; ++a) {
     AxisHost& h = p->ax[a];
@@ -699,13 +712,13 @@ extern "C" int evx_rfftn(evx_fft_plan* p, const void* x, void* spec, void* strea
   hipStream_t s = static_cast<hipStream_t>(stream);
   Prefac pf{};
   if (p->dtype == EVX_F32) {
-    EVX_TRY(run_z_r2c<float>(p, x, spec, s));
-    EVX_TRY((run_strided<float, 0>(p, 1, spec, pf, s)));
-    return run_strided<float, 0>(p, 0, spec, pf, s);
+    EVX_TRY(run_z_r2c<float>(p, x, spec, p->nh, s));
+    EVX_TRY((run_strided<float, 0>(p, 1, spec, pf, p->nh, s)));
+    return run_strided<float, 0>(p, 0, spec, pf, p->nh, s);
   }
-  EVX_TRY(run_z_r2c<double>(p, x, spec, s));
-  EVX_TRY((run_strided<double, 0>(p, 1, spec, pf, s)));
-  return run_strided<double, 0>(p, 0, spec, pf, s);
+  EVX_TRY(run_z_r2c<double>(p, x, spec, p->nh, s));
+  EVX_TRY((run_strided<double, 0>(p, 1, spec, pf, p->nh, s)));
+  return run_strided<double, 0>(p, 0, spec, pf, p->nh, s);
 }
 
 extern "C" int evx_irfftn(evx_fft_plan* p, void* spec, void* y, void* stream) {
@@ -715,13 +728,13 @@ extern "C" int evx_irfftn(evx_fft_plan* p, void* spec, void* y, void* stream) {
   Prefac pf{};
   const double scale = 1.0 / ((double)p->d.nx * (double)p->d.ny * (double)p->d.nz);
   if (p->dtype == EVX_F32) {
-    EVX_TRY((run_strided<float, 1>(p, 0, spec, pf, s)));
-    EVX_TRY((run_strided<float, 1>(p, 1, spec, pf, s)));
-    return run_z_c2r<float>(p, spec, y, nullptr, scale, s);
+    EVX_TRY((run_strided<float, 1>(p, 0, spec, pf, p->nh, s)));
+    EVX_TRY((run_strided<float, 1>(p, 1, spec, pf, p->nh, s)));
+    return run_z_c2r<float>(p, spec, y, nullptr, scale, p->nh, s);
   }
-  EVX_TRY((run_strided<double, 1>(p, 0, spec, pf, s)));
-  EVX_TRY((run_strided<double, 1>(p, 1, spec, pf, s)));
-  return run_z_c2r<double>(p, spec, y, nullptr, scale, s);
+  EVX_TRY((run_strided<double, 1>(p, 0, spec, pf, p->nh, s)));
+  EVX_TRY((run_strided<double, 1>(p, 1, spec, pf, p->nh, s)));
+  return run_z_c2r<double>(p, spec, y, nullptr, scale, p->nh, s);
 }
 
 extern "C" int evx_ch_imex_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,

@@ -92,7 +92,9 @@ __device__ __forceinline__ void dft_fixed(cpx<T>* x) {
 // --------------------------------------------------------------------------
 // Compile-time engine.  Lay::at(e) returns the LDS slot of element e of the
 // calling thread's line; the engine writes pass outputs there, synchronises,
-// and reads them back in register ownership order.
+// and reads them back in register ownership order.  A layout with SPLIT set
+// holds one real number per element: the exchange then runs twice (real
+// parts, then imaginary parts) through half the LDS.
 // --------------------------------------------------------------------------
 template <int... R>
 struct Rad {};
@@ -128,18 +130,46 @@ __device__ __forceinline__ void stockham(cpx<T> (&v)[N / TPL], const Lay& lay, i
 #pragma unroll
       for (int r = 0; r < R; ++r) v[s + r * S] = y[s][r];
   } else {
-    __syncthreads();  // previous readers of the exchange buffer are done
+    if constexpr (Lay::SPLIT) {
+      // Half-size exchange buffer: real parts, then imaginary parts.
+      __syncthreads();  // previous readers of the exchange buffer are done
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      const int i = tl + s * TPL;
-      const int k = i % P;
-      const int j = (i - k) * R + k;
+      for (int s = 0; s < S; ++s) {
+        const int i = tl + s * TPL;
+        const int k = i % P;
+        const int j = (i - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) *lay.at(j + r * P) = y[s][r];
+        for (int r = 0; r < R; ++r) *lay.at(j + r * P) = y[s][r].x;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q].x = *lay.at(tl + TPL * q);
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int i = tl + s * TPL;
+        const int k = i % P;
+        const int j = (i - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) *lay.at(j + r * P) = y[s][r].y;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q].y = *lay.at(tl + TPL * q);
+    } else {
+      __syncthreads();  // previous readers of the exchange buffer are done
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int i = tl + s * TPL;
+        const int k = i % P;
+        const int j = (i - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) *lay.at(j + r * P) = y[s][r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = *lay.at(tl + TPL * q);
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = *lay.at(tl + TPL * q);
     stockham<T, N, TPL, INV, P * R, Lay, Rest...>(v, lay, tl, tw);
   }
 }

@@ -6,6 +6,8 @@
 #include "evx_common.hpp"
 #include "evx_fft_engine.hpp"
 
+#include <type_traits>
+
 namespace evx {
 
 // ---------------------------------------------------------- strided sweeps --
@@ -104,14 +106,24 @@ struct ZGeo {
 
 template <typename T, int LS>
 struct LineLay {
+  static constexpr bool SPLIT = false;
   cpx<T>* base;
   __device__ __forceinline__ cpx<T>* at(int e) const { return base + e + (e >> 4); }
 };
 
 template <typename T, int TL>
 struct BrickLay {
+  static constexpr bool SPLIT = false;
   cpx<T>* base;
   __device__ __forceinline__ cpx<T>* at(int e) const { return base + e * TL; }
+};
+
+// Brick of real numbers (one component at a time; see the engine).
+template <typename T, int TL>
+struct BrickLaySplit {
+  static constexpr bool SPLIT = true;
+  T* base;
+  __device__ __forceinline__ T* at(int e) const { return base + e * TL; }
 };
 
 // LDS of one z tile: twiddles plus LP padded lines.
@@ -123,7 +135,7 @@ struct ZTile {
 // z R2C of tile `tile` (LP line pairs): two real lines per complex FFT.
 template <typename T, int N>
 __device__ __forceinline__ void z_r2c_tile(const T* __restrict__ x, cpx<T>* __restrict__ spec, int64_t nlines,
-                                           const cpx<T>* __restrict__ twg, int64_t tile, int tid,
+                                           int64_t pitch, const cpx<T>* __restrict__ twg, int64_t tile, int tid,
                                            unsigned char* smem) {
   using G = ZGeo<T, N>;
   constexpr int TPL = G::TPL, E = N / TPL, NH = N / 2 + 1;
@@ -148,7 +160,7 @@ __device__ __forceinline__ void z_r2c_tile(const T* __restrict__ x, cpx<T>* __re
 #pragma unroll
   for (int q = 0; q < E; ++q) *lay.at(tl + TPL * q) = v[q];
   __syncthreads();
-  cpx<T>* sa = spec + la * NH;
+  cpx<T>* sa = spec + la * pitch;
 #pragma unroll
   for (int q = 0; q < (NH + TPL - 1) / TPL; ++q) {
     const int k = tl + TPL * q;
@@ -156,7 +168,7 @@ __device__ __forceinline__ void z_r2c_tile(const T* __restrict__ x, cpx<T>* __re
       const cpx<T> Z = v[q];
       const cpx<T> Zc = *lay.at(k == 0 ? 0 : N - k);
       sa[k] = cpx<T>{(Z.x + Zc.x) * T(0.5), (Z.y - Zc.y) * T(0.5)};
-      if (has_b) sa[NH + k] = cpx<T>{(Z.y + Zc.y) * T(0.5), (Zc.x - Z.x) * T(0.5)};
+      if (has_b) sa[pitch + k] = cpx<T>{(Z.y + Zc.y) * T(0.5), (Zc.x - Z.x) * T(0.5)};
     }
   }
 }
@@ -173,8 +185,8 @@ __device__ __forceinline__ cpx<T> herm_at(const cpx<T>* __restrict__ line, int k
 // out may alias base).
 template <typename T, int N>
 __device__ __forceinline__ void z_c2r_tile(const cpx<T>* __restrict__ spec, T* out, const T* base, T scale,
-                                           int64_t nlines, const cpx<T>* __restrict__ twg, int64_t tile, int tid,
-                                           unsigned char* smem) {
+                                           int64_t nlines, int64_t pitch, const cpx<T>* __restrict__ twg,
+                                           int64_t tile, int tid, unsigned char* smem) {
   using G = ZGeo<T, N>;
   constexpr int TPL = G::TPL, E = N / TPL, NH = N / 2 + 1;
   cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem);
@@ -183,13 +195,13 @@ __device__ __forceinline__ void z_c2r_tile(const cpx<T>* __restrict__ spec, T* o
   for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
   const int64_t la = (tile * G::LP + pr) * 2;
   const bool has_a = la < nlines, has_b = la + 1 < nlines;
-  const cpx<T>* sa = spec + la * NH;
+  const cpx<T>* sa = spec + la * pitch;
   cpx<T> v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int k = tl + TPL * q;
     const cpx<T> A = has_a ? herm_at<T, N>(sa, k) : cpx<T>{T(0), T(0)};
-    const cpx<T> B = has_b ? herm_at<T, N>(sa + NH, k) : cpx<T>{T(0), T(0)};
+    const cpx<T> B = has_b ? herm_at<T, N>(sa + pitch, k) : cpx<T>{T(0), T(0)};
     v[q] = cpx<T>{A.x - B.y, A.y + B.x};
   }
   // The state added to the update is loaded now so its latency overlaps the FFT.
@@ -221,27 +233,39 @@ constexpr int pick_strided_e(int n, int l, int tl0) {
   return e;
 }
 
-template <typename T, int N>
+template <typename T, int N, int MODE = 0>
 struct SGeo {
   static constexpr int TL0 = 128 / (int)sizeof(cpx<T>);
-  static constexpr int E = pick_strided_e(N, FftSpec<N>::L, TL0);
+  // The forward*prefac*inverse sweeps hold two transforms' worth of live
+  // values: from 512 points up they use half the values per thread (twice
+  // the threads per line) to stay off the register limit.
+  static constexpr int E0 = pick_strided_e(N, FftSpec<N>::L, TL0);
+  static constexpr int E = (MODE >= 2 && N >= 512 && E0 >= 2 * FftSpec<N>::L) ? E0 / 2 : E0;
   static constexpr int TPL = N / E;
   static constexpr int TL = TL0 > 256 / TPL ? TL0 : 256 / TPL;
   static constexpr int THREADS = TL * TPL;
+  // Bricks above 32 KB exchange real and imaginary parts in turn through a
+  // half-size buffer, so that more workgroups fit a CU.
+  static constexpr bool SPLIT = (size_t)TL * N * sizeof(cpx<T>) > 32768;
 };
 
-template <typename T, int N>
+// The forward*prefac*inverse sweeps (MODE 2, 3) are register-bound, not
+// LDS-bound, so they keep the full-size exchange.
+template <typename T, int N, int MODE>
 struct STile {
-  static constexpr size_t LDS_BYTES = sizeof(cpx<T>) * (N + SGeo<T, N>::TL * N);
+  static constexpr bool SPLIT = SGeo<T, N, MODE>::SPLIT && MODE < 2;
+  static constexpr size_t LDS_BYTES =
+      sizeof(cpx<T>) * N + (SPLIT ? sizeof(T) : sizeof(cpx<T>)) * SGeo<T, N, MODE>::TL * N;
 };
 
 // Strided sweep of brick `tile` (TL adjacent lines).  MODE 0: forward,
-// 1: inverse, 2: forward * prefac * inverse (x axis only).
+// 1: inverse, 2: forward * prefac * inverse (x axis only) with the float32
+// multiplier, 3: the same with the exact-symbol float64 multiplier.
 template <typename T, int N, int MODE>
 __device__ __forceinline__ void strided_tile(cpx<T>* __restrict__ spec, int64_t nlines, const LineMap& lm,
                                              const cpx<T>* __restrict__ twg, const Prefac& pf, int64_t tile,
                                              int tid, unsigned char* smem) {
-  using G = SGeo<T, N>;
+  using G = SGeo<T, N, MODE>;
   constexpr int TPL = G::TPL, TL = G::TL, E = N / TPL;
   cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* lds = tw + N;
@@ -255,15 +279,19 @@ __device__ __forceinline__ void strided_tile(cpx<T>* __restrict__ spec, int64_t 
   for (int q = 0; q < E; ++q)
     v[q] = active ? spec[start + (int64_t)(tl + TPL * q) * lm.es] : cpx<T>{T(0), T(0)};
   __syncthreads();
-  const BrickLay<T, TL> lay{lds + t};
+  constexpr bool SPLIT = STile<T, N, MODE>::SPLIT;
+  using Lay = typename std::conditional<SPLIT, BrickLaySplit<T, TL>, BrickLay<T, TL>>::type;
+  Lay lay;
+  if constexpr (SPLIT) lay.base = reinterpret_cast<T*>(lds) + t;
+  else lay.base = lds + t;
   fft_regs<T, N, TPL, MODE == 1>(v, lay, tl, tw, typename FftSpec<N>::R{});
-  if constexpr (MODE == 2) {
+  if constexpr (MODE >= 2) {
     const int64_t outer = L / lm.nh;
     const int jz = (int)(L - outer * lm.nh);
     const int jy = (int)(outer % pf.ny);
     const T sc = static_cast<T>(pf.scale);
     const int64_t ch = active ? outer / pf.ny : 0;
-    if (pf.exact) {
+    if constexpr (MODE == 3) {
 #pragma unroll
       for (int q = 0; q < E; ++q)
         v[q] = cscale(v[q], static_cast<T>(prefac_f64(tl + TPL * q, jy, jz, pf, ch) * pf.scale));

@@ -211,23 +211,24 @@ __host__ inline int pick_xchunk(const Dims& d, int64_t tiles_yz) {
 // Cahn-Hilliard (problem_definition.py:328-371):
 //   c = clip(u,0,1); mu = ((18/eps*c)*(1-c))*(1-2c) - (2eps)*lap(cB)
 //   r = D * sum_axes grad_face( (cf*(1-cf)) * grad_face(muB) )
-// mu lives in a 4-plane LDS ring with a one-voxel (y,z) halo; c in a 6-plane
-// ring with a two-voxel halo.  mu ghosts (non-periodic faces) are the BC rule
-// applied to mu at the adjacent interior voxel, as pad_bc(mu) does.
+// mu lives in an LDS plane ring with a one-voxel (y,z) halo, c in one with a
+// two-voxel halo.  mu ghosts (non-periodic faces) are the BC rule applied to
+// mu at the adjacent interior voxel, as pad_bc(mu) does.
 // ---------------------------------------------------------------------------
 // Thread (ty, tz) owns PY consecutive rows r0 = ty*PY .. r0+PY-1 of column tz.
 // Per output plane it evaluates mu for its own rows (sharing the column loads
 // between rows) while threads 0..H-1 evaluate the one-voxel mu halo with the
 // full ghost logic; the flux through the face between two own rows is
 // computed once, and the flux through the x face x+1/2 is carried in a
-// register to the next plane.  The c ring holds 5 planes (x-2..x+2), the mu
-// ring 3 (x-1..x+1); the plane x+3 is fetched into registers during the
-// arithmetic of plane x.
+// register to the next plane.  The c ring holds 4 planes (x-1..x+2, the
+// slot of x-1 being refilled with x+3 during the output of plane x), the mu
+// ring 2 (x, x+1); plane x+4 is fetched into registers during the output of
+// plane x.
 template <typename T>
 struct ChTile {
   static constexpr int PY = TileCfg<T>::PY, BY = NTY * PY;
-  static constexpr int CW = BZ + 4, CR = BY + 4, NC = 5;
-  static constexpr int MW = BZ + 2, MR = BY + 2, NM = 3;
+  static constexpr int CW = BZ + 4, CR = BY + 4, NC = 4;
+  static constexpr int MW = BZ + 2, MR = BY + 2, NM = 2;
   static constexpr size_t LDS_BYTES = sizeof(T) * (NC * CR * CW + NM * MR * MW);
 };
 
@@ -361,29 +362,42 @@ __device__ __forceinline__ void ch_rhs_tile(const T* __restrict__ u, T* __restri
 
   PlaneLoader<T, 2, CR, CW, true> pl;
   pl.init(y0, z0, d, bc, tid);
-  for (int64_t xg = x0 - 2; xg <= x0 + 2; ++xg) {
+  const int b = tz + 2, bm = tz + 1;
+  // Warm-up: planes x0-2 .. x0+1, mu(x0-1) and mu(x0), and the flux through
+  // the x face x0-1/2 (afterwards carried in registers, so neither c(x-1)
+  // nor mu(x-1) is read again and the rings hold 4 c planes and 2 mu planes).
+  for (int64_t xg = x0 - 2; xg <= x0 + 1; ++xg) {
     pl.fetch(uc, xg, d, bc);
     pl.commit(&cs[ring<NC>(xg - x0 + 2)][0][0], bc, tid);
   }
-  if (x0 + 3 <= x1 + 1) pl.fetch(uc, x0 + 3, d, bc);
+  if (x0 + 2 <= x1 + 1) pl.fetch(uc, x0 + 2, d, bc);
   __syncthreads();
   compute_mu(x0 - 1);
   compute_mu(x0);
-  __syncthreads();  // the first commit below reuses the slot mu(x0-1) read
-  const int64_t plane = d.ny * d.nz;
+  __syncthreads();
   T carry[PY];  // flux through the x face x-1/2 of each own row
+  {
+    const T(*Cm)[CW] = cs[ring<NC>(-1 + 2)];
+    const T(*C0)[CW] = cs[ring<NC>(0 + 2)];
+    const T(*Mm)[MW] = ms[ring<NM>(-1 + 1)];
+    const T(*M0)[MW] = ms[ring<NM>(0 + 1)];
+#pragma unroll
+    for (int q = 0; q < PY; ++q)
+      carry[q] = Op::flux(Cm[r0 + q + 2][b], C0[r0 + q + 2][b], Mm[r0 + q + 1][bm], M0[r0 + q + 1][bm], g.idx);
+  }
+  // plane x0+2 goes to the slot of x0-2 (last read by mu(x0-1))
+  if (x0 + 2 <= x1 + 1) pl.commit(&cs[ring<NC>(4)][0][0], bc, tid);
+  if (x0 + 3 <= x1 + 1) pl.fetch(uc, x0 + 3, d, bc);
+  __syncthreads();  // mu(x0+1) below reuses the slot of mu(x0-1) read above
+  const int64_t plane = d.ny * d.nz;
   for (int64_t x = x0; x < x1; ++x) {
-    // plane x+3 (fetched one iteration ago) goes to the slot of plane x-2;
-    // it feeds mu(x+2), needed while x+1 < x1
-    if (x + 3 <= x1 + 1) pl.commit(&cs[ring<NC>(x + 3 - x0 + 2)][0][0], bc, tid);
-    if (x + 4 <= x1 + 1) pl.fetch(uc, x + 4, d, bc);
+    // mu(x+1) from c(x), c(x+1), c(x+2), into the slot of mu(x-1)
     compute_mu(x + 1);
     __syncthreads();
     const T(*C0)[CW] = cs[ring<NC>(x - x0 + 2)];
     const T(*Cp)[CW] = cs[ring<NC>(x + 1 - x0 + 2)];
     const T(*M0)[MW] = ms[ring<NM>(x - x0 + 1)];
     const T(*Mp)[MW] = ms[ring<NM>(x + 1 - x0 + 1)];
-    const int b = tz + 2, bm = tz + 1;
     T cc[PY + 2], mc[PY + 2], fy[PY + 1];
 #pragma unroll
     for (int j = 0; j < PY + 2; ++j) {
@@ -392,12 +406,6 @@ __device__ __forceinline__ void ch_rhs_tile(const T* __restrict__ u, T* __restri
     }
 #pragma unroll
     for (int j = 0; j <= PY; ++j) fy[j] = Op::flux(cc[j], cc[j + 1], mc[j], mc[j + 1], g.idy);
-    if (x == x0) {
-      const T(*Cm)[CW] = cs[ring<NC>(x - 1 - x0 + 2)];
-      const T(*Mm)[MW] = ms[ring<NM>(x - 1 - x0 + 1)];
-#pragma unroll
-      for (int q = 0; q < PY; ++q) carry[q] = Op::flux(Cm[r0 + q + 2][b], cc[q + 1], Mm[r0 + q + 1][bm], mc[q + 1], g.idx);
-    }
 #pragma unroll
     for (int q = 0; q < PY; ++q) {
       const int a = r0 + q + 2, am = r0 + q + 1;
@@ -410,6 +418,10 @@ __device__ __forceinline__ void ch_rhs_tile(const T* __restrict__ u, T* __restri
       const int64_t gy = y0 + r0 + q;
       if (gy < d.ny && gz < d.nz) epilogue(ep, uc, oc, x * plane + gy * d.nz + gz, r);
     }
+    // plane x+3 (fetched one iteration ago) goes to the slot of c(x-1), which
+    // neither the output above nor mu(x+1) reads; it feeds mu(x+2)
+    if (x + 3 <= x1 + 1) pl.commit(&cs[ring<NC>(x + 3 - x0 + 2)][0][0], bc, tid);
+    if (x + 4 <= x1 + 1) pl.fetch(uc, x + 4, d, bc);
     __syncthreads();
   }
 }
