@@ -41,9 +41,9 @@ def log(msg):
 
 
 def stage_bytes(n, s=4):
-    """(kernel, algorithmic HBM bytes per voxel) for the trace intervals of a
-    step: [0,1] RHS, [1,2] z R2C, [2,3] y forward, [3,4] x forward*prefac*inverse,
-    [4,5] y inverse, [5,6] z C2R + add."""
+    """(kernel, algorithmic bytes per voxel of its sweep) by the plan's trace
+    kinds: 0 RHS, 1 z R2C, 2 y forward, 3 x forward*prefac*inverse,
+    4 y inverse, 5 z C2R + add."""
     nh = n // 2 + 1
     spec = 2 * s * nh / n  # one half-spectrum array, per voxel
     return [
@@ -336,6 +336,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--slabs", type=int, default=0,
+                    help="x-slabs of the semi-implicit step (0: whole-grid sweeps)")
     ap.add_argument("--no-inversion", action="store_true", help="skip the forward+adjoint inversion leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-grid leg")
     ap.add_argument("--batch-steps", type=int, default=200)
@@ -376,6 +378,7 @@ def main():
     u = vg.init_scalar_field(c0).contiguous()
     plan, _ = stepper.workspace(u)
     lib = _lib.load()
+    _lib.check(lib.evx_fft_plan_set_option(plan.handle, _lib.OPT_SLABS, args.slabs), "slabs")
 
     mass0 = float(vg.channel_sums(u)[0])
     log(f"[bench] warmup {W} steps")
@@ -384,12 +387,13 @@ def main():
     torch.cuda.synchronize()
 
     stream = torch.cuda.current_stream()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(K)]
+    NEV = 64  # events per step: one before the first kernel, one after each kernel
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(NEV)] for _ in range(K)]
     for row in ev:  # materialise the underlying hipEvent_t handles
         for e in row:
             e.record(stream)
     torch.cuda.synchronize()
-    handles = [(ctypes.c_void_p * 7)(*[e.cuda_event for e in row]) for row in ev]
+    handles = [(ctypes.c_void_p * NEV)(*[e.cuda_event for e in row]) for row in ev]
     log(f"[bench] timing {K} steps")
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
@@ -397,7 +401,7 @@ def main():
     w0 = time.perf_counter()
     t_start.record(stream)
     for k in range(K):
-        _lib.check(lib.evx_fft_plan_trace_next_step(plan.handle, handles[k], 7), "trace")
+        _lib.check(lib.evx_fft_plan_trace_next_step(plan.handle, handles[k], NEV), "trace")
         stepper.step_into(u, u)
     t_end.record(stream)
     torch.cuda.synchronize()
@@ -405,21 +409,31 @@ def main():
     ms_step = t_start.elapsed_time(t_end) / K
     mass1 = float(vg.channel_sums(u)[0])
     log(f"[bench] {ms_step:.3f} ms/step (events), wall {1e3 * wall / K:.3f} ms/step")
+    nint = ctypes.c_int32()
+    _lib.check(lib.evx_fft_plan_trace_kinds(plan.handle, None, ctypes.byref(nint)), "trace kinds")
+    kinds = (ctypes.c_int32 * max(1, nint.value))()
+    _lib.check(lib.evx_fft_plan_trace_kinds(plan.handle, kinds, ctypes.byref(nint)), "trace kinds")
+    kinds = list(kinds)[:nint.value]
 
     V = n ** 3
     stages = stage_bytes(n)
-    times = np.array([[ev[k][i].elapsed_time(ev[k][i + 1]) for i in range(6)] for k in range(K)])
-    mean_ms = times.mean(axis=0)
+    times = np.array([[ev[k][i].elapsed_time(ev[k][i + 1]) for i in range(len(kinds))] for k in range(K)])
     kernels = []
-    for (name, bpv), ms in zip(stages, mean_ms):
-        kernels.append({"kernel": name, "ms": round(float(ms), 4), "alg_bytes_per_voxel": round(bpv, 4),
-                        "alg_bytes_per_launch": int(round(bpv * V)),
+    for kind, (name, bpv) in enumerate(stages):
+        cols = [i for i, kk in enumerate(kinds) if kk == kind]
+        ms = float(times[:, cols].sum(axis=1).mean())
+        nl = len(cols)
+        kernels.append({"kernel": name, "ms": round(ms, 4), "launches_per_step": nl,
+                        "launch_ms": round(ms / nl, 4), "alg_bytes_per_voxel": round(bpv, 4),
+                        "alg_bytes_per_launch": int(round(bpv * V / nl)),
                         "gbps": round(bpv * V / (ms * 1e-3) / 1e9, 1)})
-    dom = int(np.argmax([k["ms"] for k in kernels]))
+    mean_ms = np.array([k["ms"] for k in kernels])
+    dom = int(np.argmax(mean_ms))
     dk = kernels[dom]
     roof = {"bound": "hbm", "kernel": dk["kernel"], "achieved": dk["gbps"], "peak": PEAK_GBPS, "unit": "GB/s",
             "frac": round(dk["gbps"] / PEAK_GBPS, 4), "traffic": traffic_from_profiles(dk["kernel"]),
-            "alg_bytes_per_launch": dk["alg_bytes_per_launch"], "launch_ms": dk["ms"]}
+            "alg_bytes_per_launch": dk["alg_bytes_per_launch"], "launch_ms": dk["launch_ms"],
+            "launches_per_step": dk["launches_per_step"]}
     b_alg = 4 * (3 + 8 * (1 + 2 / n))
     step_gbps = b_alg * V / (ms_step * 1e-3) / 1e9
     out = {
@@ -436,6 +450,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"cahn_hilliard_semi_implicit_{n}^3_fp32", "grid": [n, n, n], "domain": [n, n, n],
+                   "slabs": args.slabs,
                    "dt": 0.05, "eps": 3.0, "D": 1.0, "A": 0.25, "bc": "periodic",
                    "init": "0.5+0.05*N(0,1), numpy default_rng(2)", "parallelism": "single GPU"},
         "hbm_gbps": round(step_gbps, 1),

@@ -31,6 +31,7 @@ OUT_RHS, OUT_EULER, OUT_RK4_FIRST, OUT_RK4_MID, OUT_RK4_LAST = range(5)
 
 OPT_SPECIALIZED = 1
 OPT_EXACT_SYMBOL = 2
+OPT_SLABS = 3
 
 
 class EvxBC(C.Structure):
@@ -91,6 +92,7 @@ SIGNATURES = {
     "evx_batch_autotune": [_P, _P, _P, C.c_int, _P, C.POINTER(_D)],
     "evx_batch_ch_imex_step": [_P, _P, _P, _P],
     "evx_fft_plan_trace_next_step": [_P, C.POINTER(_P), C.c_int],
+    "evx_fft_plan_trace_kinds": [_P, C.POINTER(_I32), C.POINTER(_I32)],
     "evx_fft_plan_set_option": [_P, C.c_int, C.c_int],
     "evx_reduce_workspace_bytes": [_I64, C.POINTER(C.c_size_t)],
     "evx_sum_f64": [_I32, _P, _I64, _I64, _P, _P, _P, _P],

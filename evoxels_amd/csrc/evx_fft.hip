@@ -28,7 +28,9 @@
 namespace evx {
 
 int stencil_rhs_ch_periodic(int32_t dtype, const void* u, void* out, const Dims& d, const double div_dx[3],
-                            const double div_dx2[3], double eps, double D, const double* chan, hipStream_t s);
+                            const double div_dx2[3], double eps, double D, const double* chan, hipStream_t s,
+                            int64_t xa = 0, int64_t xb = -1);
+int stencil_ch_xchunk(int32_t dtype, const Dims& d);
 size_t ch_rhs_vjp_ws_bytes(int32_t dtype, const Dims& d);
 int ch_rhs_vjp_launch(int32_t dtype, const void* u, const void* w, const void* base, void* grad,
                       double* dparams, const Dims& d, const double dd[3], const double dd2[3], double eps,
@@ -410,10 +412,14 @@ struct evx_fft_plan {
   int64_t pitch;  // row pitch (complex values) of the workspace spectrum, >= nh
   evx::AxisHost ax[3];  // x, y, z (z: complex length nz)
   // One-shot stage trace: events recorded around the kernels of the next step.
-  hipEvent_t trace[8] = {};
-  int ntrace = 0;
+  // One-shot launch trace of the next step: event 0 before the first kernel,
+  // event i after the i-th kernel, whose kind is kinds[i-1].
+  hipEvent_t trace[64] = {};
+  int ntrace = 0, tpos = 0, nkinds = 0;
+  int kinds[64] = {};
   bool fast = true;  // use the size-specialised kernels where one exists
   bool exact_symbol = false;  // float64 multiplier from exact wavenumbers
+  int slabs = 0;               // > 1: semi-implicit CH steps run x-slab by x-slab (one-channel plans)
 };
 
 namespace evx {
@@ -489,9 +495,17 @@ Geom strided_geom(const AxisHost& h, int64_t nlines) {
 }
 
 template <typename T>
-int run_z_r2c(evx_fft_plan* p, const void* x, void* spec, int64_t pitch, hipStream_t s) {
+// xa/xb (one-channel plans only) restrict a z or y sweep to the x-slab
+// [xa, xb); the pointers are offset to the slab's first line.
+int run_z_r2c(evx_fft_plan* p, const void* x, void* spec, int64_t pitch, hipStream_t s, int64_t xa = 0,
+              int64_t xb = -1) {
   const AxisHost& h = p->ax[2];
-  const int64_t nlines = p->d.c * p->d.nx * p->d.ny;
+  int64_t nlines = p->d.c * p->d.nx * p->d.ny;
+  if (xb >= 0) {
+    x = static_cast<const T*>(x) + xa * p->d.ny * p->d.nz;
+    spec = static_cast<cpx<T>*>(spec) + xa * p->d.ny * pitch;
+    nlines = (xb - xa) * p->d.ny;
+  }
   if (p->fast && fast_z_r2c<T>(h.n, x, spec, nlines, pitch, h.tw, s)) return check_launch("z_r2c_fast");
   const Geom g = z_geom<T>(h, (nlines + 1) / 2);
   const int64_t blocks = (nlines + 2 * g.lpw - 1) / (2 * g.lpw);
@@ -502,9 +516,16 @@ int run_z_r2c(evx_fft_plan* p, const void* x, void* spec, int64_t pitch, hipStre
 
 template <typename T>
 int run_z_c2r(evx_fft_plan* p, const void* spec, void* out, const void* base, double scale, int64_t pitch,
-              hipStream_t s) {
+              hipStream_t s, int64_t xa = 0, int64_t xb = -1) {
   const AxisHost& h = p->ax[2];
-  const int64_t nlines = p->d.c * p->d.nx * p->d.ny;
+  int64_t nlines = p->d.c * p->d.nx * p->d.ny;
+  if (xb >= 0) {
+    const int64_t off = xa * p->d.ny * p->d.nz;
+    out = static_cast<T*>(out) + off;
+    if (base) base = static_cast<const T*>(base) + off;
+    spec = static_cast<const cpx<T>*>(spec) + xa * p->d.ny * pitch;
+    nlines = (xb - xa) * p->d.ny;
+  }
   if (p->fast && fast_z_c2r<T>(h.n, spec, out, base, scale, nlines, pitch, h.tw, s))
     return check_launch("z_c2r_fast");
   const Geom g = z_geom<T>(h, (nlines + 1) / 2);
@@ -516,7 +537,8 @@ int run_z_c2r(evx_fft_plan* p, const void* spec, void* out, const void* base, do
 }
 
 template <typename T, int MODE>
-int run_strided(evx_fft_plan* p, int axis, void* spec, const Prefac& pf, int64_t pitch, hipStream_t s) {
+int run_strided(evx_fft_plan* p, int axis, void* spec, const Prefac& pf, int64_t pitch, hipStream_t s,
+                int64_t xa = 0, int64_t xb = -1) {
   const AxisHost& h = p->ax[axis];
   const Dims& d = p->d;
   LineMap lm;
@@ -524,6 +546,10 @@ int run_strided(evx_fft_plan* p, int axis, void* spec, const Prefac& pf, int64_t
   int64_t nlines;
   if (axis == 1) {
     nlines = d.c * d.nx * p->nh;
+    if (xb >= 0) {  // y lines of the x-slab [xa, xb)
+      spec = static_cast<cpx<T>*>(spec) + xa * d.ny * pitch;
+      nlines = (xb - xa) * p->nh;
+    }
     lm.G = 1;
     lm.GM = d.ny * pitch;
     lm.OM = 0;
@@ -575,12 +601,32 @@ size_t spec_bytes(const evx_fft_plan* p) {
   return align_up((size_t)p->d.c * p->d.nx * p->d.ny * p->pitch * (p->dtype == EVX_F32 ? 8 : 16));
 }
 
-// Records trace event k (if the caller armed one) on the step's stream.
-int mark(evx_fft_plan* p, int k, hipStream_t s) {
-  if (k < p->ntrace && p->trace[k]) {
-    if (hipEventRecord(p->trace[k], s) != hipSuccess) return fail(EVX_ERR_HIP, "hipEventRecord failed");
+// Launch trace (armed by evx_fft_plan_trace_next_step).  Kernel kinds:
+enum { TK_RHS = 0, TK_Z_R2C = 1, TK_Y_FWD = 2, TK_X = 3, TK_Y_INV = 4, TK_Z_C2R = 5, TK_OTHER = 6 };
+
+int trace_start(evx_fft_plan* p, hipStream_t s) {
+  p->tpos = 0;
+  p->nkinds = 0;
+  if (p->ntrace > 0 && p->trace[0]) {
+    if (hipEventRecord(p->trace[0], s) != hipSuccess) return fail(EVX_ERR_HIP, "hipEventRecord failed");
+    p->tpos = 1;
   }
   return EVX_OK;
+}
+
+int trace_tick(evx_fft_plan* p, int kind, hipStream_t s) {
+  if (p->tpos > 0 && p->tpos < p->ntrace && p->trace[p->tpos]) {
+    if (hipEventRecord(p->trace[p->tpos], s) != hipSuccess) return fail(EVX_ERR_HIP, "hipEventRecord failed");
+    p->kinds[p->tpos - 1] = kind;
+    ++p->tpos;
+  }
+  return EVX_OK;
+}
+
+void trace_end(evx_fft_plan* p) {
+  p->nkinds = p->tpos > 0 ? p->tpos - 1 : 0;
+  p->ntrace = 0;
+  p->tpos = 0;
 }
 
 template <typename T>
@@ -588,16 +634,54 @@ int spectral_update(evx_fft_plan* p, const void* rhs, void* spec, const void* u,
                     hipStream_t s) {
   const int64_t pt = p->pitch;
   EVX_TRY(run_z_r2c<T>(p, rhs, spec, pt, s));
-  EVX_TRY(mark(p, 2, s));
+  EVX_TRY(trace_tick(p, TK_Z_R2C, s));
   EVX_TRY((run_strided<T, 0>(p, 1, spec, pf, pt, s)));
-  EVX_TRY(mark(p, 3, s));
+  EVX_TRY(trace_tick(p, TK_Y_FWD, s));
   EVX_TRY((pf.exact ? run_strided<T, 3>(p, 0, spec, pf, pt, s) : run_strided<T, 2>(p, 0, spec, pf, pt, s)));
-  EVX_TRY(mark(p, 4, s));
+  EVX_TRY(trace_tick(p, TK_X, s));
   EVX_TRY((run_strided<T, 1>(p, 1, spec, pf, pt, s)));
-  EVX_TRY(mark(p, 5, s));
+  EVX_TRY(trace_tick(p, TK_Y_INV, s));
   EVX_TRY(run_z_c2r<T>(p, spec, u_out, u, 1.0, pt, s));
-  EVX_TRY(mark(p, 6, s));
-  p->ntrace = 0;
+  EVX_TRY(trace_tick(p, TK_Z_C2R, s));
+  trace_end(p);
+  return EVX_OK;
+}
+
+// Semi-implicit CH step x-slab by x-slab: for each slab, the right-hand side,
+// the z R2C and the forward y sweep; then the x sweep over the whole
+// spectrum; then, for each slab, the inverse y sweep and the z C2R + add.
+// The z and y sweeps couple only lines inside one x-slab, so a slab's
+// staging and spectrum (a few tens of MB) are re-read from the last-level
+// cache instead of HBM; only the x sweep and the field itself stream from
+// HBM.
+template <typename T>
+int ch_slab_step(evx_fft_plan* p, const void* u, void* u_out, void* rhs, void* spec, const Prefac& pf,
+                 const double dd[3], const double dd2[3], const evx_imex_params* prm, hipStream_t s) {
+  const Dims& d = p->d;
+  const int64_t pt = p->pitch;
+  const int64_t xc = stencil_ch_xchunk(p->dtype, d);
+  int64_t w = (d.nx + p->slabs - 1) / p->slabs;
+  w = (w + xc - 1) / xc * xc;  // slab width: a whole number of CH tile chunks
+  EVX_TRY(trace_start(p, s));
+  for (int64_t xa = 0; xa < d.nx; xa += w) {
+    const int64_t xb = xa + w < d.nx ? xa + w : d.nx;
+    EVX_TRY(stencil_rhs_ch_periodic(p->dtype, u, rhs, d, dd, dd2, prm->eps, prm->D, prm->chan_params, s, xa, xb));
+    EVX_TRY(trace_tick(p, TK_RHS, s));
+    EVX_TRY(run_z_r2c<T>(p, rhs, spec, pt, s, xa, xb));
+    EVX_TRY(trace_tick(p, TK_Z_R2C, s));
+    EVX_TRY((run_strided<T, 0>(p, 1, spec, pf, pt, s, xa, xb)));
+    EVX_TRY(trace_tick(p, TK_Y_FWD, s));
+  }
+  EVX_TRY((pf.exact ? run_strided<T, 3>(p, 0, spec, pf, pt, s) : run_strided<T, 2>(p, 0, spec, pf, pt, s)));
+  EVX_TRY(trace_tick(p, TK_X, s));
+  for (int64_t xa = 0; xa < d.nx; xa += w) {
+    const int64_t xb = xa + w < d.nx ? xa + w : d.nx;
+    EVX_TRY((run_strided<T, 1>(p, 1, spec, pf, pt, s, xa, xb)));
+    EVX_TRY(trace_tick(p, TK_Y_INV, s));
+    EVX_TRY(run_z_c2r<T>(p, spec, u_out, u, 1.0, pt, s, xa, xb));
+    EVX_TRY(trace_tick(p, TK_Z_C2R, s));
+  }
+  trace_end(p);
   return EVX_OK;
 }
 
@@ -750,9 +834,12 @@ extern "C" int evx_ch_imex_step(evx_fft_plan* p, const void* u, void* u_out, voi
   Prefac pf = make_prefac(p, 1, ((-2.0 * prm->eps) * prm->D) * prm->A, prm->dt, prm->spacing);
   pf.chan = prm->chan_params;
   pf.A = prm->A;
-  EVX_TRY(mark(p, 0, s));
+  if (p->slabs > 1 && p->d.c == 1)
+    return p->dtype == EVX_F32 ? ch_slab_step<float>(p, u, u_out, rhs, spec, pf, dd, dd2, prm, s)
+                               : ch_slab_step<double>(p, u, u_out, rhs, spec, pf, dd, dd2, prm, s);
+  EVX_TRY(trace_start(p, s));
   EVX_TRY(stencil_rhs_ch_periodic(p->dtype, u, rhs, p->d, dd, dd2, prm->eps, prm->D, prm->chan_params, s));
-  EVX_TRY(mark(p, 1, s));
+  EVX_TRY(trace_tick(p, TK_RHS, s));
   return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
                              : spectral_update<double>(p, rhs, spec, u, u_out, pf, s);
 }
@@ -766,9 +853,9 @@ extern "C" int evx_rd_imex_step(evx_fft_plan* p, const void* u, void* u_out, voi
   geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
   void* rhs = ws;
   void* spec = static_cast<char*>(ws) + real_bytes(p);
-  EVX_TRY(mark(p, 0, s));
+  EVX_TRY(trace_start(p, s));
   EVX_TRY(stencil_rhs_rd_periodic(p->dtype, u, rhs, p->d, dd2, prm->D, s));
-  EVX_TRY(mark(p, 1, s));
+  EVX_TRY(trace_tick(p, TK_RHS, s));
   // problem_definition.py:206 -> (-D)*A as a Python double
   const Prefac pf = make_prefac(p, 0, (-prm->D) * prm->A, prm->dt, prm->spacing);
   return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
@@ -802,6 +889,7 @@ extern "C" int evx_ch_imex_step_vjp(evx_fft_plan* p, const void* u0, const void*
   pf.chan = prm->chan_params;
   pf.A = prm->A;
   p->ntrace = 0;
+  p->tpos = 0;
   EVX_TRY(p->dtype == EVX_F32 ? spectral_update<float>(p, lam1, spec, nullptr, w, pf, s)
                               : spectral_update<double>(p, lam1, spec, nullptr, w, pf, s));
   return ch_rhs_vjp_launch(p->dtype, u0, w, lam1, lam0, dparams, p->d, dd, dd2, prm->eps, prm->D,
@@ -810,9 +898,18 @@ extern "C" int evx_ch_imex_step_vjp(evx_fft_plan* p, const void* u0, const void*
 
 extern "C" int evx_fft_plan_trace_next_step(evx_fft_plan* p, void* const* events, int n_events) {
   EVX_TRY(check_plan(p));
-  if (n_events < 0 || n_events > 8 || (n_events > 0 && !events)) return fail(EVX_ERR_INVALID, "bad trace events");
+  if (n_events < 0 || n_events > 64 || (n_events > 0 && !events)) return fail(EVX_ERR_INVALID, "bad trace events");
   for (int i = 0; i < n_events; ++i) p->trace[i] = static_cast<hipEvent_t>(events[i]);
   p->ntrace = n_events;
+  return EVX_OK;
+}
+
+extern "C" int evx_fft_plan_trace_kinds(const evx_fft_plan* p, int32_t* kinds, int32_t* n) {
+  EVX_TRY(check_plan(p));
+  if (!n) return fail(EVX_ERR_INVALID, "null argument");
+  if (kinds)
+    for (int i = 0; i < p->nkinds; ++i) kinds[i] = p->kinds[i];
+  *n = p->nkinds;
   return EVX_OK;
 }
 
@@ -821,6 +918,10 @@ extern "C" int evx_fft_plan_set_option(evx_fft_plan* p, int option, int value) {
   switch (option) {
     case EVX_OPT_SPECIALIZED: p->fast = value != 0; return EVX_OK;
     case EVX_OPT_EXACT_SYMBOL: p->exact_symbol = value != 0; return EVX_OK;
+    case EVX_OPT_SLABS:
+      if (value < 0 || value > 4096) return fail(EVX_ERR_INVALID, "slab count must be in [0, 4096]");
+      p->slabs = value;
+      return EVX_OK;
     default: return fail(EVX_ERR_INVALID, "unknown plan option %d", option);
   }
 }

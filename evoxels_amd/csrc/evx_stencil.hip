@@ -67,9 +67,9 @@ __global__ __launch_bounds__(NT) void rd_rhs_kernel(const T* __restrict__ u, T* 
 template <typename T, bool STRICT>
 __global__ __launch_bounds__(NT) void ch_rhs_kernel(const T* __restrict__ u, T* __restrict__ out, Dims d, Bc bc,
                                                     Geo<T> g, ChCoef<T> k, Epi ep, TileGrid tg,
-                                                    const double* __restrict__ chan) {
+                                                    const double* __restrict__ chan, unsigned tile_base) {
   __shared__ __align__(16) unsigned char smem[ChTile<T>::LDS_BYTES];
-  ch_rhs_tile<T, STRICT>(u, out, d, bc, g, k, ep, tg, chan, xcd_tile(blockIdx.x, gridDim.x),
+  ch_rhs_tile<T, STRICT>(u, out, d, bc, g, k, ep, tg, chan, tile_base + xcd_tile(blockIdx.x, gridDim.x),
                          threadIdx.y * BZ + threadIdx.x, smem);
 }
 
@@ -266,14 +266,26 @@ int launch_rd(const void* u, void* out, const Dims& d, const Bc& bc, const evx_g
   return check_launch("rd_rhs_kernel");
 }
 
+constexpr int SLAB_XCHUNK = 16;
+
 template <typename T, bool STRICT = true>
 int launch_ch(const void* u, void* out, const Dims& d, const Bc& bc, const evx_grid* g, double eps,
-              double D, const Epi& ep, hipStream_t s, const double* chan = nullptr) {
+              double D, const Epi& ep, hipStream_t s, const double* chan = nullptr, int64_t xa = 0,
+              int64_t xb = -1) {
   TileGrid tg;
   dim3 grid = stencil_grid<T>(d, &tg);
+  unsigned base = 0;
+  if (xb >= 0) {  // x-slab [xa, xb) of a one-channel field, at tile-chunk granularity
+    tg.xc = SLAB_XCHUNK;
+    tg.nxc = (int)((d.nx + tg.xc - 1) / tg.xc);
+    const int64_t per_chunk = (int64_t)tg.ntz * tg.nty;
+    const int64_t ca = xa / tg.xc, cb = (xb + tg.xc - 1) / tg.xc;
+    base = (unsigned)(ca * per_chunk);
+    grid = dim3((unsigned)((cb - ca) * per_chunk));
+  }
   ChCoef<T> k{static_cast<T>(18.0 / eps), static_cast<T>(2.0 * eps), static_cast<T>(D)};
   hipLaunchKernelGGL((ch_rhs_kernel<T, STRICT>), grid, dim3(BZ, NTY), 0, s, static_cast<const T*>(u),
-                     static_cast<T*>(out), d, bc, make_geo<T>(g), k, ep, tg, chan);
+                     static_cast<T*>(out), d, bc, make_geo<T>(g), k, ep, tg, chan, base);
   return check_launch("ch_rhs_kernel");
 }
 
@@ -303,13 +315,18 @@ namespace evx {
 // Fully periodic right-hand sides written to a plain buffer; used by the
 // semi-implicit spectral steps (timesteppers.py:85-86).
 int stencil_rhs_ch_periodic(int32_t dtype, const void* u, void* out, const Dims& d, const double div_dx[3],
-                            const double div_dx2[3], double eps, double D, const double* chan, hipStream_t s) {
+                            const double div_dx2[3], double eps, double D, const double* chan, hipStream_t s,
+                            int64_t xa, int64_t xb) {
   evx_grid g{d.c, d.nx, d.ny, d.nz, {div_dx[0], div_dx[1], div_dx[2]}, {div_dx2[0], div_dx2[1], div_dx2[2]}};
   Bc b{};
   const Epi ep{EVX_OUT_RHS, 0.0, nullptr, nullptr, nullptr};
-  return dtype == EVX_F32 ? launch_ch<float, false>(u, out, d, b, &g, eps, D, ep, s, chan)
-                          : launch_ch<double, false>(u, out, d, b, &g, eps, D, ep, s, chan);
+  return dtype == EVX_F32 ? launch_ch<float, false>(u, out, d, b, &g, eps, D, ep, s, chan, xa, xb)
+                          : launch_ch<double, false>(u, out, d, b, &g, eps, D, ep, s, chan, xa, xb);
 }
+
+// x-planes per CH tile chunk of slab launches (slabs start on a multiple of
+// it); small enough that one slab still spreads over every CU.
+int stencil_ch_xchunk(int32_t, const Dims&) { return SLAB_XCHUNK; }
 
 int stencil_rhs_rd_periodic(int32_t dtype, const void* u, void* out, const Dims& d, const double div_dx2[3],
                             double D, hipStream_t s) {

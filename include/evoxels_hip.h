@@ -254,15 +254,18 @@ int evx_batch_autotune(evx_batch_plan* plan, void* scratch_state,
  * allocation-free, capturable in a graph. */
 int evx_batch_ch_imex_step(evx_batch_plan* plan, void* state, void* workspace,
                            void* stream);
-/* Stage timing for benchmarks (no reference counterpart; the reference times
- * whole runs with a wall clock, solvers.py:108-114).  Arms the next
+/* Launch timing for benchmarks (no reference counterpart; the reference
+ * times whole runs with a wall clock, solvers.py:108-114).  Arms the next
  * evx_ch_imex_step / evx_rd_imex_step on this plan to record the caller's
- * hipEvent_t handles events[0..n_events) around its kernels: [0] before the
- * RHS, [1] after it, [2] after the z R2C sweep, [3] after forward y, [4] after
- * the fused x sweep, [5] after inverse y, [6] after the z C2R sweep.  The
- * arming is consumed by that one step. */
+ * hipEvent_t handles: events[0] before its first kernel and events[i] after
+ * its i-th kernel, for as many kernels as there are events (at most 64).
+ * The arming is consumed by that one step.  evx_fft_plan_trace_kinds then
+ * gives the kernel of each recorded interval: 0 right-hand side, 1 z R2C,
+ * 2 forward y, 3 x forward*prefac*inverse, 4 inverse y, 5 z C2R + add. */
 int evx_fft_plan_trace_next_step(evx_fft_plan* plan, void* const* events,
                                  int n_events);
+int evx_fft_plan_trace_kinds(const evx_fft_plan* plan, int32_t* kinds,
+                             int32_t* n_intervals);
 
 /* Options of a plan (no reference counterpart).
  *   EVX_OPT_SPECIALIZED   1 (default): size-specialised FFT kernels where one
@@ -272,8 +275,14 @@ int evx_fft_plan_trace_next_step(evx_fft_plan* plan, void* const* events,
  *                         the reference's float32 k-table; 1: it is computed
  *                         in float64 from the exact wavenumbers, a smooth
  *                         function of eps and D for finite-difference checks
- *                         and float64 parameter estimation. */
-enum evx_plan_option { EVX_OPT_SPECIALIZED = 1, EVX_OPT_EXACT_SYMBOL = 2 };
+ *                         and float64 parameter estimation.
+ *   EVX_OPT_SLABS         0 (default): evx_ch_imex_step runs each sweep over
+ *                         the whole grid; K > 1 (one-channel plans): the
+ *                         right-hand side, z and y sweeps run x-slab by
+ *                         x-slab so a slab's intermediates stay in the
+ *                         last-level cache (trace events then mark the four
+ *                         phase boundaries). */
+enum evx_plan_option { EVX_OPT_SPECIALIZED = 1, EVX_OPT_EXACT_SYMBOL = 2, EVX_OPT_SLABS = 3 };
 int evx_fft_plan_set_option(evx_fft_plan* plan, int option, int value);
 
 /* ---- reductions ----------------------------------------------------------- */
