@@ -78,6 +78,8 @@ SIGNATURES = {
     "evx_irfftn": [_P, _P, _P, _P],
     "evx_ch_imex_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
     "evx_rd_imex_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
+    "evx_ch_etd1_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
+    "evx_rd_etd1_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
     "evx_ch_rhs_vjp_workspace_bytes": [_I32, C.POINTER(EvxGrid), C.POINTER(C.c_size_t)],
     "evx_ch_rhs_vjp": [_I32, _P, _P, _P, _P, _P, C.POINTER(EvxGrid), _D, _D, _P, _P, _P],
     "evx_ch_imex_vjp_workspace_bytes": [_P, C.POINTER(C.c_size_t)],

@@ -261,8 +261,9 @@ __global__ void strided_kernel(cpx<T>* spec, int64_t nlines, LineMap lm, Axis<T>
       const int jy = (int)(outer % pf.ny);
       const float ssc = symbol_scalar(pf, outer / pf.ny);
       for (int j = tl; j < n; j += jstep) {
-        const T f = pf.exact ? static_cast<T>(prefac_f64(j, jy, jz, pf, outer / pf.ny) * pf.scale)
-                             : static_cast<T>(prefac_f32(j, jy, jz, pf, ssc)) * sc;
+        const T f = pf.etd     ? static_cast<T>(etd1_f32(j, jy, jz, pf, ssc)) * sc
+                    : pf.exact ? static_cast<T>(prefac_f64(j, jy, jz, pf, outer / pf.ny) * pf.scale)
+                               : static_cast<T>(prefac_f32(j, jy, jz, pf, ssc)) * sc;
         buf[j * lpw + line] = cscale(buf[j * lpw + line], f);
       }
     }
@@ -585,6 +586,7 @@ Prefac make_prefac(const evx_fft_plan* p, int ch, double sym_scalar, double dt, 
   pf.chan = nullptr;
   pf.A = 0.0;
   pf.exact = p->exact_symbol ? 1 : 0;
+  pf.etd = 0;
   pf.scd = sym_scalar;
   pf.dtd = dt;
   for (int a = 0; a < 3; ++a) pf.inv_ndd[a] = 1.0 / ((double)n[a] * spacing[a]);
@@ -599,6 +601,14 @@ size_t real_bytes(const evx_fft_plan* p) {
 // Bytes of the workspace spectrum (rows of p->pitch complex values).
 size_t spec_bytes(const evx_fft_plan* p) {
   return align_up((size_t)p->d.c * p->d.nx * p->d.ny * p->pitch * (p->dtype == EVX_F32 ? 8 : 16));
+}
+
+// The x sweep: forward x transform, spectral multiplier, inverse x transform.
+template <typename T>
+int run_x_multiplier(evx_fft_plan* p, void* spec, const Prefac& pf, int64_t pt, hipStream_t s) {
+  if (pf.etd) return run_strided<T, 4>(p, 0, spec, pf, pt, s);
+  if (pf.exact) return run_strided<T, 3>(p, 0, spec, pf, pt, s);
+  return run_strided<T, 2>(p, 0, spec, pf, pt, s);
 }
 
 // Launch trace (armed by evx_fft_plan_trace_next_step).  Kernel kinds:
@@ -637,7 +647,7 @@ int spectral_update(evx_fft_plan* p, const void* rhs, void* spec, const void* u,
   EVX_TRY(trace_tick(p, TK_Z_R2C, s));
   EVX_TRY((run_strided<T, 0>(p, 1, spec, pf, pt, s)));
   EVX_TRY(trace_tick(p, TK_Y_FWD, s));
-  EVX_TRY((pf.exact ? run_strided<T, 3>(p, 0, spec, pf, pt, s) : run_strided<T, 2>(p, 0, spec, pf, pt, s)));
+  EVX_TRY(run_x_multiplier<T>(p, spec, pf, pt, s));
   EVX_TRY(trace_tick(p, TK_X, s));
   EVX_TRY((run_strided<T, 1>(p, 1, spec, pf, pt, s)));
   EVX_TRY(trace_tick(p, TK_Y_INV, s));
@@ -672,7 +682,7 @@ int ch_slab_step(evx_fft_plan* p, const void* u, void* u_out, void* rhs, void* s
     EVX_TRY((run_strided<T, 0>(p, 1, spec, pf, pt, s, xa, xb)));
     EVX_TRY(trace_tick(p, TK_Y_FWD, s));
   }
-  EVX_TRY((pf.exact ? run_strided<T, 3>(p, 0, spec, pf, pt, s) : run_strided<T, 2>(p, 0, spec, pf, pt, s)));
+  EVX_TRY(run_x_multiplier<T>(p, spec, pf, pt, s));
   EVX_TRY(trace_tick(p, TK_X, s));
   for (int64_t xa = 0; xa < d.nx; xa += w) {
     const int64_t xb = xa + w < d.nx ? xa + w : d.nx;
@@ -821,11 +831,14 @@ extern "C" int evx_irfftn(evx_fft_plan* p, void* spec, void* y, void* stream) {
   return run_z_c2r<double>(p, spec, y, nullptr, scale, p->nh, s);
 }
 
-extern "C" int evx_ch_imex_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
-                                void* stream) {
+namespace evx {
+namespace {
+
+// One semi-implicit (etd = 0) or exponential Euler (etd = 1) CH step.
+int ch_spectral_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
+                     hipStream_t s, int etd) {
   EVX_TRY(imex_common(p, u, u_out, ws, prm));
   if (!prm->chan_params && !(prm->eps != 0.0)) return fail(EVX_ERR_INVALID, "eps must be nonzero");
-  hipStream_t s = static_cast<hipStream_t>(stream);
   double dd[3], dd2[3];
   geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
   void* rhs = ws;
@@ -834,6 +847,8 @@ extern "C" int evx_ch_imex_step(evx_fft_plan* p, const void* u, void* u_out, voi
   Prefac pf = make_prefac(p, 1, ((-2.0 * prm->eps) * prm->D) * prm->A, prm->dt, prm->spacing);
   pf.chan = prm->chan_params;
   pf.A = prm->A;
+  pf.etd = etd;
+  if (etd) pf.exact = 0;
   if (p->slabs > 1 && p->d.c == 1)
     return p->dtype == EVX_F32 ? ch_slab_step<float>(p, u, u_out, rhs, spec, pf, dd, dd2, prm, s)
                                : ch_slab_step<double>(p, u, u_out, rhs, spec, pf, dd, dd2, prm, s);
@@ -844,11 +859,10 @@ extern "C" int evx_ch_imex_step(evx_fft_plan* p, const void* u, void* u_out, voi
                              : spectral_update<double>(p, rhs, spec, u, u_out, pf, s);
 }
 
-extern "C" int evx_rd_imex_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
-                                void* stream) {
+int rd_spectral_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
+                     hipStream_t s, int etd) {
   EVX_TRY(imex_common(p, u, u_out, ws, prm));
   if (prm->chan_params) return fail(EVX_ERR_UNSUPPORTED, "per-channel parameters are implemented for CH only");
-  hipStream_t s = static_cast<hipStream_t>(stream);
   double dd[3], dd2[3];
   geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
   void* rhs = ws;
@@ -857,9 +871,36 @@ extern "C" int evx_rd_imex_step(evx_fft_plan* p, const void* u, void* u_out, voi
   EVX_TRY(stencil_rhs_rd_periodic(p->dtype, u, rhs, p->d, dd2, prm->D, s));
   EVX_TRY(trace_tick(p, TK_RHS, s));
   // problem_definition.py:206 -> (-D)*A as a Python double
-  const Prefac pf = make_prefac(p, 0, (-prm->D) * prm->A, prm->dt, prm->spacing);
+  Prefac pf = make_prefac(p, 0, (-prm->D) * prm->A, prm->dt, prm->spacing);
+  pf.etd = etd;
+  if (etd) pf.exact = 0;
   return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
                              : spectral_update<double>(p, rhs, spec, u, u_out, pf, s);
+}
+
+}  // namespace
+}  // namespace evx
+
+using namespace evx;
+
+extern "C" int evx_ch_imex_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
+                                void* stream) {
+  return ch_spectral_step(p, u, u_out, ws, prm, static_cast<hipStream_t>(stream), 0);
+}
+
+extern "C" int evx_rd_imex_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
+                                void* stream) {
+  return rd_spectral_step(p, u, u_out, ws, prm, static_cast<hipStream_t>(stream), 0);
+}
+
+extern "C" int evx_ch_etd1_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
+                                void* stream) {
+  return ch_spectral_step(p, u, u_out, ws, prm, static_cast<hipStream_t>(stream), 1);
+}
+
+extern "C" int evx_rd_etd1_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
+                                void* stream) {
+  return rd_spectral_step(p, u, u_out, ws, prm, static_cast<hipStream_t>(stream), 1);
 }
 
 extern "C" int evx_ch_imex_vjp_workspace_bytes(const evx_fft_plan* p, size_t* bytes) {

@@ -39,6 +39,9 @@ struct Prefac {
   // Exact-symbol mode (plan option): the multiplier in float64 from the
   // exact wavenumbers instead of the reference's float32 table.
   int exact;
+  // 1: exponential Euler multiplier dt*phi_1(dt*sigma) instead of the
+  // semi-implicit dt/(1 - dt*sigma).
+  int etd;
   double scd, dtd, inv_ndd[3];
 };
 
@@ -64,6 +67,40 @@ __device__ __forceinline__ double prefac_f64(int jx, int jy, int jz, const Prefa
   const double k2 = kx * kx + ky * ky + kz * kz;
   const double sym = p.ch ? sc * (k2 * k2) : sc * k2;
   return p.dtd / (1.0 - p.dtd * sym);
+}
+
+// Exponential Euler (ETD1) multiplier of the reference's ExponentialEuler
+// (timesteppers.py:136-199): z = dt*sigma(k) on the float32 k-table,
+// phi_1(z) = (exp(z)-1)/z, replaced for |z| < 0.5 by the (6,6) Pade form of
+// Hochbruck, Lubich & Selhofer (1998) evaluated by Horner's rule, all in
+// float32 as the reference's torch expressions round; returns dt*phi_1(z).
+__device__ __forceinline__ float etd1_f32(int jx, int jy, int jz, const Prefac& p, float sc) {
+#pragma clang fp contract(off)
+  const int sx = jx < (p.nx + 1) / 2 ? jx : jx - p.nx;
+  const int sy = jy < (p.ny + 1) / 2 ? jy : jy - p.ny;
+  const float kx = p.twopi * (static_cast<float>(sx) * p.inv_nd[0]);
+  const float ky = p.twopi * (static_cast<float>(sy) * p.inv_nd[1]);
+  const float kz = p.twopi * (static_cast<float>(jz) * p.inv_nd[2]);
+  const float k2 = (kx * kx + ky * ky) + kz * kz;
+  const float sym = p.ch ? sc * (k2 * k2) : sc * k2;
+  const float z = p.dt * sym;
+  float phi;
+  if (fabsf(z) < 0.5f) {
+    const float nc[7] = {1.0f, (float)(1.0 / 26), (float)(5.0 / 156), (float)(1.0 / 858), (float)(1.0 / 5720),
+                         (float)(1.0 / 205920), (float)(1.0 / 8648640)};
+    const float dc[7] = {1.0f, (float)(-6.0 / 13), (float)(5.0 / 52), (float)(-5.0 / 429), (float)(1.0 / 1144),
+                         (float)(-1.0 / 25740), (float)(1.0 / 1235520)};
+    float num = nc[6], den = dc[6];
+#pragma unroll
+    for (int k = 5; k >= 0; --k) {
+      num = num * z + nc[k];
+      den = den * z + dc[k];
+    }
+    phi = num / den;
+  } else {
+    phi = (expf(z) - 1.0f) / z;
+  }
+  return p.dt * phi;
 }
 
 __device__ __forceinline__ float prefac_f32(int jx, int jy, int jz, const Prefac& p, float sc) {
@@ -260,7 +297,8 @@ struct STile {
 
 // Strided sweep of brick `tile` (TL adjacent lines).  MODE 0: forward,
 // 1: inverse, 2: forward * prefac * inverse (x axis only) with the float32
-// multiplier, 3: the same with the exact-symbol float64 multiplier.
+// multiplier, 3: the same with the exact-symbol float64 multiplier, 4: the
+// same with the exponential Euler multiplier.
 template <typename T, int N, int MODE>
 __device__ __forceinline__ void strided_tile(cpx<T>* __restrict__ spec, int64_t nlines, const LineMap& lm,
                                              const cpx<T>* __restrict__ twg, const Prefac& pf, int64_t tile,
@@ -295,6 +333,10 @@ __device__ __forceinline__ void strided_tile(cpx<T>* __restrict__ spec, int64_t 
 #pragma unroll
       for (int q = 0; q < E; ++q)
         v[q] = cscale(v[q], static_cast<T>(prefac_f64(tl + TPL * q, jy, jz, pf, ch) * pf.scale));
+    } else if constexpr (MODE == 4) {
+      const float ssc = symbol_scalar(pf, ch);
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = cscale(v[q], static_cast<T>(etd1_f32(tl + TPL * q, jy, jz, pf, ssc)) * sc);
     } else {
       const float ssc = symbol_scalar(pf, ch);
 #pragma unroll

@@ -115,8 +115,14 @@ class PseudoSpectralIMEX(TimeStepper):
             elif isinstance(p, ReactionDiffusion):
                 self._kind = "rd"
         if self._kind is None:
-            self._fft_prefac = self.dt / (1 - self.dt * p.fourier_symbol)
+            self._fft_prefac = self._multiplier()
         self._ws = None
+
+    _FUSED = {"ch": "evx_ch_imex_step", "rd": "evx_rd_imex_step"}
+
+    def _multiplier(self):
+        """Spectral multiplier of the generic path (timesteppers.py:77)."""
+        return self.dt / (1 - self.dt * self.problem.fourier_symbol)
 
     @property
     def order(self) -> int:
@@ -147,7 +153,7 @@ class PseudoSpectralIMEX(TimeStepper):
         """Fused step writing u + update into ``out`` (``out`` may be ``u``)."""
         plan, ws = self.workspace(u)
         prm = self._params()
-        fn = "evx_ch_imex_step" if self._kind == "ch" else "evx_rd_imex_step"
+        fn = self._FUSED[self._kind]
         _lib.call(fn, plan.handle, _lib.ptr(u), _lib.ptr(out), _lib.ptr(ws), C.byref(prm),
                   _lib.stream_handle(u.device))
         return out
@@ -161,3 +167,40 @@ class PseudoSpectralIMEX(TimeStepper):
         dc_fft = self._fft_prefac * vg.rfftn(dc, dc.shape)
         update = vg.irfftn(dc_fft, dc.shape)[:, :u.shape[1]]
         return u + update
+
+
+@dataclass
+class ExponentialEuler(PseudoSpectralIMEX):
+    """First-order exponential time differencing (ETD1) for semilinear
+    problems (timesteppers.py:136-199; Hochbruck, Lubich, Selhofer 1998):
+    u_{n+1} = u_n + dt*phi_1(dt*L)*rhs(u_n), phi_1(z) = (exp(z)-1)/z, with the
+    (6,6) Pade form for |z| < 0.5.  The multiplier is built from the same
+    float32 symbol as the semi-implicit step.  Fully periodic Cahn-Hilliard
+    and reaction-diffusion problems with default closures run the fused
+    five-sweep pipeline (evx_*_etd1_step)."""
+
+    _FUSED = {"ch": "evx_ch_etd1_step", "rd": "evx_rd_etd1_step"}
+    _PADE_N = (1, 1 / 26, 5 / 156, 1 / 858, 1 / 5720, 1 / 205920, 1 / 8648640)
+    _PADE_D = (1, -6 / 13, 5 / 52, -5 / 429, 1 / 1144, -1 / 25740, 1 / 1235520)
+
+    def _multiplier(self):
+        return self.dt * self.phi1(self.dt * self.problem.fourier_symbol)
+
+    def phi1(self, z):
+        """(exp(z)-1)/z, with the degree-6 Pade approximant where |z| < 0.5
+        (timesteppers.py:155-169)."""
+        phi = (torch.exp(z) - 1) / z
+        idx = torch.abs(z) < 0.5
+        phi[idx] = self.phiPade(z[idx], 6, self._PADE_N, self._PADE_D)
+        return phi
+
+    @staticmethod
+    def phiPade(z, Q, Ncoeff, Dcoeff):
+        """(Q,Q)-Pade approximation of phi_1 by Horner's rule
+        (timesteppers.py:171-190)."""
+        numerator = Ncoeff[Q]
+        denominator = Dcoeff[Q]
+        for k in range(Q - 1, -1, -1):
+            numerator = numerator * z + Ncoeff[k]
+            denominator = denominator * z + Dcoeff[k]
+        return numerator / denominator

@@ -180,6 +180,17 @@ int evx_ch_imex_step(evx_fft_plan* plan, const void* u, void* u_out,
 int evx_rd_imex_step(evx_fft_plan* plan, const void* u, void* u_out,
                      void* workspace, const evx_imex_params* p, void* stream);
 
+/* Exponential Euler (ETD1) steps, u_out = u + irfftn(dt*phi_1(dt*sigma) *
+ * rfftn(rhs(u))), phi_1(z) = (exp(z)-1)/z with the (6,6) Pade form for
+ * |z| < 0.5, on the same float32 symbol as the semi-implicit steps.
+ * Replace ExponentialEuler.step (timesteppers.py:136-199) for CahnHilliard
+ * and ReactionDiffusion (f == 0) on periodic grids.  Same workspace and
+ * parameters as evx_ch_imex_step / evx_rd_imex_step. */
+int evx_ch_etd1_step(evx_fft_plan* plan, const void* u, void* u_out,
+                     void* workspace, const evx_imex_params* p, void* stream);
+int evx_rd_etd1_step(evx_fft_plan* plan, const void* u, void* u_out,
+                     void* workspace, const evx_imex_params* p, void* stream);
+
 /* ---- adjoints (reverse mode) ----------------------------------------------
  * The reference differentiates its solvers with JAX (inversion.py:20-233);
  * these entry points give the vector-Jacobian products of the torch path's

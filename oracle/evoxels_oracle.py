@@ -35,6 +35,9 @@ Parity pins (each checked in tests/test_oracle_*.py):
   rd_rhs/fe/rk4     closed forms on single Fourier modes
   k tables/imex     float32 fftfreq identity, single-mode closed form
                     u1 = u0 + prefac(k)*(-D*lambda_h(k))*u0, brute-force DFT
+  phi1/etd1         Pade branch against expm1(z)/z in float64 (and the
+                    formula branch at the switch point), single-mode closed
+                    form u1 = u0*(1 + dt*phi_1(dt*sigma_k)*D*lambda_h(k))
   solve loop        u+1 with two fields (test_solvers.py:14-27), frame times
   ch_energy         no counterpart in the reference (build-defined discrete
                     functional); pinned by a closed form on a single mode
@@ -54,7 +57,8 @@ __all__ = [
     "to_face", "grad_face", "grad_center", "gradient_norm_squared",
     "normal_laplace", "rd_rhs", "ch_rhs", "ac_rhs", "fft_axes", "rfft_axes",
     "rfft_k_squared", "rd_symbol", "ch_symbol", "ac_symbol", "imex_prefac",
-    "fe_step", "rk4_step", "imex_step", "ch_energy", "average", "run_loop",
+    "fe_step", "rk4_step", "imex_step", "phi1", "etd1_multiplier", "etd1_step", "ch_energy", "average",
+    "run_loop",
     "spacing_for",
 ]
 
@@ -357,6 +361,38 @@ def rk4_step(u, rhs_fn, dt, t=0.0):
     k3 = rhs_fn(t + 0.5 * dt, u + 0.5 * dt * k2)
     k4 = rhs_fn(t + dt, u + dt * k3)
     return u + (dt / 6) * (k1 + 2 * k2 + 2 * k3 + k4)
+
+
+_PADE_N = (1, 1 / 26, 5 / 156, 1 / 858, 1 / 5720, 1 / 205920, 1 / 8648640)
+_PADE_D = (1, -6 / 13, 5 / 52, -5 / 429, 1 / 1144, -1 / 25740, 1 / 1235520)
+
+
+def phi1(z):
+    """ExponentialEuler.phi1 (timesteppers.py:155-169): (exp(z)-1)/z, with the
+    degree-6 Pade approximant (timesteppers.py:171-190, Hochbruck, Lubich,
+    Selhofer 1998) where |z| < 0.5."""
+    phi = (torch.exp(z) - 1) / z
+    idx = torch.abs(z) < 0.5
+    zz = z[idx]
+    num, den = _PADE_N[6], _PADE_D[6]
+    for k in range(5, -1, -1):
+        num = num * zz + _PADE_N[k]
+        den = den * zz + _PADE_D[k]
+    phi[idx] = num / den
+    return phi
+
+
+def etd1_multiplier(dt, symbol):
+    """dt * phi_1(dt * sigma) of ExponentialEuler (timesteppers.py:151,197)."""
+    return dt * phi1(dt * symbol)
+
+
+def etd1_step(u, rhs_fn, mult, t=0.0):
+    """ExponentialEuler.step for a periodic problem (timesteppers.py:195-199)."""
+    dc = rhs_fn(t, u)
+    dc_fft = mult * torch.fft.rfftn(dc, s=dc.shape)
+    update = torch.fft.irfftn(dc_fft, s=dc.shape)[:, :u.shape[1]]
+    return u + update
 
 
 def imex_step(u, rhs_fn, prefac, t=0.0):

@@ -77,6 +77,13 @@ def test_bc_normalisation_matches_reference():
             ReactionDiffusion(_stub_vg(), D=1.0, bc=bad)
 
 
+def test_exponential_euler_rejects_full_neumann_semilinear_problem():
+    """test_solvers.py:165-171."""
+    from evoxels_amd.timesteppers import ExponentialEuler
+    with pytest.raises(ValueError, match="support at most one non-periodic axis"):
+        ExponentialEuler(TwoPhaseAllenCahn(_stub_vg()), 0.1)
+
+
 def test_fft_bc_validation():
     """test_solvers.py:165-171 and problem_definition.py:138-167."""
     with pytest.raises(ValueError, match="support at most one non-periodic axis"):

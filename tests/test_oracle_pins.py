@@ -329,3 +329,41 @@ def test_ch_energy_gradient_term_closed_form():
     grad_part = eps * delta ** 2 * np.sin(2 * np.pi / L) ** 2 * (16 * 16 / 2)
     bulk = 9 / eps * np.sum(np.broadcast_to((c ** 2 * (1 - c) ** 2)[:, None, None], shape))
     assert abs(e - (grad_part + bulk)) < 1e-12 * e
+
+
+def test_phi1_pade_branch_and_formula_branch():
+    """phi_1(z) = expm1(z)/z: the (6,6) Pade branch (|z| < 0.5) agrees with the
+    exact value to float64 roundoff (a dropped or mis-signed coefficient
+    fails), the formula branch is the direct expression, phi_1(0) = 1."""
+    z = torch.tensor([0.0, 1e-8, -1e-3, 0.1, -0.3, 0.49, -0.4999, 0.5, -0.75, -3.0, -40.0, 2.0],
+                     dtype=torch.float64)
+    got = O.phi1(z.clone()).numpy()
+    want = np.array([1.0 if v == 0 else math.expm1(v) / v for v in z.tolist()])
+    assert np.abs(got - want).max() <= 2e-15 * np.abs(want).max()
+    small = np.abs(z.numpy()) < 0.5
+    # Pade (6,6) error is O(z^13): ~1e-17 at |z| = 0.5, far below float64 eps
+    assert np.abs(got[small] - want[small]).max() < 1e-15
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_etd1_single_mode_closed_form(precision):
+    """ReactionDiffusion + ExponentialEuler on one Fourier mode:
+    u1 = u0 * (1 + m(k) * D * lambda_h(k)), m = dt*phi_1(dt*sigma_k) from the
+    float32 symbol sigma = -D*A*k^2."""
+    n, D, dt = 16, 0.8, 2.5
+    g = grid((n, n, n), precision=precision)
+    a = (1, 2, 3)
+    x = np.arange(n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    u0 = np.cos(2 * np.pi * (a[0] * X + a[1] * Y + a[2] * Z) / n)
+    k2 = O.rfft_k_squared((n, n, n), g.spacing)
+    mult = O.etd1_multiplier(dt, O.rd_symbol(k2, D))
+    u1 = O.etd1_step(g.field(u0), lambda t, v: O.rd_rhs(v, g, P3, D), mult)[0].numpy()
+    lam = -sum(4 * np.sin(np.pi * ai / n) ** 2 for ai in a)
+    kk = [np.float32(2 * np.pi) * (np.float32(ai) * np.float32(1.0 / n)) for ai in a]
+    kq = (kk[0] * kk[0] + kk[1] * kk[1]) + kk[2] * kk[2]
+    zf = np.float32(dt) * (np.float32(-D * 0.25) * kq)
+    m = float(np.float32(dt) * np.float32(math.expm1(float(zf)) / float(zf)))
+    expect = u0 * (1 + m * D * lam)
+    # m is a float32 value: one ulp of it (float32 exp vs expm1) is ~1e-6 here
+    assert np.abs(u1 - expect).max() < 5e-6 * np.abs(expect).max()
