@@ -67,6 +67,8 @@ SIGNATURES = {
     "evx_normal_laplace_padded": [_I32, _P, _P, C.POINTER(EvxGrid), _P],
     "evx_rd_rhs": [_I32, _P, _P, C.POINTER(EvxGrid), C.POINTER(EvxBC), _D, C.POINTER(EvxEpilogue), _P],
     "evx_ch_rhs": [_I32, _P, _P, C.POINTER(EvxGrid), C.POINTER(EvxBC), _D, _D, C.POINTER(EvxEpilogue), _P],
+    "evx_coupled_rd_rhs": [_I32, _P, _P, C.POINTER(EvxGrid), C.POINTER(EvxBC), _D, _D, _D, _D,
+                           C.POINTER(EvxEpilogue), _P],
     "evx_ac_rhs": [_I32, _P, _P, C.POINTER(EvxGrid), C.POINTER(EvxBC), _D, _D, _D, _D, _D,
                    C.POINTER(EvxEpilogue), _P],
     "evx_axpy": [_I32, _P, _P, _D, _P, _I64, _P],

@@ -64,6 +64,82 @@ __global__ __launch_bounds__(NT) void rd_rhs_kernel(const T* __restrict__ u, T* 
 }
 
 
+// ---------------------------------------------------------------------------
+// Two species with the Gray-Scott interaction a*b^2
+// (problem_definition.py:612-633):
+//   r_a = D_a*lap(a) - a*b^2 + F*(1 - a),   r_b = D_b*lap(b) + a*b^2 - k*b.
+// One block streams both species through their own plane rings, so the
+// interaction reads each value once; the tile grid has no channel dimension.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct GsCoef {
+  T Da, Db, feed, kill;
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void gs_rhs_kernel(const T* __restrict__ u, T* __restrict__ out, Dims d, Bc bc,
+                                                    Geo<T> g, GsCoef<T> k, Epi ep, TileGrid tg) {
+  constexpr int PY = TileCfg<T>::PY, BY = NTY * PY;
+  constexpr int W = BZ + 2, ROWS = BY + 2, NR = 4;
+  __shared__ T sa[NR][ROWS][W];
+  __shared__ T sb[NR][ROWS][W];
+  const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * BZ + tz;
+  const TileOrigin o = tile_origin<BY>(tg, d, xcd_tile(blockIdx.x, gridDim.x));
+  const int64_t z0 = o.z0, y0 = o.y0, x0 = o.x0, x1 = o.x1;
+  const int64_t vol = d.vol();
+  const T* ua = u;
+  const T* ub = u + vol;
+  PlaneLoader<T, 1, ROWS, W, false> pa, pb;
+  pa.init(y0, z0, d, bc, tid);
+  pb.init(y0, z0, d, bc, tid);
+  for (int64_t xg = x0 - 1; xg <= x0 + 1; ++xg) {
+    pa.fetch(ua, xg, d, bc);
+    pb.fetch(ub, xg, d, bc);
+    pa.commit(&sa[ring<NR>(xg - x0)][0][0], bc, tid);
+    pb.commit(&sb[ring<NR>(xg - x0)][0][0], bc, tid);
+  }
+  if (x0 + 2 <= x1) {
+    pa.fetch(ua, x0 + 2, d, bc);
+    pb.fetch(ub, x0 + 2, d, bc);
+  }
+  __syncthreads();
+  const int64_t plane = d.ny * d.nz;
+  const int64_t gz = z0 + tz;
+  for (int64_t x = x0; x < x1; ++x) {
+    if (x + 2 <= x1) {
+      pa.commit(&sa[ring<NR>(x + 2 - x0)][0][0], bc, tid);
+      pb.commit(&sb[ring<NR>(x + 2 - x0)][0][0], bc, tid);
+    }
+    if (x + 3 <= x1) {
+      pa.fetch(ua, x + 3, d, bc);
+      pb.fetch(ub, x + 3, d, bc);
+    }
+    const T(*Am)[W] = sa[ring<NR>(x - 1 - x0)];
+    const T(*A0)[W] = sa[ring<NR>(x - x0)];
+    const T(*Ap)[W] = sa[ring<NR>(x + 1 - x0)];
+    const T(*Bm)[W] = sb[ring<NR>(x - 1 - x0)];
+    const T(*B0)[W] = sb[ring<NR>(x - x0)];
+    const T(*Bp)[W] = sb[ring<NR>(x + 1 - x0)];
+#pragma unroll
+    for (int q = 0; q < PY; ++q) {
+      const int yy = ty + NTY * q;
+      const int64_t gy = y0 + yy;
+      if (gy >= d.ny || gz >= d.nz) continue;
+      const int a = yy + 1, b = tz + 1;
+      const T av = A0[a][b], bv = B0[a][b];
+      const T la = lap7(av, Ap[a][b], Am[a][b], A0[a + 1][b], A0[a - 1][b], A0[a][b + 1], A0[a][b - 1], g);
+      const T lb = lap7(bv, Bp[a][b], Bm[a][b], B0[a + 1][b], B0[a - 1][b], B0[a][b + 1], B0[a][b - 1], g);
+      const T inter = av * (bv * bv);
+      const T ra = (k.Da * la - inter) + k.feed * (T(1) - av);
+      const T rb = (k.Db * lb + inter) - k.kill * bv;
+      const int64_t i = x * plane + gy * d.nz + gz;
+      epilogue_v(ep, av, out, i, ra);
+      epilogue_v(ep, bv, out, vol + i, rb);
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T, bool STRICT>
 __global__ __launch_bounds__(NT) void ch_rhs_kernel(const T* __restrict__ u, T* __restrict__ out, Dims d, Bc bc,
                                                     Geo<T> g, ChCoef<T> k, Epi ep, TileGrid tg,
@@ -266,6 +342,19 @@ int launch_rd(const void* u, void* out, const Dims& d, const Bc& bc, const evx_g
   return check_launch("rd_rhs_kernel");
 }
 
+template <typename T>
+int launch_gs(const void* u, void* out, const Dims& d, const Bc& bc, const evx_grid* g, double Da, double Db,
+              double feed, double kill, const Epi& ep, hipStream_t s) {
+  TileGrid tg;
+  Dims d1 = d;
+  d1.c = 1;  // one block covers both species of its tile
+  dim3 grid = stencil_grid<T>(d1, &tg);
+  GsCoef<T> k{static_cast<T>(Da), static_cast<T>(Db), static_cast<T>(feed), static_cast<T>(kill)};
+  hipLaunchKernelGGL(gs_rhs_kernel<T>, grid, dim3(BZ, NTY), 0, s, static_cast<const T*>(u),
+                     static_cast<T*>(out), d1, bc, make_geo<T>(g), k, ep, tg);
+  return check_launch("gs_rhs_kernel");
+}
+
 constexpr int SLAB_XCHUNK = 16;
 
 template <typename T, bool STRICT = true>
@@ -394,6 +483,21 @@ extern "C" int evx_rd_rhs(int32_t dtype, const void* u, void* out, const evx_gri
   hipStream_t s = static_cast<hipStream_t>(stream);
   return dtype == EVX_F32 ? launch_rd<float>(u, out, d, b, g, D, ep, s)
                           : launch_rd<double>(u, out, d, b, g, D, ep, s);
+}
+
+extern "C" int evx_coupled_rd_rhs(int32_t dtype, const void* u, void* out, const evx_grid* g, const evx_bc* bc,
+                                  double D_A, double D_B, double feed, double kill, const evx_epilogue* e,
+                                  void* stream) {
+  Dims d;
+  Bc b;
+  Epi ep;
+  if (!bc) return fail(EVX_ERR_INVALID, "null bc");
+  EVX_TRY(check_common(dtype, u, out, g, bc, &d, &b));
+  EVX_TRY(check_epi(e, u, out, &ep));
+  if (d.c != 2) return fail(EVX_ERR_INVALID, "the two-species right-hand side needs exactly two channels");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return dtype == EVX_F32 ? launch_gs<float>(u, out, d, b, g, D_A, D_B, feed, kill, ep, s)
+                          : launch_gs<double>(u, out, d, b, g, D_A, D_B, feed, kill, ep, s);
 }
 
 extern "C" int evx_ch_rhs(int32_t dtype, const void* u, void* out, const evx_grid* g, const evx_bc* bc,

@@ -17,6 +17,7 @@ from abc import ABC, abstractmethod
 from dataclasses import dataclass, field
 from typing import Any, Callable
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -345,3 +346,218 @@ class TwoPhaseAllenCahn(SemiLinearODE):
         df_dphi = self.gab * (laplace + n_laplace - potential / 2 / self.eps) \
             + 3 / self.eps * phi * (1 - phi) * self.force
         return self.M * df_dphi
+
+
+@dataclass
+class MultiPhaseAllenCahn(SemiLinearODE):
+    """problem_definition.py:450-573: N phase fractions in the channels,
+    projected onto the Gibbs simplex, double-well potential, curvature-
+    controlled gradient term through the HIP Laplacian and normal Laplacian
+    on padded phases."""
+    vg: VoxelGridHip
+    eps: float = 3.0
+    gab: float = 1.0
+    M: float = 1.0
+    force: float = 0.0
+    curvature: float = 1.0
+    potential: str = "well"
+    fast: bool = True
+    bc: tuple = ("periodic", "periodic", "periodic")
+    _fourier_symbol: Any = field(init=False, repr=False, default=None)
+
+    def __post_init__(self):
+        self.initialize_boundary_conditions()
+        self._fourier_symbol = None
+        if self.potential == "well":
+            self.pot_factor = 9 / (2 * self.eps ** 2)
+            self.calc_potential_derivatives = self._calc_well_derivatives
+        else:
+            raise ValueError(f"Unknown potential type: {self.potential}")
+        self.project_to_simplex = (self._sloppy_simplex_projection if self.fast
+                                   else self._euclidean_simplex_projection)
+
+    @property
+    def order(self):
+        return 2
+
+    @property
+    def fourier_symbol(self):
+        if self._fourier_symbol is None:
+            self._fourier_symbol = -self.M * self.gab * self.k_squared()
+        return self._fourier_symbol
+
+    def rhs_analytic(self, t, phis):
+        sum_phi_squared = sum(phi ** 2 for phi in phis)
+        df_dphi = []
+        for phi in phis:
+            grad = spv.gradient(phi)
+            laplace = spv.laplacian(phi)
+            norm_grad = sp.sqrt(grad.dot(grad))
+            curv = norm_grad * spv.divergence(grad / norm_grad)
+            grad_term = -self.curvature * laplace - (1 - self.curvature) * (laplace - curv)
+            pot_term = self.pot_factor * (3 * phi * (sum_phi_squared - phi ** 2) + phi ** 3 - phi)
+            df_dphi.append(grad_term + pot_term)
+        mean_df = sum(df_dphi) / len(df_dphi)
+        return tuple(-self.M * self.gab * (df - mean_df) for df in df_dphi)
+
+    def _sloppy_simplex_projection(self, phis):
+        phis = torch.clip(phis, min=0.0)
+        return phis / torch.sum(phis, dim=0, keepdim=True)
+
+    def _euclidean_simplex_projection(self, phis):
+        """Euclidean projection onto {phi >= 0, sum phi = 1} per voxel."""
+        n = phis.shape[0]
+        u = self.vg.sort(phis, dim=0, descending=True)
+        cssv = self.vg.cumsum(u, dim=0) - 1.0
+        k = self.vg.arange(1, n + 1).reshape(n, 1, 1, 1)
+        cond = (u - cssv / k) > 0
+        n_active = torch.clip(torch.sum(cond, dim=0, keepdim=False), min=1)
+        theta = self.vg.take_along_dim(cssv, (n_active - 1)[None, ...], dim=0) / n_active[None, ...]
+        return torch.clip(phis - theta, min=0.0)
+
+    def _calc_well_derivatives(self, phis):
+        sum_phi_squared = torch.sum(phis ** 2, dim=0, keepdim=True)
+        df_dphi = 3 * phis * (sum_phi_squared - phis ** 2) + phis ** 3 - phis
+        return self.pot_factor * df_dphi
+
+    def rhs(self, t, phis):
+        phis = self._check_state(phis)
+        phis = self.project_to_simplex(phis)
+        phi_pad = self.pad_bc(phis)
+        dfgrad_dphi = -self.curvature * self.vg.laplace(phi_pad)
+        dfgrad_dphi -= (1 - self.curvature) * self.vg.normal_laplace(phi_pad)
+        df_dphi = dfgrad_dphi + self.calc_potential_derivatives(phis)
+        return -self.M * self.gab * (df_dphi - torch.mean(df_dphi, dim=0, keepdim=True))
+
+
+@dataclass
+class CoupledReactionDiffusion(SemiLinearODE):
+    """problem_definition.py:576-633: two species A = u[0], B = u[1] with
+    Gray-Scott type kinetics.  The default interaction u[0]*u[1]^2 runs as one
+    fused HIP kernel (both species per block, Euler/RK4 in its epilogue); a
+    user interaction takes the generic path (HIP pad and Laplacian, device
+    tensor algebra)."""
+    vg: VoxelGridHip
+    D_A: float = 1.0
+    D_B: float = 0.5
+    feed: float = 0.055
+    kill: float = 0.117
+    interaction: Callable | None = None
+    _fourier_symbol: Any = field(init=False, repr=False, default=None)
+
+    def __post_init__(self):
+        self.initialize_boundary_conditions()
+        self._fourier_symbol = None
+        if self.interaction is None:
+            self.interaction = _default(lambda u, lib=None: u[0] * u[1] ** 2)
+
+    @property
+    def order(self):
+        return 2
+
+    @property
+    def fourier_symbol(self):
+        if self._fourier_symbol is None:
+            self._fourier_symbol = -max(self.D_A, self.D_B) * self.k_squared()
+        return self._fourier_symbol
+
+    @property
+    def fused(self):
+        return _is_default(self.interaction)
+
+    def _eval_interaction(self, u, lib):
+        try:
+            return self.interaction(u, lib)
+        except TypeError:
+            return self.interaction(u)
+
+    def launch(self, u, out, epilogue):
+        if u.shape[0] != 2:
+            raise ValueError(f"CoupledReactionDiffusion needs two channels (A, B), got {u.shape[0]}")
+        g = self._grid(u)
+        b = _lib.make_bc(self.bc)
+        _lib.call("evx_coupled_rd_rhs", _lib.dtype_code(u.dtype), _lib.ptr(u), _lib.ptr(out), C.byref(g),
+                  C.byref(b), float(self.D_A), float(self.D_B), float(self.feed), float(self.kill),
+                  C.byref(epilogue), _lib.stream_handle(u.device))
+
+    def rhs_analytic(self, t, u):
+        interaction = self._eval_interaction(u, sp)
+        dc_A = self.D_A * spv.laplacian(u[0]) - interaction + self.feed * (1 - u[0])
+        dc_B = self.D_B * spv.laplacian(u[1]) + interaction - self.kill * u[1]
+        return (dc_A, dc_B)
+
+    def rhs(self, t, u):
+        u = self._check_state(u)
+        if u.shape[0] != 2:
+            raise ValueError(f"CoupledReactionDiffusion needs two channels (A, B), got {u.shape[0]}")
+        if self.fused:
+            out = torch.empty_like(u)
+            self.launch(u, out, _lib.make_epilogue())
+            return out
+        interaction = self._eval_interaction(u, torch)
+        laplace = self.vg.laplace(self.pad_bc(u))
+        dc_A = self.D_A * laplace[0] - interaction + self.feed * (1 - u[0])
+        dc_B = self.D_B * laplace[1] + interaction - self.kill * u[1]
+        return torch.stack((dc_A, dc_B), 0)
+
+
+class SmoothedBoundaryODE(ODE):
+    """problem_definition.py:183-188."""
+
+    @property
+    @abstractmethod
+    def mask(self):
+        ...
+
+
+@dataclass
+class ReactionDiffusionSBM(ReactionDiffusion):
+    """problem_definition.py:231-287: reaction-diffusion on a smoothed
+    boundary (diffuse domain) given by ``mask``; ``bc_flux`` is the flux
+    imposed through the diffuse boundary.  cell_center convention."""
+    mask: Any | None = None
+    bc_flux: Callable | float = 0.0
+
+    def __post_init__(self):
+        super().__post_init__()
+        vg = self.vg
+        if self.mask is None:
+            m = vg.init_scalar_field(np.ones(vg.shape))
+            self.mask = vg.pad_periodic(vg.bc.trim_boundary_nodes(m))
+            self.norm = 1.0
+        else:
+            m = vg.init_scalar_field(np.asarray(self.mask))
+            mask_0 = m[:, 0, :, :]
+            mask_1 = m[:, -1, :, :]
+            self.mask = vg.pad_periodic(vg.bc.trim_boundary_nodes(m))
+            if self.bc_type[0] != "periodic":
+                self.mask[:, 0, 1:-1, 1:-1] = mask_0
+                self.mask[:, -1, 1:-1, 1:-1] = mask_1
+            self.norm = torch.sqrt(vg.gradient_norm_squared(self.mask))
+            self.mask = torch.clip(self.mask, 1e-4, 1)
+            x_bc, y_bc, z_bc = self.bc
+            if x_bc[0] == "dirichlet":
+                self.bc = (("dirichlet", (x_bc[1][0] * self.mask[:, 0, :, :], x_bc[1][1] * self.mask[:, -1, :, :])),
+                           y_bc, z_bc)
+                self._pad_bc = lambda f: self.vg.bc.pad_bc(f, self.bc)
+
+    @property
+    def fused(self):
+        return False
+
+    def rhs_analytic(self, t, u, mask):
+        grad_m = spv.gradient(mask)
+        norm_grad_m = sp.sqrt(grad_m.dot(grad_m))
+        divergence = spv.divergence(self.D * (spv.gradient(u) - u / mask * grad_m))
+        return divergence + norm_grad_m * self.bc_flux + mask * self._eval_f(t, u / mask, sp)
+
+    def rhs(self, t, u):
+        u = self._check_state(u)
+        vg = self.vg
+        z = self.pad_bc(u)
+        m = self.mask
+        divergence = vg.grad_x_face(vg.grad_x_face(z) - vg.to_x_face(z / m) * vg.grad_x_face(m))[:, :, 1:-1, 1:-1]
+        divergence += vg.grad_y_face(vg.grad_y_face(z) - vg.to_y_face(z / m) * vg.grad_y_face(m))[:, 1:-1, :, 1:-1]
+        divergence += vg.grad_z_face(vg.grad_z_face(z) - vg.to_z_face(z / m) * vg.grad_z_face(m))[:, 1:-1, 1:-1, :]
+        inner = m[:, 1:-1, 1:-1, 1:-1]
+        return self.D * divergence + self.norm * self.bc_flux + inner * self._eval_f(t, u / inner, torch)

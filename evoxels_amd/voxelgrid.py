@@ -221,6 +221,8 @@ class VoxelGridHip(FDStencils):
         return field.cpu().numpy()
 
     def pad_bc(self, field, bc):
+        if any(kind == "dirichlet" and any(isinstance(v, torch.Tensor) for v in values) for kind, values in bc):
+            return self._pad_bc_field_values(field, bc)
         field = field.contiguous()
         c, nx, ny, nz = field.shape
         out = torch.empty((c, nx + 2, ny + 2, nz + 2), dtype=field.dtype, device=field.device)
@@ -229,6 +231,24 @@ class VoxelGridHip(FDStencils):
         _lib.call("evx_pad_bc", _lib.dtype_code(field.dtype), _lib.ptr(field), _lib.ptr(out), C.byref(g),
                   C.byref(b), _lib.stream_handle(field.device))
         return out
+
+    def _pad_bc_field_values(self, field, bc):
+        """pad_bc with Dirichlet values given as fields on the boundary planes
+        (the smoothed-boundary problem scales them by its mask): the periodic
+        pad in HIP, then the per-axis ghost rules in the order x, y, z as
+        boundary_conditions.py:33-59 applies them."""
+        padded = self.pad_periodic(field)
+        for a, (kind, values) in enumerate(bc):
+            if kind == "periodic":
+                continue
+            idx = lambda i: (slice(None),) * (a + 1) + (i,)  # noqa: E731
+            if kind == "neumann":
+                padded[idx(0)] = padded[idx(1)]
+                padded[idx(-1)] = padded[idx(-2)]
+            else:
+                padded[idx(0)] = 2.0 * values[0] - padded[idx(1)]
+                padded[idx(-1)] = 2.0 * values[1] - padded[idx(-2)]
+        return padded
 
     def pad_periodic(self, field):
         return self.pad_bc(field, PERIODIC3)
@@ -281,6 +301,31 @@ class VoxelGridHip(FDStencils):
 
     def arange(self, start, stop):
         return torch.arange(start, stop, dtype=self.precision, device=self.device)
+
+    def argmax(self, field, dim=None, keepdim=False):
+        return torch.argmax(field, dim=dim, keepdim=keepdim)
+
+    def cumsum(self, field, dim=None):
+        return torch.cumsum(field, dim=dim)
+
+    def sort(self, field, dim=0, descending=False):
+        vals, _ = torch.sort(field, dim=dim, descending=descending)
+        return vals
+
+    def take_along_dim(self, field, idx, dim=0):
+        return torch.take_along_dim(field, idx, dim=dim)
+
+    def meshgrid(self):
+        """Full 3-D mesh of the cell coordinates (voxelgrid.py:92-95)."""
+        return tuple(torch.meshgrid(*self.axes(), indexing="ij"))
+
+    def fft_mesh(self):
+        return tuple(torch.meshgrid(*self.fft_axes(), indexing="ij"))
+
+    def fft_k_squared(self):
+        """Full-spectrum k^2 (voxelgrid.py:105-108), float32 as fftfreq returns."""
+        kx, ky, kz = torch.meshgrid(*self.fft_axes(), indexing="ij")
+        return kx ** 2 + ky ** 2 + kz ** 2
 
     # ---------------------------------------------------------- k-space
     def axes(self):

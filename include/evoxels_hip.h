@@ -127,6 +127,15 @@ int evx_ac_rhs(int32_t dtype, const void* u, void* out, const evx_grid* g,
                const evx_bc* bc, double eps, double gab, double M, double force,
                double curvature, const evx_epilogue* ep, void* stream);
 
+/* CoupledReactionDiffusion with the default interaction u[0]*u[1]^2 on a
+ * two-channel state (A = channel 0, B = channel 1):
+ * r_A = D_A*lap(A) - A*B^2 + feed*(1-A),  r_B = D_B*lap(B) + A*B^2 - kill*B.
+ * Replaces CoupledReactionDiffusion.rhs (problem_definition.py:612-633);
+ * g->c must be 2. */
+int evx_coupled_rd_rhs(int32_t dtype, const void* u, void* out, const evx_grid* g,
+                       const evx_bc* bc, double D_A, double D_B, double feed,
+                       double kill, const evx_epilogue* ep, void* stream);
+
 /* Elementwise out = a + coef*b (coef rounded to the working precision).
  * Used for generic-path Euler / RK4 stage updates (timesteppers.py:42-61). */
 int evx_axpy(int32_t dtype, const void* a, const void* b, double coef,

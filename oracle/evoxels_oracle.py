@@ -39,6 +39,13 @@ Parity pins (each checked in tests/test_oracle_*.py):
                     formula branch at the switch point), single-mode closed
                     form u1 = u0*(1 + dt*phi_1(dt*sigma_k)*D*lambda_h(k))
   solve loop        u+1 with two fields (test_solvers.py:14-27), frame times
+  multiphase_ac_rhs MMS slope of test_rhs.py:61-79 (Neumann x, float64),
+                    phase sum of the rhs is zero, simplex projections agree on
+                    points already on the simplex
+  coupled_rd_rhs    MMS slope of test_rhs.py:43-52 (float64), spatially
+                    uniform state gives the ODE kinetics exactly
+  rd_sbm_rhs        with mask == 1 it reduces to rd_rhs + bc_flux; MMS slope
+                    of the cell_center variant of test_rhs.py:82-97
   ch_energy         no counterpart in the reference (build-defined discrete
                     functional); pinned by a closed form on a single mode
 """
@@ -55,7 +62,8 @@ import torch
 __all__ = [
     "OracleGrid", "normalize_bc", "pad_periodic", "pad_bc", "laplace",
     "to_face", "grad_face", "grad_center", "gradient_norm_squared",
-    "normal_laplace", "rd_rhs", "ch_rhs", "ac_rhs", "fft_axes", "rfft_axes",
+    "normal_laplace", "rd_rhs", "ch_rhs", "ac_rhs", "multiphase_ac_rhs", "coupled_rd_rhs", "rd_sbm_setup",
+    "rd_sbm_rhs", "fft_axes", "rfft_axes",
     "rfft_k_squared", "rd_symbol", "ch_symbol", "ac_symbol", "imex_prefac",
     "fe_step", "rk4_step", "imex_step", "phi1", "etd1_multiplier", "etd1_step", "ch_energy", "average",
     "run_loop",
@@ -303,6 +311,79 @@ def ac_rhs(phi, g: OracleGrid, bc, eps: float = 2.0, gab: float = 1.0,
     n_lap = (1 - curvature) * normal_laplace(phi_pad, g.div_dx, g.div_dx2)
     df_dphi = gab * (lap + n_lap - pot / 2 / eps) + 3 / eps * phi * (1 - phi) * force
     return M * df_dphi
+
+
+def multiphase_ac_rhs(phis, g: OracleGrid, bc, eps=3.0, gab=1.0, M=1.0, curvature=1.0, fast=True):
+    """MultiPhaseAllenCahn.rhs (problem_definition.py:508-573), 'well'
+    potential; ``fast`` selects the sloppy or the Euclidean simplex
+    projection (problem_definition.py:508-530)."""
+    if fast:
+        phis = torch.clip(phis, min=0.0)
+        phis = phis / torch.sum(phis, dim=0, keepdim=True)
+    else:
+        n = phis.shape[0]
+        u, _ = torch.sort(phis, dim=0, descending=True)
+        cssv = torch.cumsum(u, dim=0) - 1.0
+        k = torch.arange(1, n + 1, dtype=phis.dtype).reshape(n, 1, 1, 1)
+        cond = (u - cssv / k) > 0
+        n_active = torch.clip(torch.sum(cond, dim=0, keepdim=False), min=1)
+        theta = torch.take_along_dim(cssv, (n_active - 1)[None, ...], dim=0) / n_active[None, ...]
+        phis = torch.clip(phis - theta, min=0.0)
+    phi_pad = pad_bc(phis, bc)
+    dfgrad = -curvature * laplace(phi_pad, g.div_dx2)
+    dfgrad -= (1 - curvature) * normal_laplace(phi_pad, g.div_dx, g.div_dx2)
+    pot_factor = 9 / (2 * eps ** 2)
+    sum_sq = torch.sum(phis ** 2, dim=0, keepdim=True)
+    dfpot = pot_factor * (3 * phis * (sum_sq - phis ** 2) + phis ** 3 - phis)
+    df = dfgrad + dfpot
+    return -M * gab * (df - torch.mean(df, dim=0, keepdim=True))
+
+
+def coupled_rd_rhs(u, g: OracleGrid, bc, D_A=1.0, D_B=0.5, feed=0.055, kill=0.117):
+    """CoupledReactionDiffusion.rhs (problem_definition.py:612-633), default
+    interaction u[0]*u[1]^2."""
+    interaction = u[0] * u[1] ** 2
+    lap = laplace(pad_bc(u, bc), g.div_dx2)
+    dc_A = D_A * lap[0] - interaction + feed * (1 - u[0])
+    dc_B = D_B * lap[1] + interaction - kill * u[1]
+    return torch.stack((dc_A, dc_B), 0)
+
+
+def rd_sbm_setup(g: OracleGrid, bc, mask=None):
+    """ReactionDiffusionSBM.__post_init__ (problem_definition.py:236-262) for
+    the cell_center convention: returns (padded mask, norm, bc)."""
+    if mask is None:
+        m = pad_periodic(g.field(np.ones(g.shape)))
+        return m, 1.0, bc
+    mf = g.field(mask)
+    mask_0, mask_1 = mf[:, 0, :, :], mf[:, -1, :, :]
+    m = pad_periodic(mf)
+    if bc[0][0] != "periodic":
+        m[:, 0, 1:-1, 1:-1] = mask_0
+        m[:, -1, 1:-1, 1:-1] = mask_1
+    norm = torch.sqrt(gradient_norm_squared(m, g.div_dx))
+    m = torch.clip(m, 1e-4, 1)
+    if bc[0][0] == "dirichlet":
+        v = bc[0][1]
+        bc = (("dirichlet", (v[0] * m[:, 0, :, :], v[1] * m[:, -1, :, :])), bc[1], bc[2])
+    return m, norm, bc
+
+
+def rd_sbm_rhs(u, g: OracleGrid, bc, D, m, norm, bc_flux=0.0, f=None):
+    """ReactionDiffusionSBM.rhs (problem_definition.py:273-287); ``f(u)`` is
+    the source term (zero when None)."""
+    z = pad_bc(u, bc)
+    div = grad_face(grad_face(z, 0, g.div_dx) - to_face(z / m, 0) * grad_face(m, 0, g.div_dx),
+                    0, g.div_dx)[:, :, 1:-1, 1:-1]
+    div += grad_face(grad_face(z, 1, g.div_dx) - to_face(z / m, 1) * grad_face(m, 1, g.div_dx),
+                     1, g.div_dx)[:, 1:-1, :, 1:-1]
+    div += grad_face(grad_face(z, 2, g.div_dx) - to_face(z / m, 2) * grad_face(m, 2, g.div_dx),
+                     2, g.div_dx)[:, 1:-1, 1:-1, :]
+    out = D * div + norm * bc_flux
+    if f is not None:
+        inner = m[:, 1:-1, 1:-1, 1:-1]
+        out = out + inner * f(u / inner)
+    return out
 
 
 # --------------------------------------------------------------------------

@@ -60,3 +60,56 @@ def convergence(expr, analytic, rhs_numeric, precision="float64", powers=(3, 4, 
     dx, errs = np.array(dx), np.array(errs)
     slope = np.polyfit(np.log(dx), np.log(errs), 1)[0]
     return dx, errs, slope
+
+
+def coupled_rd_analytic(us, D_A=1.0, D_B=0.5, feed=0.055, kill=0.117):
+    """CoupledReactionDiffusion.rhs_analytic (problem_definition.py:605-610)."""
+    inter = us[0] * us[1] ** 2
+    return (D_A * spv.laplacian(us[0]) - inter + feed * (1 - us[0]),
+            D_B * spv.laplacian(us[1]) + inter - kill * us[1])
+
+
+def multiphase_ac_analytic(phis, eps=3.0, gab=1.0, M=1.0, curvature=1.0):
+    """MultiPhaseAllenCahn.rhs_analytic (problem_definition.py:489-506), 'well'."""
+    pot_factor = 9 / (2 * eps ** 2)
+    sum_sq = sum(p ** 2 for p in phis)
+    dfs = []
+    for phi in phis:
+        lap = spv.laplacian(phi)
+        grad_term = -lap
+        if curvature != 1:
+            grad = spv.gradient(phi)
+            ng = sp.sqrt(grad.dot(grad))
+            grad_term = -curvature * lap - (1 - curvature) * (lap - ng * spv.divergence(grad / ng))
+        dfs.append(grad_term + pot_factor * (3 * phi * (sum_sq - phi ** 2) + phi ** 3 - phi))
+    mean = sum(dfs) / len(dfs)
+    return tuple(-M * gab * (df - mean) for df in dfs)
+
+
+def sbm_analytic(u, mask, D=1.0, bc_flux=0.0):
+    """ReactionDiffusionSBM.rhs_analytic (problem_definition.py:264-271), f = 0."""
+    grad_m = spv.gradient(mask)
+    return spv.divergence(D * (spv.gradient(u) - u / mask * grad_m)) + sp.sqrt(grad_m.dot(grad_m)) * bc_flux
+
+
+def convergence_channels(exprs, analytics, rhs_numeric, precision="float64", powers=(3, 4, 5, 6, 7),
+                         mask=None):
+    """Multi-channel form of ``convergence`` (utils.py:58-126): the channels
+    are stacked, ``rhs_numeric(array (C,n,n,n), n, mask_array)`` returns the
+    (C,n,n,n) RHS; returns (dx, errors (C, P), slopes (C,))."""
+    fs = [sp.lambdify((CS.x, CS.y, CS.z), e, "numpy") for e in exprs]
+    gs = [sp.lambdify((CS.x, CS.y, CS.z), a, "numpy") for a in analytics]
+    fm = sp.lambdify((CS.x, CS.y, CS.z), mask, "numpy") if mask is not None else None
+    dx, errs = [], []
+    for p in powers:
+        n = 2 ** p
+        X, Y, Z = grid_points(n, precision)
+        init = np.stack([np.broadcast_to(f(X, Y, Z), X.shape) for f in fs])
+        m = np.broadcast_to(fm(X, Y, Z), X.shape) if fm is not None else None
+        comp = rhs_numeric(np.ascontiguousarray(init), n, m)
+        exact = [np.broadcast_to(g(X, Y, Z), X.shape) for g in gs]
+        errs.append([np.linalg.norm(comp[j] - exact[j]) / np.linalg.norm(exact[j]) for j in range(len(exact))])
+        dx.append(1.0 / n)
+    dx, errs = np.array(dx), np.array(errs).T
+    slopes = np.array([np.polyfit(np.log(dx), np.log(e), 1)[0] for e in errs])
+    return dx, errs, slopes

@@ -367,3 +367,151 @@ def test_etd1_single_mode_closed_form(precision):
     expect = u0 * (1 + m * D * lam)
     # m is a float32 value: one ulp of it (float32 exp vs expm1) is ~1e-6 here
     assert np.abs(u1 - expect).max() < 5e-6 * np.abs(expect).max()
+
+
+# ------------------------------------------- multiphase AC / coupled RD / SBM --
+MPAC_FUNS = (
+    sp.Rational(1, 3) + sp.sin(2 * sp.pi * CS.z) / 20 + sp.cos(2 * sp.pi * CS.x) / 10,
+    sp.Rational(1, 3) + sp.sin(2 * sp.pi * CS.y) / 10 - sp.cos(2 * sp.pi * CS.x) / 10,
+    sp.Rational(1, 3) - sp.sin(2 * sp.pi * CS.z) / 20 - sp.sin(2 * sp.pi * CS.y) / 10,
+)
+N3 = (("neumann", None), ("periodic", None), ("periodic", None))
+
+
+def _oracle_channels(fn):
+    def f(arr, n, mask):
+        g = O.OracleGrid.from_domain((n, n, n), (1, 1, 1), "float64")
+        return fn(torch.tensor(arr), g, mask).numpy()
+    return f
+
+
+@pytest.mark.parametrize("fast", [True, False])
+def test_mms_multiphase_allen_cahn_slopes(fast):
+    """test_rhs.py:55-79: eps 3, gab 1, M 1, curvature 1, Neumann x, float64;
+    both simplex projections (the test phases already lie on the simplex)."""
+    _, _, slopes = mms.convergence_channels(
+        MPAC_FUNS, mms.multiphase_ac_analytic(MPAC_FUNS),
+        _oracle_channels(lambda u, g, m: O.multiphase_ac_rhs(u, g, N3, fast=fast)))
+    assert all(abs(s - 2) < 0.1 for s in slopes), slopes
+
+
+@pytest.mark.parametrize("curvature", [1.0, 0.4])
+def test_multiphase_two_phases_reduce_to_two_phase_allen_cahn(curvature):
+    """With phases (phi, 1 - phi) the multiphase well potential difference is
+    4 phi (1-phi)(1-2phi) and the normal Laplacian is odd in phi, so channel 0
+    equals the pinned two-phase Allen-Cahn RHS (force 0) for any curvature."""
+    g = grid((9, 12, 7), (4.0, 5.0, 3.0))
+    phi = _rand_field((9, 12, 7), 21, 0.05, 0.95)
+    phis = torch.cat([phi, 1 - phi])
+    bc = (("neumann", None), ("periodic", None), ("neumann", None))
+    for eps in (2.0, 3.5):
+        got = O.multiphase_ac_rhs(phis, g, bc, eps=eps, gab=1.3, M=0.7, curvature=curvature)
+        want = O.ac_rhs(phi, g, bc, eps=eps, gab=1.3, M=0.7, force=0.0, curvature=curvature)
+        np.testing.assert_allclose(got[0:1].numpy(), want.numpy(), rtol=0, atol=1e-12)
+        np.testing.assert_allclose(got[1:2].numpy(), -want.numpy(), rtol=0, atol=1e-12)
+
+
+def _simplex_by_bisection(v):
+    """Euclidean projection of one point onto the simplex by bisection on the
+    threshold theta of sum(max(v - theta, 0)) = 1."""
+    lo, hi = v.min() - 1.0, v.max()
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if np.maximum(v - mid, 0).sum() > 1:
+            lo = mid
+        else:
+            hi = mid
+    return np.maximum(v - 0.5 * (lo + hi), 0)
+
+
+def test_euclidean_simplex_projection_against_bisection():
+    """The sort/cumsum projection inside multiphase_ac_rhs (problem_definition.py:
+    514-530) equals an independent bisection; recovered here from the rhs of a
+    zero-gradient state, where the RHS is a function of the projected point."""
+    rng = np.random.default_rng(5)
+    pts = rng.normal(0.3, 0.6, size=(4, 40))
+    g = grid((1, 1, 1), (1.0, 1.0, 1.0))
+    for v in pts.T:
+        want = _simplex_by_bisection(v)
+        u = torch.tensor(v).reshape(4, 1, 1, 1)
+        # constant field: the gradient term vanishes, the well term of the
+        # projected point remains, and it is reproduced from ``want``
+        got = O.multiphase_ac_rhs(u, g, P3, eps=3.0, fast=False)
+        w = torch.tensor(want).reshape(4, 1, 1, 1)
+        pf = 9 / (2 * 9.0)
+        df = pf * (3 * w * (torch.sum(w ** 2, 0, keepdim=True) - w ** 2) + w ** 3 - w)
+        np.testing.assert_allclose(got.numpy(), -(df - df.mean(0, keepdim=True)).numpy(), rtol=0, atol=1e-12)
+        assert abs(want.sum() - 1) < 1e-12 and want.min() >= 0
+
+
+def test_sloppy_projection_and_phase_sum():
+    """Clip-and-normalise projection: the projected phases are nonnegative
+    and sum to one; the RHS of every voxel sums to zero over the phases."""
+    g = grid((8, 6, 10))
+    rng = np.random.default_rng(9)
+    u = torch.tensor(rng.uniform(-0.1, 1.0, (3, 8, 6, 10)))
+    u[0] = u[0].abs() + 0.01  # a voxel whose phases all clip to zero is 0/0, as in the reference
+    r = O.multiphase_ac_rhs(u, g, N3, fast=True)
+    assert float(r.sum(0).abs().max()) < 1e-12
+    # a point already on the simplex is left alone by both projections
+    phis = torch.tensor(rng.dirichlet([2, 2, 2], size=(8, 6, 10)).transpose(3, 0, 1, 2).copy())
+    np.testing.assert_allclose(O.multiphase_ac_rhs(phis, g, N3, fast=True).numpy(),
+                               O.multiphase_ac_rhs(phis, g, N3, fast=False).numpy(), rtol=0, atol=1e-12)
+
+
+RD_FUNS = (0.5 + 0.3 * sp.cos(4 * sp.pi * CS.x) * sp.cos(4 * sp.pi * CS.y) ** 3,
+           0.4 + 0.1 * sp.sin(2 * sp.pi * CS.x) * sp.cos(4 * sp.pi * CS.z))
+
+
+def test_mms_coupled_reaction_diffusion_slopes():
+    """test_rhs.py:40-52: D_A 1, D_B 0.5, default feed/kill, periodic, float64."""
+    _, _, slopes = mms.convergence_channels(
+        RD_FUNS, mms.coupled_rd_analytic(RD_FUNS, 1.0, 0.5),
+        _oracle_channels(lambda u, g, m: O.coupled_rd_rhs(u, g, P3, 1.0, 0.5)))
+    assert all(abs(s - 2) < 0.1 for s in slopes), slopes
+
+
+def test_coupled_rd_uniform_state_is_the_kinetics_ode():
+    """A uniform state has zero Laplacian: the RHS is the Gray-Scott kinetics
+    -a b^2 + F (1 - a), a b^2 - k b at every voxel."""
+    g = grid((5, 4, 6))
+    a, b, F, k = 0.3, 0.6, 0.04, 0.09
+    u = torch.stack([torch.full((5, 4, 6), a, dtype=torch.float64), torch.full((5, 4, 6), b, dtype=torch.float64)])
+    r = O.coupled_rd_rhs(u, g, N3, D_A=0.8, D_B=0.3, feed=F, kill=k)
+    assert float((r[0] - (-0.108 + 0.028)).abs().max()) < 1e-15
+    assert float((r[1] - (0.108 - 0.054)).abs().max()) < 1e-15
+
+
+def _sbm_oracle(bc, D=1.0, bc_flux=0.0):
+    def fn(u, g, mask):
+        m, norm, bcm = O.rd_sbm_setup(g, bc, mask)
+        return O.rd_sbm_rhs(u, g, bcm, D, m, norm, bc_flux)
+    return fn
+
+
+def test_mms_smoothed_boundary_periodic_slope():
+    """Cell-centred form of test_rhs.py:82-97 (the reference runs it on a
+    staggered_x grid): periodic test function times the reference's mask,
+    bc_flux 1, float64, second order."""
+    mask_fun = 0.5 + 0.3 * sp.cos(4 * sp.pi * CS.x) * sp.cos(2 * sp.pi * CS.y)
+    u = (0.6 + 0.2 * sp.sin(2 * sp.pi * CS.x) * sp.cos(2 * sp.pi * CS.z)) * mask_fun
+    _, _, slopes = mms.convergence_channels(
+        (u,), (mms.sbm_analytic(u, mask_fun, 1.0, 1.0),),
+        _oracle_channels(_sbm_oracle(P3, 1.0, 1.0)), mask=mask_fun)
+    assert abs(slopes[0] - 2) < 0.1, slopes
+
+
+@pytest.mark.parametrize("bc", [P3, N3, (("dirichlet", (0.2, 0.7)), ("periodic", None), ("neumann", None))])
+def test_smoothed_boundary_reductions(bc):
+    """Without a mask the SBM RHS is the plain diffusion RHS plus bc_flux; with
+    a constant mask the mask gradients vanish and it is the diffusion RHS."""
+    g = grid((7, 9, 6), (3.0, 4.0, 2.5))
+    u = _rand_field((7, 9, 6), 31)
+    m, norm, bcm = O.rd_sbm_setup(g, bc, None)
+    np.testing.assert_allclose(O.rd_sbm_rhs(u, g, bcm, 0.7, m, norm, 0.25).numpy(),
+                               (O.rd_rhs(u, g, bc, 0.7) + 0.25).numpy(), rtol=0, atol=1e-12)
+    if bc[0][0] != "dirichlet":
+        m, norm, bcm = O.rd_sbm_setup(g, bc, np.full((7, 9, 6), 0.6))
+        assert float(norm.abs().max()) == 0.0
+        np.testing.assert_allclose(O.rd_sbm_rhs(u, g, bcm, 0.7, m, norm, 0.25).numpy(),
+                                   O.rd_rhs(u, g, bc, 0.7).numpy(), rtol=0, atol=1e-12)
