@@ -178,8 +178,13 @@ def make_grid(shape4, div_dx, div_dx2) -> EvxGrid:
     return g
 
 
-def make_bc(bc) -> EvxBC:
-    """bc: normalised ((kind, values), ...) triple."""
+BC_NODE_VALUE, BC_NODE_EXTRAP = 3, 4
+
+
+def make_bc(bc, convention="cell_center") -> EvxBC:
+    """bc: normalised ((kind, values), ...) triple.  On a staggered_x grid the
+    x rules are the boundary-node ones (Dirichlet: the node holds the value,
+    Neumann: 4/3 u1 - 1/3 u2)."""
     b = EvxBC()
     for a, (kind, values) in enumerate(bc):
         if kind not in BC_KIND:
@@ -188,6 +193,8 @@ def make_bc(bc) -> EvxBC:
         if kind == "dirichlet":
             b.lo[a] = float(values[0])
             b.hi[a] = float(values[1])
+        if convention == "staggered_x" and a == 0 and kind != "periodic":
+            b.kind[a] = BC_NODE_VALUE if kind == "dirichlet" else BC_NODE_EXTRAP
     return b
 
 

@@ -37,7 +37,10 @@ struct Bc {
   int kind[3];
   double two_lo[3];
   double two_hi[3];
+  double node_lo, node_hi;  // staggered_x boundary node values of the x axis
 };
+
+__host__ __device__ inline bool staggered_x(const Bc& bc) { return bc.kind[0] >= EVX_BC_NODE_VALUE; }
 
 int make_dims(const evx_grid* g, Dims* d);
 int make_bc(const evx_bc* bc, Bc* out);

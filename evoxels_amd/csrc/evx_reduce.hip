@@ -149,6 +149,7 @@ extern "C" int evx_ch_energy(int32_t dtype, const void* c, const evx_grid* g, co
   if (!c || !out || !workspace || !bc) return fail(EVX_ERR_INVALID, "bad arguments");
   EVX_TRY(make_dims(g, &d));
   EVX_TRY(make_bc(bc, &b));
+  if (staggered_x(b)) return fail(EVX_ERR_UNSUPPORTED, "the energy implements the cell_center rules only");
   if (!(eps != 0.0)) return fail(EVX_ERR_INVALID, "eps must be nonzero");
   if (d.c > 65535) return fail(EVX_ERR_UNSUPPORTED, "too many channels");
   hipStream_t s = static_cast<hipStream_t>(stream);

@@ -41,13 +41,20 @@ int make_dims(const evx_grid* g, Dims* d) {
 int make_bc(const evx_bc* bc, Bc* out) {
   for (int a = 0; a < 3; ++a) {
     const int k = bc->kind[a];
-    if (k != EVX_BC_PERIODIC && k != EVX_BC_NEUMANN && k != EVX_BC_DIRICHLET)
+    if (k == EVX_BC_NODE_VALUE || k == EVX_BC_NODE_EXTRAP) {
+      // staggered_x rules: x only, y and z periodic (boundary_conditions.py:85-131)
+      if (a != 0 || bc->kind[1] != EVX_BC_PERIODIC || bc->kind[2] != EVX_BC_PERIODIC)
+        return fail(EVX_ERR_INVALID, "staggered_x boundary rules need a non-periodic x axis and periodic y, z");
+    } else if (k != EVX_BC_PERIODIC && k != EVX_BC_NEUMANN && k != EVX_BC_DIRICHLET) {
       return fail(EVX_ERR_INVALID, "Unsupported BC type code %d on axis %d", k, a);
+    }
     out->kind[a] = k;
     // 2.0 * value in Python double (boundary_conditions.py:54-55)
     out->two_lo[a] = 2.0 * bc->lo[a];
     out->two_hi[a] = 2.0 * bc->hi[a];
   }
+  out->node_lo = bc->lo[0];
+  out->node_hi = bc->hi[0];
   return EVX_OK;
 }
 

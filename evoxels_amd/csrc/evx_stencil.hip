@@ -32,7 +32,7 @@ __global__ __launch_bounds__(NT) void rd_rhs_kernel(const T* __restrict__ u, T* 
   const int64_t z0 = o.z0, y0 = o.y0, x0 = o.x0, x1 = o.x1;
   const T* uc = u + o.ch * d.vol();
   T* oc = out + o.ch * d.vol();
-  PlaneLoader<T, 1, ROWS, W, false> pl;
+  PlaneLoader<T, 1, ROWS, W, false, true> pl;
   pl.init(y0, z0, d, bc, tid);
   for (int64_t xg = x0 - 1; xg <= x0 + 1; ++xg) {
     pl.fetch(uc, xg, d, bc);
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(NT) void gs_rhs_kernel(const T* __restrict__ u, T* 
   const int64_t vol = d.vol();
   const T* ua = u;
   const T* ub = u + vol;
-  PlaneLoader<T, 1, ROWS, W, false> pa, pb;
+  PlaneLoader<T, 1, ROWS, W, false, true> pa, pb;
   pa.init(y0, z0, d, bc, tid);
   pb.init(y0, z0, d, bc, tid);
   for (int64_t xg = x0 - 1; xg <= x0 + 1; ++xg) {
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(NT) void ac_rhs_kernel(const T* __restrict__ u, T* 
   const int64_t z0 = o.z0, y0 = o.y0, x0 = o.x0, x1 = o.x1;
   const T* uc = u + o.ch * d.vol();
   T* oc = out + o.ch * d.vol();
-  PlaneLoader<T, 1, ROWS, W, true> pl;
+  PlaneLoader<T, 1, ROWS, W, true, true> pl;
   pl.init(y0, z0, d, bc, tid);
   for (int64_t xg = x0 - 1; xg <= x0 + 1; ++xg) {
     pl.fetch(uc, xg, d, bc);
@@ -249,7 +249,18 @@ __global__ void pad_bc_kernel(const T* __restrict__ u, T* __restrict__ out, Dims
     t /= py;
     int64_t xi = t % px, ch = t / px;
     Resolved<T> r = resolve<T>(xi - 1, j - 1, k - 1, d, bc);
-    out[i] = apply_rules(u[ch * d.vol() + r.off], r.o, bc);
+    const T* uc = u + ch * d.vol();
+    if (r.o[0] != 0 && staggered_x(bc)) {
+      // staggered_x ghost plane (boundary_conditions.py:96-121); y, z are periodic
+      if (bc.kind[0] == EVX_BC_NODE_VALUE) {
+        out[i] = static_cast<T>(r.o[0] < 0 ? bc.node_lo : bc.node_hi);
+      } else {
+        const int64_t inward = r.o[0] < 0 ? d.ny * d.nz : -d.ny * d.nz;
+        out[i] = static_cast<T>(4.0 / 3.0) * uc[r.off] - static_cast<T>(1.0 / 3.0) * uc[r.off + inward];
+      }
+      continue;
+    }
+    out[i] = apply_rules(uc[r.off], r.o, bc);
   }
 }
 
@@ -297,7 +308,11 @@ int check_common(int32_t dtype, const void* u, const void* out, const evx_grid* 
   if (dtype != EVX_F32 && dtype != EVX_F64) return fail(EVX_ERR_INVALID, "dtype must be EVX_F32 or EVX_F64");
   if (!u || !out) return fail(EVX_ERR_INVALID, "null field pointer");
   EVX_TRY(make_dims(g, d));
-  if (bc) EVX_TRY(make_bc(bc, b));
+  if (bc) {
+    EVX_TRY(make_bc(bc, b));
+    if (b->kind[0] == EVX_BC_NODE_EXTRAP && d->nx < 2)
+      return fail(EVX_ERR_INVALID, "the staggered_x zero-flux rule needs at least two interior x planes");
+  }
   return EVX_OK;
 }
 
@@ -509,6 +524,8 @@ extern "C" int evx_ch_rhs(int32_t dtype, const void* u, void* out, const evx_gri
   EVX_TRY(check_common(dtype, u, out, g, bc, &d, &b));
   EVX_TRY(check_epi(e, u, out, &ep));
   if (!(eps != 0.0)) return fail(EVX_ERR_INVALID, "eps must be nonzero");
+  if (staggered_x(b))
+    return fail(EVX_ERR_UNSUPPORTED, "the fused Cahn-Hilliard kernel implements the cell_center rules only");
   hipStream_t s = static_cast<hipStream_t>(stream);
   return dtype == EVX_F32 ? launch_ch<float>(u, out, d, b, g, eps, D, ep, s)
                           : launch_ch<double>(u, out, d, b, g, eps, D, ep, s);

@@ -106,14 +106,17 @@ __host__ Geo<T> make_geo(const evx_grid* g) {
 // loads: periodic axes wrap, non-periodic ones clamp and remember the ghost
 // rule.  fetch() then only resolves the (uniform) x index of the plane and
 // issues one load per element; commit() applies clip and the ghost rules and
-// writes the element to LDS slot tid + k*NT of the plane.
-template <typename T, int H, int ROWS, int W, bool CLIP>
+// writes the element to LDS slot tid + k*NT of the plane.  With STAG the
+// staggered_x x rules are available too: an x ghost plane is then the
+// boundary node value or the zero-flux extrapolation of the first two
+// interior planes (of the clipped field when CLIP), formed in fetch().
+template <typename T, int H, int ROWS, int W, bool CLIP, bool STAG = false>
 struct PlaneLoader {
   static constexpr int NPT = (ROWS * W + NT - 1) / NT;
   int off[NPT];      // in-plane offset y*nz + z of the source point, -1 if unused
   uint8_t yzf[NPT];  // y/z ghost flags: 4/8 = y low/high, 16/32 = z low/high
   T v[NPT];
-  uint8_t xf;        // x ghost flag of the fetched plane: 1 = low, 2 = high
+  uint8_t xf;        // x ghost flag of the fetched plane: 1 = low, 2 = high, 64 = final staggered value
 
   __device__ __forceinline__ void init(int64_t y0, int64_t z0, const Dims& d, const Bc& bc, int tid) {
 #pragma unroll
@@ -138,10 +141,39 @@ struct PlaneLoader {
     }
   }
 
+  // staggered_x ghost plane (boundary_conditions.py:96-121), in the order
+  // of the reference's expression fac1*u1 - fac2*u2 with the factors rounded
+  // to the working precision.
+  __device__ __forceinline__ void fetch_node(const T* __restrict__ uc, bool low, const Dims& d, const Bc& bc) {
+    xf = 64;
+    if (bc.kind[0] == EVX_BC_NODE_VALUE) {
+      const T v0 = static_cast<T>(low ? bc.node_lo : bc.node_hi);
+#pragma unroll
+      for (int k = 0; k < NPT; ++k) v[k] = v0;
+      return;
+    }
+    const int64_t plane = d.ny * d.nz;
+    const T* p1 = uc + (low ? 0 : d.nx - 1) * plane;
+    const T* p2 = uc + (low ? 1 : d.nx - 2) * plane;
+    const T f1 = static_cast<T>(4.0 / 3.0), f2 = static_cast<T>(1.0 / 3.0);
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      if (off[k] >= 0) {
+        T a = p1[off[k]], b = p2[off[k]];
+        if (CLIP) {
+          a = clip01(a);
+          b = clip01(b);
+        }
+        v[k] = f1 * a - f2 * b;
+      }
+    }
+  }
+
   __device__ __forceinline__ void fetch(const T* __restrict__ uc, int64_t xg, const Dims& d, const Bc& bc) {
     xf = 0;
     if (xg < 0 || xg >= d.nx) {
       if (bc.kind[0] == EVX_BC_PERIODIC) xg = wrap_near(xg, d.nx);
+      else if (STAG && staggered_x(bc)) { fetch_node(uc, xg < 0, d, bc); return; }
       else { xf = xg < 0 ? 1 : 2; xg = xg < 0 ? 0 : d.nx - 1; }
     }
     const T* pl = uc + xg * (d.ny * d.nz);
@@ -154,6 +186,10 @@ struct PlaneLoader {
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
       if (off[k] >= 0) {
+        if (STAG && (xf & 64)) {
+          flat[tid + k * NT] = v[k];
+          continue;
+        }
         T x = CLIP ? clip01(v[k]) : v[k];
         const int f = xf | yzf[k];
         if (f) {

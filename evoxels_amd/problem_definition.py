@@ -55,6 +55,9 @@ class ODE(ABC):
         bc = getattr(self, "bc", None)
         if bc is None or bc == "fully_periodic":
             self.bc = (("periodic", None), ("periodic", None), ("periodic", None))
+            if self.vg.convention == "staggered_x":
+                raise NotImplementedError("a staggered_x grid carries boundary nodes in x; a fully periodic "
+                                          "problem belongs on a cell_center grid (boundary_conditions.py:89-94)")
             self._pad_bc = lambda f: self.vg.bc.pad_bc(f, self.bc)
             return
         if len(bc) != 3:
@@ -86,11 +89,35 @@ class ODE(ABC):
                     raise ValueError(f"{kind} BCs do not accept boundary values.")
                 normalized.append((kind, None))
         self.bc = tuple(normalized)
+        if self.vg.convention == "staggered_x":
+            # the reference's specialised paddings (problem_definition.py:105-115);
+            # any other combination reaches StaggeredXBCs.pad_bc, which raises
+            if self.bc_type == ("dirichlet", "periodic", "periodic"):
+                self._pad_bc = lambda f: self.vg.bc.pad_dirichlet_periodic(f, self.bc[0][1][0], self.bc[0][1][1])
+            elif self.bc_type == ("neumann", "periodic", "periodic"):
+                self._pad_bc = self.vg.bc.pad_zero_flux_periodic
+            elif self.bc_type == ("periodic", "periodic", "periodic"):
+                raise NotImplementedError("a staggered_x grid carries boundary nodes in x; a fully periodic "
+                                          "problem belongs on a cell_center grid (boundary_conditions.py:89-94)")
+            else:
+                self._pad_bc = lambda f: self.vg.bc.pad_bc(f, self.bc)
+            return
         self._pad_bc = lambda f: self.vg.bc.pad_bc(f, self.bc)
 
     @property
     def bc_type(self):
         return tuple(axis_bc[0] for axis_bc in self.bc)
+
+    def _kernel_bcs(self) -> bool:
+        """Whether the fused kernels implement this problem's ghost rules:
+        every cell_center combination; on staggered_x the two x paddings with
+        scalar node values and periodic y, z."""
+        if self.vg.convention == "cell_center":
+            return True
+        if self.bc_type == ("neumann", "periodic", "periodic"):
+            return True
+        return self.bc_type == ("dirichlet", "periodic", "periodic") and \
+            not any(isinstance(v, torch.Tensor) for v in self.bc[0][1])
 
     # ---- fused-kernel protocol used by the time steppers -----------------
     fused = False
@@ -101,8 +128,10 @@ class ODE(ABC):
     def _check_state(self, u):
         if not isinstance(u, torch.Tensor) or u.device.type != "cuda":
             raise ValueError("state must be a torch tensor on the ROCm device")
-        if u.dim() != 4 or tuple(u.shape[1:]) != tuple(self.vg.shape):
-            raise ValueError(f"state must have shape (C, {self.vg.shape[0]}, {self.vg.shape[1]}, "
+        # the state of a staggered_x grid excludes the two boundary nodes in x
+        nx = self.vg.shape[0] - (2 if self.vg.convention == "staggered_x" else 0)
+        if u.dim() != 4 or tuple(u.shape[1:]) != (nx,) + tuple(self.vg.shape[1:]):
+            raise ValueError(f"state must have shape (C, {nx}, {self.vg.shape[1]}, "
                              f"{self.vg.shape[2]}), got {tuple(u.shape)}")
         return u.contiguous()
 
@@ -173,7 +202,7 @@ class ReactionDiffusion(SemiLinearODE):
 
     @property
     def fused(self):
-        return _is_default(self.f)
+        return _is_default(self.f) and self._kernel_bcs()
 
     @property
     def order(self):
@@ -196,7 +225,7 @@ class ReactionDiffusion(SemiLinearODE):
 
     def launch(self, u, out, epilogue):
         g = self._grid(u)
-        b = _lib.make_bc(self.bc)
+        b = _lib.make_bc(self.bc, self.vg.convention)
         _lib.call("evx_rd_rhs", _lib.dtype_code(u.dtype), _lib.ptr(u), _lib.ptr(out), C.byref(g), C.byref(b),
                   float(self.D), C.byref(epilogue), _lib.stream_handle(u.device))
 
@@ -228,7 +257,7 @@ class CahnHilliard(SemiLinearODE):
 
     @property
     def fused(self):
-        return _is_default(self.mu_hom)
+        return _is_default(self.mu_hom) and self.vg.convention == "cell_center"
 
     @property
     def order(self):
@@ -253,7 +282,7 @@ class CahnHilliard(SemiLinearODE):
 
     def launch(self, u, out, epilogue):
         g = self._grid(u)
-        b = _lib.make_bc(self.bc)
+        b = _lib.make_bc(self.bc, self.vg.convention)
         _lib.call("evx_ch_rhs", _lib.dtype_code(u.dtype), _lib.ptr(u), _lib.ptr(out), C.byref(g), C.byref(b),
                   float(self.eps), float(self.D), C.byref(epilogue), _lib.stream_handle(u.device))
 
@@ -295,7 +324,7 @@ class TwoPhaseAllenCahn(SemiLinearODE):
 
     @property
     def fused(self):
-        return _is_default(self.potential)
+        return _is_default(self.potential) and self._kernel_bcs()
 
     @property
     def order(self):
@@ -326,7 +355,7 @@ class TwoPhaseAllenCahn(SemiLinearODE):
 
     def launch(self, u, out, epilogue):
         g = self._grid(u)
-        b = _lib.make_bc(self.bc)
+        b = _lib.make_bc(self.bc, self.vg.convention)
         _lib.call("evx_ac_rhs", _lib.dtype_code(u.dtype), _lib.ptr(u), _lib.ptr(out), C.byref(g), C.byref(b),
                   float(self.eps), float(self.gab), float(self.M), float(self.force), float(self.curvature),
                   C.byref(epilogue), _lib.stream_handle(u.device))
@@ -463,7 +492,7 @@ class CoupledReactionDiffusion(SemiLinearODE):
 
     @property
     def fused(self):
-        return _is_default(self.interaction)
+        return _is_default(self.interaction) and self._kernel_bcs()
 
     def _eval_interaction(self, u, lib):
         try:
@@ -475,7 +504,7 @@ class CoupledReactionDiffusion(SemiLinearODE):
         if u.shape[0] != 2:
             raise ValueError(f"CoupledReactionDiffusion needs two channels (A, B), got {u.shape[0]}")
         g = self._grid(u)
-        b = _lib.make_bc(self.bc)
+        b = _lib.make_bc(self.bc, self.vg.convention)
         _lib.call("evx_coupled_rd_rhs", _lib.dtype_code(u.dtype), _lib.ptr(u), _lib.ptr(out), C.byref(g),
                   C.byref(b), float(self.D_A), float(self.D_B), float(self.feed), float(self.kill),
                   C.byref(epilogue), _lib.stream_handle(u.device))
@@ -539,7 +568,7 @@ class ReactionDiffusionSBM(ReactionDiffusion):
             if x_bc[0] == "dirichlet":
                 self.bc = (("dirichlet", (x_bc[1][0] * self.mask[:, 0, :, :], x_bc[1][1] * self.mask[:, -1, :, :])),
                            y_bc, z_bc)
-                self._pad_bc = lambda f: self.vg.bc.pad_bc(f, self.bc)
+                self.initialize_boundary_conditions()
 
     @property
     def fused(self):

@@ -99,10 +99,16 @@ class BaseSolver(ABC):
 
     def _handle_outputs(self, u, frame, time, vtk_out, verbose, plot_bounds, colormap):
         """solvers.py:152-187.  For cell_center, trim_ghost_nodes(pad_bc(u))
-        is u itself, so u is exported directly."""
+        is u itself, so u is exported directly; on staggered_x grids the
+        exported field gets its boundary nodes back from the ghost rule."""
         flags = torch.zeros(2, dtype=torch.int32, device=u.device)
         mass = self.vg.channel_sums(u, flags)
-        self._export_fields(u)
+        if self.vg.convention == "cell_center":
+            self._export_fields(u)
+        elif getattr(self, "problem", None) is not None:
+            self._export_fields(self.vg.bc.trim_ghost_nodes(self.problem.pad_bc(u)))
+        else:
+            self._export_fields(self.vg.bc.trim_ghost_nodes(self.vg.pad_zeros(u)))
         self.mass_history.append((frame, time, mass.cpu().numpy()))
         if verbose:
             self.profiler.update_memory_stats()

@@ -137,6 +137,54 @@ class CellCenteredBCs:
         raise ValueError(f"The provided field has the wrong shape {self.vg.shape}.")
 
 
+class StaggeredXBCs:
+    """Ghost rules of the staggered_x convention (boundary_conditions.py:
+    85-156): x carries the boundary nodes, so the x ghost plane of the
+    trimmed (interior) field is the node itself.  The two paddings the
+    reference implements run in one HIP kernel; the others raise as there."""
+
+    def __init__(self, vg):
+        self.vg = vg
+
+    def pad_periodic_BC_staggered_x(self, field):
+        raise NotImplementedError
+
+    def pad_dirichlet_periodic(self, field, bc0=0, bc1=0):
+        """Periodic pad, then the x ghost planes set to bc0 / bc1 (boundary
+        node values; scalars or fields over the padded boundary plane)."""
+        return self.vg.pad_staggered_x(field, "dirichlet", (bc0, bc1))
+
+    def pad_zero_flux_periodic(self, field):
+        """Periodic pad, then x ghost = 4/3 u1 - 1/3 u2 (zero slope of the
+        quadratic through the ghost node and the first two interior nodes)."""
+        return self.vg.pad_staggered_x(field, "neumann", None)
+
+    def pad_bc(self, field, bc):
+        raise NotImplementedError
+
+    def pad_fft_periodic(self, field):
+        raise NotImplementedError
+
+    def pad_fft_dirichlet_periodic(self, field):
+        """Odd extension with the zero boundary nodes: length 2*(Nx-1) for a
+        grid of Nx nodes (boundary_conditions.py:137-140)."""
+        bc = torch.zeros_like(field[:, 0:1])
+        return torch.cat((field, bc, -torch.flip(field, [1]), bc), 1)
+
+    def pad_fft_zero_flux_periodic(self, field):
+        raise NotImplementedError
+
+    def trim_boundary_nodes(self, field):
+        if field.shape[1] == self.vg.shape[0]:
+            return field[:, _i_, :, :]
+        raise ValueError(f"The provided field must have the shape {self.vg.shape}.")
+
+    def trim_ghost_nodes(self, field):
+        if field[0, :, _i_, _i_].shape == self.vg.shape:
+            return field[:, :, _i_, _i_]
+        raise ValueError(f"The provided field has the wrong shape {self.vg.shape}.")
+
+
 class _FFTPlan:
     """Owns one evx_fft_plan (twiddle tables only)."""
 
@@ -181,8 +229,8 @@ class VoxelGridHip(FDStencils):
         self.origin = grid.origin
         self.spacing = grid.spacing
         self.convention = grid.convention
-        if self.convention != "cell_center":
-            raise NotImplementedError("the 'hip' backend implements the cell_center convention")
+        if self.convention not in ("cell_center", "staggered_x"):
+            raise ValueError("Chosen convention must be cell_center or staggered_x.")
         # voxelgrid.py:29-30: spacing cast to the working precision, then 1/x and 1/x**2.
         sp = torch.tensor(np.array(self.spacing), dtype=self.precision)
         dd, dd2 = 1 / sp, 1 / sp ** 2
@@ -190,7 +238,7 @@ class VoxelGridHip(FDStencils):
         self.div_dx2_host = tuple(float(v) for v in dd2)
         self.div_dx = dd.to(self.device)
         self.div_dx2 = dd2.to(self.device)
-        self.bc = CellCenteredBCs(self)
+        self.bc = CellCenteredBCs(self) if self.convention == "cell_center" else StaggeredXBCs(self)
         self._plans = {}
         self._reduce_ws = None
 
@@ -249,6 +297,26 @@ class VoxelGridHip(FDStencils):
                 padded[idx(0)] = 2.0 * values[0] - padded[idx(1)]
                 padded[idx(-1)] = 2.0 * values[1] - padded[idx(-2)]
         return padded
+
+    def pad_staggered_x(self, field, kind, values):
+        """staggered_x x padding with y, z periodic: one HIP kernel for scalar
+        node values and the zero-flux rule; node values given as fields (the
+        smoothed-boundary problem scales them by its mask) are written over
+        the periodic pad."""
+        field = field.contiguous()
+        if kind == "dirichlet" and any(isinstance(v, torch.Tensor) for v in values):
+            padded = self.pad_periodic(field)
+            padded[:, 0, :, :] = values[0]
+            padded[:, -1, :, :] = values[1]
+            return padded
+        bc = ((kind, tuple(values) if values is not None else None), ("periodic", None), ("periodic", None))
+        c, nx, ny, nz = field.shape
+        out = torch.empty((c, nx + 2, ny + 2, nz + 2), dtype=field.dtype, device=field.device)
+        g = self._grid_desc(field.shape)
+        b = _lib.make_bc(bc, "staggered_x")
+        _lib.call("evx_pad_bc", _lib.dtype_code(field.dtype), _lib.ptr(field), _lib.ptr(out), C.byref(g),
+                  C.byref(b), _lib.stream_handle(field.device))
+        return out
 
     def pad_periodic(self, field):
         return self.pad_bc(field, PERIODIC3)
@@ -348,8 +416,10 @@ class VoxelGridHip(FDStencils):
         return kx ** 2 + ky ** 2 + kz ** 2
 
     def rfft_k_squared_nonperiodic(self):
-        """voxelgrid.py:116-124 (cell_center mirror extension of x)."""
-        a_x = 2 * np.pi * torch.fft.fftfreq(2 * self.shape[0], d=self.spacing[0], device=self.device)
+        """voxelgrid.py:116-124: x frequencies of the mirror extension, 2*Nx
+        points (cell_center) or 2*Nx-2 (staggered_x, Nx nodes)."""
+        n = 2 * self.shape[0] if self.convention == "cell_center" else 2 * self.shape[0] - 2
+        a_x = 2 * np.pi * torch.fft.fftfreq(n, d=self.spacing[0], device=self.device)
         _, a_y, _ = self.fft_axes()
         _, _, a_z = self.rfft_axes()
         kx, ky, kz = torch.meshgrid(a_x, a_y, a_z, indexing="ij")
@@ -377,8 +447,13 @@ class VoxelGridHip(FDStencils):
         accumulation, returned in the working precision."""
         if tuple(field.shape[1:]) != self.shape:
             raise ValueError(f"The provided field must have the shape {self.shape}.")
-        s = self.channel_sums(field)
-        return (s / float(np.prod(self.shape))).to(field.dtype)
+        if self.convention == "cell_center":
+            s = self.channel_sums(field)
+            return (s / float(np.prod(self.shape))).to(field.dtype)
+        # staggered_x: the first and last x slices are half cells
+        s = self.channel_sums(field[:, 1:-1]) + 0.5 * self.channel_sums(field[:, 0:1]) \
+            + 0.5 * self.channel_sums(field[:, -1:])
+        return (s / float((self.shape[0] - 1) * self.shape[1] * self.shape[2])).to(field.dtype)
 
     def ch_energy(self, field, eps, bc=PERIODIC3):
         """Discrete CH free energy per channel (float64)."""

@@ -41,11 +41,27 @@ enum evx_status {
 
 enum evx_dtype { EVX_F32 = 0, EVX_F64 = 1 };
 
-/* Boundary kinds of one axis (problem_definition.py:60-103). */
-enum evx_bc_kind { EVX_BC_PERIODIC = 0, EVX_BC_NEUMANN = 1, EVX_BC_DIRICHLET = 2 };
+/* Boundary kinds of one axis (problem_definition.py:60-103).  The first
+ * three are the cell_center rules; the last two are the staggered_x rules of
+ * the x axis, where the ghost plane is the boundary node itself:
+ *   EVX_BC_NODE_VALUE   ghost := v          (StaggeredXBCs.pad_dirichlet_periodic,
+ *                                            boundary_conditions.py:96-106)
+ *   EVX_BC_NODE_EXTRAP  ghost := 4/3 u1 - 1/3 u2 (zero flux,
+ *                                            StaggeredXBCs.pad_zero_flux_periodic,
+ *                                            boundary_conditions.py:108-121)
+ * The staggered kinds are valid on x only, with y and z periodic (the only
+ * staggered_x paddings the reference implements), and are accepted by
+ * evx_pad_bc, evx_rd_rhs, evx_ac_rhs and evx_coupled_rd_rhs. */
+enum evx_bc_kind {
+  EVX_BC_PERIODIC = 0,
+  EVX_BC_NEUMANN = 1,
+  EVX_BC_DIRICHLET = 2,
+  EVX_BC_NODE_VALUE = 3,
+  EVX_BC_NODE_EXTRAP = 4
+};
 
-/* Per-axis boundary description for the cell_center convention.
- * lo/hi are the Dirichlet values (ignored otherwise).  Ghosts follow
+/* Per-axis boundary description.  lo/hi are the Dirichlet (or boundary node)
+ * values, ignored otherwise.  cell_center ghosts follow
  * CellCenteredBCs.pad_bc (boundary_conditions.py:33-59): periodic wrap, then
  * x, y, z rules in that order (Neumann u0 := u1, Dirichlet u0 := 2v - u1). */
 typedef struct evx_bc {

@@ -46,6 +46,13 @@ Parity pins (each checked in tests/test_oracle_*.py):
                     uniform state gives the ODE kinetics exactly
   rd_sbm_rhs        with mask == 1 it reduces to rd_rhs + bc_flux; MMS slope
                     of the cell_center variant of test_rhs.py:82-97
+  pad_staggered_x   slopes of the reference notebook for the staggered_x
+                    Laplacian (10-appendix-convergence-testing.ipynb cells 9,
+                    11), test_laplace.py:63-76 and the staggered smoothed-
+                    boundary test test_rhs.py:82-97
+  spectral_step_bc  closed forms on the eigenmodes of each extension:
+                    cos/sin(pi m (i+1/2)/n) (cell_center Neumann/Dirichlet)
+                    and sin(pi m i/N) (staggered_x Dirichlet)
   ch_energy         no counterpart in the reference (build-defined discrete
                     functional); pinned by a closed form on a single mode
 """
@@ -63,7 +70,8 @@ __all__ = [
     "OracleGrid", "normalize_bc", "pad_periodic", "pad_bc", "laplace",
     "to_face", "grad_face", "grad_center", "gradient_norm_squared",
     "normal_laplace", "rd_rhs", "ch_rhs", "ac_rhs", "multiphase_ac_rhs", "coupled_rd_rhs", "rd_sbm_setup",
-    "rd_sbm_rhs", "fft_axes", "rfft_axes",
+    "rd_sbm_rhs", "fft_axes", "rfft_axes", "pad_staggered_x", "pad_for", "trim_boundary_nodes", "pad_fft",
+    "rfft_k_squared_nonperiodic", "spectral_step_bc",
     "rfft_k_squared", "rd_symbol", "ch_symbol", "ac_symbol", "imex_prefac",
     "fe_step", "rk4_step", "imex_step", "phi1", "etd1_multiplier", "etd1_step", "ch_energy", "average",
     "run_loop",
@@ -189,6 +197,61 @@ def pad_bc(field: torch.Tensor, bc) -> torch.Tensor:
     return padded
 
 
+def pad_staggered_x(field: torch.Tensor, kind, values=None) -> torch.Tensor:
+    """StaggeredXBCs.pad_dirichlet_periodic / pad_zero_flux_periodic
+    (boundary_conditions.py:96-121): periodic pad, then the whole x ghost
+    planes become the boundary node values (Dirichlet) or the zero-slope
+    extrapolation 4/3 u1 - 1/3 u2, low plane first."""
+    padded = pad_periodic(field)
+    if kind == "dirichlet":
+        padded[:, 0, :, :] = values[0]
+        padded[:, -1, :, :] = values[1]
+    elif kind == "neumann":
+        padded[:, 0, :, :] = 4 / 3 * padded[:, 1, :, :] - 1 / 3 * padded[:, 2, :, :]
+        padded[:, -1, :, :] = 4 / 3 * padded[:, -2, :, :] - 1 / 3 * padded[:, -3, :, :]
+    else:
+        raise ValueError(f"Unsupported staggered_x BC type: {kind}")
+    return padded
+
+
+def pad_for(field: torch.Tensor, bc, convention="cell_center") -> torch.Tensor:
+    """ODE.pad_bc as initialize_boundary_conditions wires it
+    (problem_definition.py:60-115): cell_center grids use the generic ghost
+    rules (the specialised paths give identical values); staggered_x grids
+    implement only (dirichlet | neumann, periodic, periodic)."""
+    if convention == "cell_center":
+        return pad_bc(field, bc)
+    kinds = tuple(k for k, _ in bc)
+    if kinds == ("dirichlet", "periodic", "periodic"):
+        return pad_staggered_x(field, "dirichlet", bc[0][1])
+    if kinds == ("neumann", "periodic", "periodic"):
+        return pad_staggered_x(field, "neumann")
+    raise NotImplementedError(f"staggered_x padding for {kinds}")
+
+
+def trim_boundary_nodes(field, convention="cell_center"):
+    """CellCenteredBCs / StaggeredXBCs.trim_boundary_nodes
+    (boundary_conditions.py:73-74,141-148)."""
+    return field if convention == "cell_center" else field[:, 1:-1, :, :]
+
+
+def pad_fft(field, x_kind="periodic", convention="cell_center"):
+    """pad_fft_* of boundary_conditions.py:61-71,133-144: identity (periodic),
+    even (Neumann) or odd (Dirichlet) mirror extension of x; on staggered_x
+    grids the odd extension carries the zero boundary nodes."""
+    if x_kind == "periodic":
+        if convention != "cell_center":
+            raise NotImplementedError
+        return field
+    if convention == "cell_center":
+        sign = 1.0 if x_kind == "neumann" else -1.0
+        return torch.cat((field, sign * torch.flip(field, [1])), 1)
+    if x_kind != "dirichlet":
+        raise NotImplementedError
+    z = torch.zeros_like(field[:, 0:1])
+    return torch.cat((field, z, -torch.flip(field, [1]), z), 1)
+
+
 def pad_zeros(field):
     """Zero ghost layer (voxelgrid.py:197-198)."""
     return torch.nn.functional.pad(field, (1, 1, 1, 1, 1, 1), mode="constant", value=0)
@@ -272,7 +335,7 @@ def normal_laplace(field, div_dx, div_dx2):
 # --------------------------------------------------------------------------
 def rd_rhs(u, g: OracleGrid, bc, D: float, f: Optional[Callable] = None, t=0.0):
     """ReactionDiffusion.rhs (problem_definition.py:226-229); default f = 0 (:202-203)."""
-    lap = laplace(pad_bc(u, bc), g.div_dx2)
+    lap = laplace(pad_for(u, bc, g.convention), g.div_dx2)
     fv = 0 if f is None else f(t, u, torch)
     return D * lap + fv
 
@@ -284,10 +347,10 @@ def ch_rhs(c, g: OracleGrid, bc, eps: float = 3.0, D: float = 1.0,
     if mu_hom is None:
         mu_hom = lambda c_: 18 / eps * c_ * (1 - c_) * (1 - 2 * c_)  # noqa: E731
     c = torch.clip(c, 0, 1)
-    c_bc = pad_bc(c, bc)
+    c_bc = pad_for(c, bc, g.convention)
     lap = laplace(c_bc, g.div_dx2)
     mu = mu_hom(c) - 2 * eps * lap
-    mu = pad_bc(mu, bc)
+    mu = pad_for(mu, bc, g.convention)
     div = grad_face(to_face(c_bc, 0) * (1 - to_face(c_bc, 0)) *
                     grad_face(mu, 0, g.div_dx), 0, g.div_dx)[:, :, 1:-1, 1:-1]
     div += grad_face(to_face(c_bc, 1) * (1 - to_face(c_bc, 1)) *
@@ -306,7 +369,7 @@ def ac_rhs(phi, g: OracleGrid, bc, eps: float = 2.0, gab: float = 1.0,
         potential = lambda u: 18 / eps * u * (1 - u) * (1 - 2 * u)  # noqa: E731
     phi = torch.clip(phi, 0, 1)
     pot = potential(phi)
-    phi_pad = pad_bc(phi, bc)
+    phi_pad = pad_for(phi, bc, g.convention)
     lap = curvature * laplace(phi_pad, g.div_dx2)
     n_lap = (1 - curvature) * normal_laplace(phi_pad, g.div_dx, g.div_dx2)
     df_dphi = gab * (lap + n_lap - pot / 2 / eps) + 3 / eps * phi * (1 - phi) * force
@@ -329,7 +392,7 @@ def multiphase_ac_rhs(phis, g: OracleGrid, bc, eps=3.0, gab=1.0, M=1.0, curvatur
         n_active = torch.clip(torch.sum(cond, dim=0, keepdim=False), min=1)
         theta = torch.take_along_dim(cssv, (n_active - 1)[None, ...], dim=0) / n_active[None, ...]
         phis = torch.clip(phis - theta, min=0.0)
-    phi_pad = pad_bc(phis, bc)
+    phi_pad = pad_for(phis, bc, g.convention)
     dfgrad = -curvature * laplace(phi_pad, g.div_dx2)
     dfgrad -= (1 - curvature) * normal_laplace(phi_pad, g.div_dx, g.div_dx2)
     pot_factor = 9 / (2 * eps ** 2)
@@ -343,21 +406,22 @@ def coupled_rd_rhs(u, g: OracleGrid, bc, D_A=1.0, D_B=0.5, feed=0.055, kill=0.11
     """CoupledReactionDiffusion.rhs (problem_definition.py:612-633), default
     interaction u[0]*u[1]^2."""
     interaction = u[0] * u[1] ** 2
-    lap = laplace(pad_bc(u, bc), g.div_dx2)
+    lap = laplace(pad_for(u, bc, g.convention), g.div_dx2)
     dc_A = D_A * lap[0] - interaction + feed * (1 - u[0])
     dc_B = D_B * lap[1] + interaction - kill * u[1]
     return torch.stack((dc_A, dc_B), 0)
 
 
 def rd_sbm_setup(g: OracleGrid, bc, mask=None):
-    """ReactionDiffusionSBM.__post_init__ (problem_definition.py:236-262) for
-    the cell_center convention: returns (padded mask, norm, bc)."""
+    """ReactionDiffusionSBM.__post_init__ (problem_definition.py:236-262):
+    returns (padded mask, norm, bc).  ``mask`` covers the whole grid (on
+    staggered_x grids including the boundary nodes)."""
     if mask is None:
-        m = pad_periodic(g.field(np.ones(g.shape)))
+        m = pad_periodic(trim_boundary_nodes(g.field(np.ones(g.shape)), g.convention))
         return m, 1.0, bc
     mf = g.field(mask)
     mask_0, mask_1 = mf[:, 0, :, :], mf[:, -1, :, :]
-    m = pad_periodic(mf)
+    m = pad_periodic(trim_boundary_nodes(mf, g.convention))
     if bc[0][0] != "periodic":
         m[:, 0, 1:-1, 1:-1] = mask_0
         m[:, -1, 1:-1, 1:-1] = mask_1
@@ -372,7 +436,7 @@ def rd_sbm_setup(g: OracleGrid, bc, mask=None):
 def rd_sbm_rhs(u, g: OracleGrid, bc, D, m, norm, bc_flux=0.0, f=None):
     """ReactionDiffusionSBM.rhs (problem_definition.py:273-287); ``f(u)`` is
     the source term (zero when None)."""
-    z = pad_bc(u, bc)
+    z = pad_for(u, bc, g.convention)
     div = grad_face(grad_face(z, 0, g.div_dx) - to_face(z / m, 0) * grad_face(m, 0, g.div_dx),
                     0, g.div_dx)[:, :, 1:-1, 1:-1]
     div += grad_face(grad_face(z, 1, g.div_dx) - to_face(z / m, 1) * grad_face(m, 1, g.div_dx),
@@ -402,6 +466,17 @@ def rfft_axes(shape, spacing):
 def rfft_k_squared(shape, spacing):
     """voxelgrid.py:110-114 (float32 table)."""
     a_x, a_y, _ = fft_axes(shape, spacing)
+    _, _, a_z = rfft_axes(shape, spacing)
+    kx, ky, kz = torch.meshgrid(a_x, a_y, a_z, indexing="ij")
+    return kx ** 2 + ky ** 2 + kz ** 2
+
+
+def rfft_k_squared_nonperiodic(shape, spacing, convention="cell_center"):
+    """voxelgrid.py:116-124: the x axis of the mirror extension has 2*Nx
+    points (cell_center) or 2*Nx-2 (staggered_x, Nx nodes); float32 table."""
+    n = 2 * shape[0] if convention == "cell_center" else 2 * shape[0] - 2
+    a_x = 2 * math.pi * torch.fft.fftfreq(n, spacing[0])
+    _, a_y, _ = fft_axes(shape, spacing)
     _, _, a_z = rfft_axes(shape, spacing)
     kx, ky, kz = torch.meshgrid(a_x, a_y, a_z, indexing="ij")
     return kx ** 2 + ky ** 2 + kz ** 2
@@ -484,6 +559,17 @@ def imex_step(u, rhs_fn, prefac, t=0.0):
     frequency, so that axis cancels."""
     dc = rhs_fn(t, u)
     dc_fft = prefac * torch.fft.rfftn(dc, s=dc.shape)
+    update = torch.fft.irfftn(dc_fft, s=dc.shape)[:, :u.shape[1]]
+    return u + update
+
+
+def spectral_step_bc(u, rhs_fn, mult, x_kind="periodic", convention="cell_center", t=0.0):
+    """PseudoSpectralIMEX.step / ExponentialEuler.step with a non-periodic x
+    axis (timesteppers.py:85-89,195-199): the RHS is mirror-extended in x,
+    multiplied in k-space (``mult`` from rfft_k_squared_nonperiodic) and the
+    first Nx planes of the inverse transform are the update."""
+    dc = pad_fft(rhs_fn(t, u), x_kind, convention)
+    dc_fft = mult * torch.fft.rfftn(dc, s=dc.shape)
     update = torch.fft.irfftn(dc_fft, s=dc.shape)[:, :u.shape[1]]
     return u + update
 

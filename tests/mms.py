@@ -44,17 +44,31 @@ def grid_points(n, precision):
     return np.meshgrid(ax, ax, ax, indexing="ij")
 
 
-def convergence(expr, analytic, rhs_numeric, precision="float64", powers=(3, 4, 5, 6, 7)):
-    """Returns (dx, errors, slope).  rhs_numeric(array, n) -> numpy array."""
+def staggered_grid_points(n, precision):
+    """VoxelFields((n+1,n,n),(1,1,1),convention='staggered_x').meshgrid():
+    x nodes 0, 1/n, ..., 1 and cell-centred y, z (voxelfields.py:87-89,183-194)."""
+    sp_ = 1.0 / n
+    ax = np.arange(0, n + 1, dtype=precision) * sp_
+    ay = np.arange(0, n, dtype=precision) * sp_ + sp_ / 2
+    return np.meshgrid(ax, ay, ay, indexing="ij")
+
+
+def convergence(expr, analytic, rhs_numeric, precision="float64", powers=(3, 4, 5, 6, 7),
+                convention="cell_center"):
+    """Returns (dx, errors, slope).  rhs_numeric(array, n) -> numpy array; on
+    staggered_x grids the array holds the boundary nodes and the result the
+    interior, which is compared with the interior of the analytic RHS."""
     f = sp.lambdify((CS.x, CS.y, CS.z), expr, "numpy")
     g = sp.lambdify((CS.x, CS.y, CS.z), analytic, "numpy")
     dx, errs = [], []
     for p in powers:
         n = 2 ** p
-        X, Y, Z = grid_points(n, precision)
+        X, Y, Z = grid_points(n, precision) if convention == "cell_center" else staggered_grid_points(n, precision)
         init = np.broadcast_to(f(X, Y, Z), X.shape)
         comp = rhs_numeric(np.ascontiguousarray(init), n)
         exact = np.broadcast_to(g(X, Y, Z), X.shape)
+        if convention != "cell_center":
+            exact = exact[1:-1]
         errs.append(np.linalg.norm(comp - exact) / np.linalg.norm(exact))
         dx.append(1.0 / n)
     dx, errs = np.array(dx), np.array(errs)
@@ -93,7 +107,7 @@ def sbm_analytic(u, mask, D=1.0, bc_flux=0.0):
 
 
 def convergence_channels(exprs, analytics, rhs_numeric, precision="float64", powers=(3, 4, 5, 6, 7),
-                         mask=None):
+                         mask=None, convention="cell_center"):
     """Multi-channel form of ``convergence`` (utils.py:58-126): the channels
     are stacked, ``rhs_numeric(array (C,n,n,n), n, mask_array)`` returns the
     (C,n,n,n) RHS; returns (dx, errors (C, P), slopes (C,))."""
@@ -103,11 +117,13 @@ def convergence_channels(exprs, analytics, rhs_numeric, precision="float64", pow
     dx, errs = [], []
     for p in powers:
         n = 2 ** p
-        X, Y, Z = grid_points(n, precision)
+        X, Y, Z = grid_points(n, precision) if convention == "cell_center" else staggered_grid_points(n, precision)
         init = np.stack([np.broadcast_to(f(X, Y, Z), X.shape) for f in fs])
         m = np.broadcast_to(fm(X, Y, Z), X.shape) if fm is not None else None
         comp = rhs_numeric(np.ascontiguousarray(init), n, m)
         exact = [np.broadcast_to(g(X, Y, Z), X.shape) for g in gs]
+        if convention != "cell_center":
+            exact = [e[1:-1] for e in exact]
         errs.append([np.linalg.norm(comp[j] - exact[j]) / np.linalg.norm(exact[j]) for j in range(len(exact))])
         dx.append(1.0 / n)
     dx, errs = np.array(dx), np.array(errs).T

@@ -515,3 +515,110 @@ def test_smoothed_boundary_reductions(bc):
         assert float(norm.abs().max()) == 0.0
         np.testing.assert_allclose(O.rd_sbm_rhs(u, g, bcm, 0.7, m, norm, 0.25).numpy(),
                                    O.rd_rhs(u, g, bc, 0.7).numpy(), rtol=0, atol=1e-12)
+
+
+# ---------------------------------------------------- staggered_x and spectral BCs --
+def _staggered_rd(bc, precision="float32", D=1.0):
+    def f(arr, n):
+        g = O.OracleGrid((n + 1, n, n), (1.0 / n, 1.0 / n, 1.0 / n), precision, "staggered_x")
+        u = O.trim_boundary_nodes(g.field(arr), "staggered_x")
+        return O.rd_rhs(u, g, bc, D)[0].numpy()
+    return f
+
+
+def test_mms_staggered_neumann_laplace_matches_notebook_slope():
+    """Zero-flux 4/3, -1/3 ghost on staggered_x, float32: notebook slope
+    2.0915744090903576 (10-appendix-convergence-testing.ipynb cell 9)."""
+    e = sp.cos(sp.pi * CS.x) ** 3
+    bc = (("neumann", None), ("periodic", None), ("periodic", None))
+    _, _, slope = mms.convergence(e, mms.rd_analytic(e), _staggered_rd(bc), "float32", convention="staggered_x")
+    assert abs(slope - KAT["staggered_neumann_laplace_slope"]["value"]) < 2e-3
+
+
+def test_mms_staggered_dirichlet_laplace_matches_notebook_slope():
+    """Boundary-node Dirichlet on staggered_x, float32: notebook slope
+    2.0314069655338005 (cell 11)."""
+    e = (CS.x * (1 - CS.x)) ** 2
+    bc = (("dirichlet", (0, 0)), ("periodic", None), ("periodic", None))
+    _, _, slope = mms.convergence(e, mms.rd_analytic(e), _staggered_rd(bc), "float32", convention="staggered_x")
+    assert abs(slope - KAT["staggered_dirichlet_laplace_slope"]["value"]) < 2e-3
+
+
+def test_mms_staggered_nonzero_dirichlet_slope():
+    """test_laplace.py:63-76: cos(pi x)^3 with node values (1, -1), order 2."""
+    e = sp.cos(sp.pi * CS.x) ** 3
+    bc = (("dirichlet", (1.0, -1.0)), ("periodic", None), ("periodic", None))
+    _, _, slope = mms.convergence(e, mms.rd_analytic(e), _staggered_rd(bc), "float32", convention="staggered_x")
+    assert abs(slope - 2) < 0.1
+
+
+def test_mms_staggered_smoothed_boundary_slope():
+    """test_rhs.py:82-97: smoothed-boundary diffusion on staggered_x, node
+    values (0, 1) scaled by the mask, bc_flux 1, float32, order 2."""
+    mask_fun = 0.5 + 0.3 * sp.cos(4 * sp.pi * CS.x) * sp.cos(2 * sp.pi * CS.y)
+    u = (CS.x + CS.x * (1 - CS.x)) * mask_fun
+    bc = (("dirichlet", (0.0, 1.0)), ("periodic", None), ("periodic", None))
+
+    def fn(arr, n, mask):
+        g = O.OracleGrid((n + 1, n, n), (1.0 / n, 1.0 / n, 1.0 / n), "float32", "staggered_x")
+        m, norm, bcm = O.rd_sbm_setup(g, bc, mask)
+        v = O.trim_boundary_nodes(torch.tensor(arr, dtype=torch.float32), "staggered_x")
+        return O.rd_sbm_rhs(v, g, bcm, 1.0, m, norm, 1.0).numpy()
+    _, _, slopes = mms.convergence_channels((u,), (mms.sbm_analytic(u, mask_fun, 1.0, 1.0),), fn, "float32",
+                                            mask=mask_fun, convention="staggered_x")
+    assert abs(slopes[0] - 2) < 0.1, slopes
+
+
+def test_staggered_pad_by_hand():
+    """Ghost planes of the two staggered_x paddings on a 3x2x2 interior,
+    written out element by element."""
+    u = torch.arange(1.0, 13.0, dtype=torch.float64).reshape(1, 3, 2, 2)
+    p = O.pad_staggered_x(u, "dirichlet", (5.0, -2.0))
+    assert torch.all(p[:, 0] == 5.0) and torch.all(p[:, -1] == -2.0)
+    assert torch.equal(p[:, 1:-1], O.pad_periodic(u)[:, 1:-1])
+    q = O.pad_staggered_x(u, "neumann")
+    per = O.pad_periodic(u)
+    for j in range(4):
+        for k in range(4):
+            assert float(q[0, 0, j, k]) == 4 / 3 * float(per[0, 1, j, k]) - 1 / 3 * float(per[0, 2, j, k])
+            assert float(q[0, -1, j, k]) == 4 / 3 * float(per[0, -2, j, k]) - 1 / 3 * float(per[0, -3, j, k])
+    # a quadratic with zero slope at the boundary node is reproduced exactly
+    xs = torch.arange(1, 4, dtype=torch.float64)  # interior nodes 1..3, boundary node 0
+    quad = (0.7 + 0.2 * xs ** 2).reshape(1, 3, 1, 1).expand(1, 3, 2, 2).contiguous()
+    assert abs(float(O.pad_staggered_x(quad, "neumann")[0, 0, 1, 1]) - 0.7) < 1e-14
+
+
+@pytest.mark.parametrize("case", ["cc_neumann", "cc_dirichlet", "stag_dirichlet"])
+@pytest.mark.parametrize("m", [1, 3])
+def test_nonperiodic_imex_step_on_eigenmodes(case, m):
+    """The mirror extension turns the boundary eigenmodes of the
+    finite-difference Laplacian into single Fourier modes, so one
+    semi-implicit RD step is u1 = u0 (1 + prefac(k) D lambda_h) with
+    k_x = pi m / L and lambda_h the discrete eigenvalue."""
+    n, ny, nz, D, dt = 12, 8, 6, 0.8, 0.3
+    conv = "cell_center" if case.startswith("cc") else "staggered_x"
+    nx_nodes = n if conv == "cell_center" else n + 2
+    L = (3.0, 2.0, 1.5)
+    sp_, _ = O.spacing_for((nx_nodes, ny, nz), L, conv)
+    g = O.OracleGrid((nx_nodes, ny, nz), sp_, "float64", conv)
+    i = np.arange(n)
+    if case == "cc_neumann":
+        mode_x, lam_x = np.cos(np.pi * m * (i + 0.5) / n), np.sin(np.pi * m / (2 * n)) ** 2
+    elif case == "cc_dirichlet":
+        mode_x, lam_x = np.sin(np.pi * m * (i + 0.5) / n), np.sin(np.pi * m / (2 * n)) ** 2
+    else:  # interior nodes 1..N-1 of N = n + 1 intervals
+        N = n + 1
+        mode_x, lam_x = np.sin(np.pi * m * (i + 1) / N), np.sin(np.pi * m / (2 * N)) ** 2
+    j = np.arange(ny)
+    u0 = (mode_x[:, None, None] * np.cos(2 * np.pi * j / ny)[None, :, None] * np.ones(nz)[None, None, :])
+    lam = -4 / sp_[0] ** 2 * lam_x - 4 / sp_[1] ** 2 * np.sin(np.pi / ny) ** 2
+    kx = np.pi * m / (n * sp_[0] if conv == "cell_center" else (n + 1) * sp_[0])
+    k2 = kx ** 2 + (2 * np.pi / (ny * sp_[1])) ** 2
+    prefac = dt / (1 + dt * D * 0.25 * k2)
+    kind = "neumann" if case == "cc_neumann" else "dirichlet"
+    bc = ((kind, None if kind == "neumann" else (0.0, 0.0)), ("periodic", None), ("periodic", None))
+    mult = O.imex_prefac(dt, O.rd_symbol(O.rfft_k_squared_nonperiodic(g.shape, g.spacing, conv), D))
+    u1 = O.spectral_step_bc(g.field(u0), lambda t, x: O.rd_rhs(x, g, bc, D), mult, kind, conv)
+    want = u0 * (1 + prefac * D * lam)
+    # the multiplier table is float32, as the reference's
+    np.testing.assert_allclose(u1[0].numpy(), want, rtol=0, atol=2e-7 * np.abs(u0).max())
