@@ -328,6 +328,45 @@ def batched_leg(steps, seq_steps, cpu_steps):
     return out
 
 
+def frames_leg(n, steps, every):
+    """Frame output cost: run_cahn_hilliard_solver settings (eps 3, D 1,
+    PseudoSpectralIMEX, dt 0.1, jit) on an n^3 float32 grid for `steps` steps
+    with an output frame every `every` steps, exported synchronously (copy and
+    NaN check at each frame, as the reference loop does) and asynchronously
+    (pinned side-stream copies overlapped with the next steps); the run with
+    one output at each end gives the step cost alone."""
+    import evoxels_amd as evx
+    from evoxels_amd.problem_definition import CahnHilliard
+    from evoxels_amd.solvers import TimeDependentSolver
+    from evoxels_amd.timesteppers import PseudoSpectralIMEX
+    c0 = (0.5 + 0.05 * np.random.default_rng(5).standard_normal((n, n, n))).astype(np.float32)
+
+    def run(frames, async_output):
+        vf = evx.VoxelFields((n, n, n), (n, n, n))
+        vf.add_field("c", c0.copy())
+        s = TimeDependentSolver(vf, "c", backend="hip", problem_cls=CahnHilliard, timestepper_cls=PseudoSpectralIMEX,
+                                device="cuda", async_output=async_output)
+        s.solve(time_increment=0.1, frames=frames, max_iters=steps, problem_kwargs={"eps": 3.0, "D": 1.0},
+                jit=True, verbose=False)
+        return s.computation_time, vf.fields["c"]
+
+    run(1, False)  # plans, graph capture and pinned buffers warm up
+    t_none, _ = run(1, False)
+    t_sync, f_sync = run(steps // every, False)
+    t_async, f_async = run(steps // every, True)
+    nf = steps // every + 1
+    out = {"workload": f"ch_imex_{n}cubed_fp32_frames", "steps": steps, "frames_written": nf,
+           "frame_mb": round(4 * n ** 3 / 2 ** 20, 1),
+           "ms_per_step_no_frames": round(1e3 * t_none / steps, 4),
+           "ms_per_step_sync_frames": round(1e3 * t_sync / steps, 4),
+           "ms_per_step_async_frames": round(1e3 * t_async / steps, 4),
+           "ms_per_frame_sync": round(1e3 * (t_sync - t_none) / (nf - 2), 3),
+           "ms_per_frame_async": round(1e3 * (t_async - t_none) / (nf - 2), 3),
+           "final_frames_equal": bool(np.array_equal(f_sync, f_async))}
+    log(f"[bench] frames: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -340,6 +379,7 @@ def main():
                     help="x-slabs of the semi-implicit step (0: whole-grid sweeps)")
     ap.add_argument("--no-inversion", action="store_true", help="skip the forward+adjoint inversion leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-grid leg")
+    ap.add_argument("--no-frames", action="store_true", help="skip the frame-output leg")
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--inv-steps", type=int, default=200)
     ap.add_argument("--inv-reps", type=int, default=3)
@@ -471,6 +511,13 @@ def main():
         except Exception as e:  # reported in the JSON line; the headline stands on its own
             out["batched"] = {"error": f"{type(e).__name__}: {e}"}
             log(f"[bench] batched leg failed: {e}")
+        torch.cuda.empty_cache()
+    if not args.no_frames:
+        try:
+            out["frames"] = frames_leg(256, 100, 5)
+        except Exception as e:  # reported in the JSON line; the headline stands on its own
+            out["frames"] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] frames leg failed: {e}")
         torch.cuda.empty_cache()
     if not args.no_inversion:
         try:

@@ -11,12 +11,20 @@ for a stepper whose launches do not depend on ``t`` the steps between two
 outputs are captured once and replayed.  The per-frame NaN check is a flag
 fused into the compensated float64 mass reduction (one small read-back
 instead of a full-field ``isnan().any()``).
+
+Frames are exported asynchronously by default (``async_output``): the frame
+is copied on the device, then to pinned host memory on a side stream while
+the next steps run, and handed to ``vf.fields`` (with the NaN abort and the
+VTK file, written by a background thread) when the following frame is
+produced or the run ends.  ``vf.fields``, the files and the abort message are
+those of the synchronous loop; only their timing differs.
 """
 from __future__ import annotations
 
 import sys
 import warnings
 from abc import ABC, abstractmethod
+from collections import deque
 from dataclasses import dataclass
 from timeit import default_timer as timer
 from typing import Any, Callable, Type
@@ -26,6 +34,91 @@ import torch
 
 from .problem_definition import ODE
 from .timesteppers import TimeStepper
+from .vtk_io import AsyncWriter
+
+
+class _Slot:
+    """Device snapshot, pinned host copy and read-back of one output frame."""
+
+    def __init__(self, shape, dtype, channels, device):
+        self.key = (tuple(shape), dtype)
+        self.snap = torch.empty(shape, dtype=dtype, device=device)
+        self.host = torch.empty(shape, dtype=dtype, pin_memory=True)
+        self.flags = torch.zeros(2, dtype=torch.int32, device=device)
+        self.flags_host = torch.empty(2, dtype=torch.int32, pin_memory=True)
+        self.mass_host = torch.empty(channels, dtype=torch.float64, pin_memory=True)
+        self.event = torch.cuda.Event()
+        self.mass = None
+
+
+class _FrameExporter:
+    """Double-buffered asynchronous frame output.  ``submit`` runs on the
+    compute stream only the device-side work (mass + NaN flags, snapshot
+    copy) and queues the device-to-host copies on a side stream of default
+    priority; ``complete`` waits for the oldest frame and performs the host
+    side of solvers.py:152-187 for it."""
+
+    DEPTH = 2
+
+    def __init__(self, solver, device):
+        self.solver = solver
+        self.device = device
+        self.stream = torch.cuda.Stream(device=device)
+        self.free: list[_Slot] = []
+        self.pending: deque = deque()
+        self.writer = None
+
+    def submit(self, u, u_out, frame, time, vtk_out):
+        while len(self.pending) >= self.DEPTH:
+            self.complete()
+        key = (tuple(u_out.shape), u_out.dtype)
+        slot = next((s for s in self.free if s.key == key), None)
+        if slot is None:
+            slot = _Slot(u_out.shape, u_out.dtype, u.shape[0], self.device)
+        else:
+            self.free.remove(slot)
+        cur = torch.cuda.current_stream(self.device)
+        slot.flags.zero_()
+        slot.mass = self.solver.vg.channel_sums(u, slot.flags)
+        slot.snap.copy_(u_out)
+        self.stream.wait_stream(cur)
+        with torch.cuda.stream(self.stream):
+            slot.host.copy_(slot.snap, non_blocking=True)
+            slot.mass_host.copy_(slot.mass, non_blocking=True)
+            slot.flags_host.copy_(slot.flags, non_blocking=True)
+            slot.event.record(self.stream)
+        self.pending.append((slot, frame, time, vtk_out))
+
+    def complete(self):
+        slot, frame, time, vtk_out = self.pending.popleft()
+        slot.event.synchronize()
+        self.solver._export_fields(slot.host.clone())
+        self.solver.mass_history.append((frame, time, slot.mass_host.numpy().copy()))
+        slot.mass = None
+        self.free.append(slot)
+        if int(slot.flags_host[0]):
+            self.drain_files()
+            print(f"NaN detected in frame {frame} at time {time}. Aborting simulation.")
+            sys.exit(1)
+        if vtk_out:
+            vf = self.solver.vf
+            if not hasattr(vf, "export_to_vtk"):
+                raise NotImplementedError("vtk_out needs a VoxelFields with export_to_vtk")
+            if self.writer is None:
+                self.writer = AsyncWriter()
+            names = self.solver.fieldnames
+            self.writer.submit(self.solver._vtk_name(frame), {n: vf.fields[n] for n in names},
+                               vf.spacing, vf.origin)
+
+    def drain_files(self):
+        if self.writer is not None:
+            self.writer.close()
+            self.writer = None
+
+    def flush(self):
+        while self.pending:
+            self.complete()
+        self.drain_files()
 
 
 @dataclass
@@ -38,6 +131,7 @@ class BaseSolver(ABC):
     timestepper_cls: Type[TimeStepper] | None = None
     step_fn: Callable | None = None
     device: str = "cuda"
+    async_output: bool = True
 
     def __post_init__(self):
         if self.backend == "hip":
@@ -83,9 +177,17 @@ class BaseSolver(ABC):
         """solvers.py:80-114."""
         u = self._init_fields()
         step = self._init_stepper(time_increment, problem_kwargs, jit)
+        self._exporter = _FrameExporter(self, self.vg.device) if self.async_output else None
         torch.cuda.synchronize(self.vg.device)
         start = timer()
-        u = self._run_loop(u, step, time_increment, frames, max_iters, vtk_out, verbose, plot_bounds, colormap)
+        try:
+            u = self._run_loop(u, step, time_increment, frames, max_iters, vtk_out, verbose, plot_bounds,
+                               colormap)
+            if self._exporter is not None:
+                self._exporter.flush()
+        finally:
+            if self._exporter is not None:
+                self._exporter.drain_files()
         torch.cuda.synchronize(self.vg.device)
         end = timer()
         self.computation_time = end - start
@@ -97,18 +199,33 @@ class BaseSolver(ABC):
         for i, name in enumerate(self.fieldnames):
             self.vf.set_field(name, host[i])
 
+    def _vtk_name(self, frame):
+        """solvers.py:167-169."""
+        return (self.problem_cls.__name__ if self.problem_cls else "step") + "_" + self.fieldnames[0] + \
+            f"_{frame:03d}.vtk"
+
+    def _output_field(self, u):
+        """trim_ghost_nodes(pad_bc(u)) (solvers.py:154-157): u itself on
+        cell_center grids; on staggered_x grids the boundary nodes come back
+        from the ghost rule."""
+        if self.vg.convention == "cell_center":
+            return u
+        if getattr(self, "problem", None) is not None:
+            return self.vg.bc.trim_ghost_nodes(self.problem.pad_bc(u))
+        return self.vg.bc.trim_ghost_nodes(self.vg.pad_zeros(u))
+
     def _handle_outputs(self, u, frame, time, vtk_out, verbose, plot_bounds, colormap):
-        """solvers.py:152-187.  For cell_center, trim_ghost_nodes(pad_bc(u))
-        is u itself, so u is exported directly; on staggered_x grids the
-        exported field gets its boundary nodes back from the ghost rule."""
+        """solvers.py:152-187 (synchronous form, or queued on the exporter)."""
+        if getattr(self, "_exporter", None) is not None:
+            self._exporter.submit(u, self._output_field(u), frame, time, vtk_out)
+            if verbose:
+                self.profiler.update_memory_stats()
+            if verbose == "plot":
+                warnings.warn("live plotting is not part of the 'hip' backend; frames are exported to vf.fields")
+            return
         flags = torch.zeros(2, dtype=torch.int32, device=u.device)
         mass = self.vg.channel_sums(u, flags)
-        if self.vg.convention == "cell_center":
-            self._export_fields(u)
-        elif getattr(self, "problem", None) is not None:
-            self._export_fields(self.vg.bc.trim_ghost_nodes(self.problem.pad_bc(u)))
-        else:
-            self._export_fields(self.vg.bc.trim_ghost_nodes(self.vg.pad_zeros(u)))
+        self._export_fields(self._output_field(u))
         self.mass_history.append((frame, time, mass.cpu().numpy()))
         if verbose:
             self.profiler.update_memory_stats()
@@ -118,8 +235,7 @@ class BaseSolver(ABC):
         if vtk_out:
             if not hasattr(self.vf, "export_to_vtk"):
                 raise NotImplementedError("vtk_out needs a VoxelFields with export_to_vtk")
-            name = (self.problem_cls.__name__ if self.problem_cls else "step") + "_" + self.fieldnames[0]
-            self.vf.export_to_vtk(filename=name + f"_{frame:03d}.vtk", field_names=self.fieldnames)
+            self.vf.export_to_vtk(filename=self._vtk_name(frame), field_names=self.fieldnames)
         if verbose == "plot":
             warnings.warn("live plotting is not part of the 'hip' backend; frames are exported to vf.fields")
 

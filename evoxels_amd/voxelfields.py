@@ -2,9 +2,9 @@
 
 Geometry and field semantics follow evoxels/voxelfields.py:65-194: spacing and
 origin per convention, ``precision`` string, ``set_field`` shape/type checks,
-``add_field`` zero default, ``set_voxel_sphere``, ``average``, ``axes`` and
-``meshgrid``.  Plotting and VTK export are host I/O outside the time-stepping
-path and are not part of this package; the solvers also accept the
+``add_field`` zero default, ``set_voxel_sphere``, ``average``, ``axes``,
+``meshgrid`` and ``export_to_vtk`` (legacy VTK written directly, no
+pyvista).  Plotting is not part of this package; the solvers also accept the
 reference's own ``VoxelFields`` object (they use only ``grid_info``,
 ``precision``, ``fields``, ``set_field``, ``shape``).
 """
@@ -106,3 +106,11 @@ class VoxelFields:
 
     def meshgrid(self):
         return tuple(np.meshgrid(*self.axes(), indexing="ij"))
+
+    def export_to_vtk(self, filename="output.vtk", field_names=None, binary=True):
+        """voxelfields.py:196-214 without pyvista: the fields as cell data of a
+        legacy-VTK image with N+1 points per axis and its corner at
+        origin - spacing/2."""
+        from .vtk_io import write_structured_points
+        names = field_names if field_names else list(self.fields.keys())
+        write_structured_points(filename, {n: self.fields[n] for n in names}, self.spacing, self.origin, binary)

@@ -126,6 +126,58 @@ def test_nan_aborts_like_reference():
         solver.solve(frames=1, max_iters=2, verbose=False, jit=False)
 
 
+@pytest.mark.parametrize("async_output", [False, True])
+def test_nan_abort_frame_and_fields(capsys, async_output):
+    """The abort names the first frame holding a NaN, and vf.fields holds that
+    frame, with synchronous and asynchronous export alike."""
+    vf = evx.VoxelFields((6, 5, 4))
+    vf.add_field("a", np.ones(vf.shape))
+
+    def step(t, u):
+        return u * float("nan") if t >= 0.35 else u + 1
+    solver = TimeDependentSolver(vf, "a", backend="hip", step_fn=step, device="cuda", async_output=async_output)
+    with pytest.raises(SystemExit):
+        solver.solve(time_increment=0.1, frames=5, max_iters=10, verbose=False, jit=False)
+    # outputs at i = 0, 2, 4, 6, 8: the NaN appears after the step at t = 0.4
+    assert "NaN detected in frame 3 at time 0.6000000000000001" in capsys.readouterr().out
+    assert np.isnan(vf.fields["a"]).all()
+
+
+@pytest.mark.parametrize("jit", [False, True])
+def test_async_output_equals_synchronous(tmp_path, monkeypatch, jit):
+    """Frames, mass history and VTK files of the asynchronous export are those
+    of the synchronous loop, byte for byte."""
+    shape, dt, iters, frames = (32, 24, 40), 0.05, 20, 5
+    c0 = 0.5 + 0.05 * np.random.default_rng(4).standard_normal(shape)
+    results = {}
+    orig = TimeDependentSolver._export_fields
+    for mode in (False, True):
+        vf = evx.VoxelFields(shape, shape)
+        vf.add_field("c", c0.copy())
+        seen = []
+
+        def spy(self, u_out):
+            orig(self, u_out)
+            seen.append(self.vf.fields["c"].copy())
+        monkeypatch.setattr(TimeDependentSolver, "_export_fields", spy)
+        d = tmp_path / f"async_{mode}"
+        d.mkdir()
+        monkeypatch.chdir(d)
+        solver = TimeDependentSolver(vf, "c", backend="hip", problem_cls=CahnHilliard,
+                                     timestepper_cls=PseudoSpectralIMEX, device="cuda", async_output=mode)
+        solver.solve(time_increment=dt, frames=frames, max_iters=iters, problem_kwargs={"eps": 3.0, "D": 1.0},
+                     jit=jit, verbose=False, vtk_out=True)
+        files = {f.name: f.read_bytes() for f in sorted(d.iterdir())}
+        results[mode] = (seen, solver.mass_history, files)
+    (s0, m0, f0), (s1, m1, f1) = results[False], results[True]
+    assert len(s0) == len(s1) == 6
+    assert all(np.array_equal(a, b) for a, b in zip(s0, s1))
+    assert [(f, t) for f, t, _ in m0] == [(f, t) for f, t, _ in m1]
+    assert all(np.array_equal(a, b) for (_, _, a), (_, _, b) in zip(m0, m1))
+    assert sorted(f0) == sorted(f1) == [f"CahnHilliard_c_{k:03d}.vtk" for k in range(6)]
+    assert all(f0[k] == f1[k] for k in f0)
+
+
 def test_inf_does_not_abort():
     """The reference checks isnan only (solvers.py:163)."""
     vf = evx.VoxelFields((4, 4, 4))
