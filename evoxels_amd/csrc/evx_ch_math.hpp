@@ -75,11 +75,15 @@ template <>
 struct Ops<true> {
   EVX_OPS_BODY(_Pragma("clang fp contract(off)"))
 };
-// Contracted flavour for the semi-implicit path.  lap7 and mu are the same
-// expressions (multiply-adds may fuse).  The face flux is returned as
-// s(2-s)*(m_hi-m_lo) with s = c_lo + c_hi, i.e. 4*cf(1-cf)*grad_face(mu)/id:
-// the 1/4 and the face-gradient weight id are folded into the divergence
-// weights, D*id^2/4 per axis, which saves two operations per face.
+// Contracted flavour for the semi-implicit path (multiply-adds may fuse and
+// the algebra is rearranged; the results differ from the strict flavour by
+// roundings only):
+// * mu = (18/eps*c)*p(c) - 2eps*lap(c) with p(c) = (1-c)(1-2c) = 1 - 3c + 2c^2
+//   in Horner form and 2eps folded into the Laplacian weights;
+// * the face flux is returned as s(2-s)*(m_hi-m_lo) = (1 - t^2)*(m_hi-m_lo)
+//   with t = 1 - c_lo - c_hi, i.e. 4*cf(1-cf)*grad_face(mu)/id: the 1/4 and
+//   the face-gradient weight id are folded into the divergence weights,
+//   D*id^2/4 per axis.  flux_q takes q = 1 - c of one side precomputed.
 template <>
 struct Ops<false> {
   template <typename T>
@@ -89,13 +93,20 @@ struct Ops<false> {
   template <typename T>
   static __device__ __forceinline__ T mu(T c, T xp, T xm, T yp, T ym, T zp, T zm, const Geo<T>& g,
                                          const ChCoef<T>& k) {
-    const T lap = lap7(c, xp, xm, yp, ym, zp, zm, g);
-    return ((k.s18 * c) * (T(1) - c)) * (T(1) - T(2) * c) - k.two_eps * lap;
+    const T w = k.two_eps;
+    const T l = (((xp + xm) * (w * g.idx2) + (yp + ym) * (w * g.idy2)) + (zp + zm) * (w * g.idz2)) -
+                c * ((T(2) * w) * g.sum2);
+    const T p = (T(2) * c - T(3)) * c + T(1);
+    return (k.s18 * c) * p - l;
+  }
+  template <typename T>
+  static __device__ __forceinline__ T flux_q(T q_lo, T c_hi, T m_lo, T m_hi) {
+    const T t = q_lo - c_hi;
+    return (T(1) - t * t) * (m_hi - m_lo);
   }
   template <typename T>
   static __device__ __forceinline__ T flux(T c_lo, T c_hi, T m_lo, T m_hi, T /*id*/) {
-    const T s = c_hi + c_lo;
-    return (s * (T(2) - s)) * (m_hi - m_lo);
+    return flux_q(T(1) - c_lo, c_hi, m_lo, m_hi);
   }
   template <typename T>
   static __device__ __forceinline__ T div(T fxm, T fxp, T fym, T fyp, T fzm, T fzp, const Geo<T>& g, T D) {

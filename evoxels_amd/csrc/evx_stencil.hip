@@ -424,6 +424,8 @@ int stencil_rhs_ch_periodic(int32_t dtype, const void* u, void* out, const Dims&
   evx_grid g{d.c, d.nx, d.ny, d.nz, {div_dx[0], div_dx[1], div_dx[2]}, {div_dx2[0], div_dx2[1], div_dx2[2]}};
   Bc b{};
   const Epi ep{EVX_OUT_RHS, 0.0, nullptr, nullptr, nullptr};
+  if (d.ny * d.nz * (dtype == EVX_F32 ? 4 : 8) >= (int64_t(1) << 32))
+    return fail(EVX_ERR_UNSUPPORTED, "a (y,z) plane of more than 4 GiB");
   return dtype == EVX_F32 ? launch_ch<float, false>(u, out, d, b, &g, eps, D, ep, s, chan, xa, xb)
                           : launch_ch<double, false>(u, out, d, b, &g, eps, D, ep, s, chan, xa, xb);
 }

@@ -6,6 +6,8 @@
 #include "evx_common.hpp"
 #include "evx_fft_engine.hpp"
 
+#include <type_traits>
+
 namespace evx {
 
 constexpr int BZ = 64;   // threads (and tile width) along z
@@ -203,6 +205,44 @@ struct PlaneLoader {
   }
 };
 
+// Loader of the fully periodic tiles (the semi-implicit path): every
+// coordinate wraps, so there are no ghost rules.  Source points are kept as
+// 32-bit byte offsets from the plane base, which lets the loads use a scalar
+// base address plus a 32-bit vector offset (no 64-bit address arithmetic per
+// element).  Requires ny*nz*sizeof(T) < 2^32.
+template <typename T, int H, int ROWS, int W, bool CLIP>
+struct PeriodicLoader {
+  static constexpr int NPT = (ROWS * W + NT - 1) / NT;
+  uint32_t boff[NPT];
+  T v[NPT];
+  bool last_ok;  // whether this thread's last element lies in the tile
+
+  __device__ __forceinline__ void init(int64_t y0, int64_t z0, const Dims& d, int tid) {
+    last_ok = tid + (NPT - 1) * NT < ROWS * W;
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int i = min(tid + k * NT, ROWS * W - 1);
+      const int64_t gy = wrap_near(y0 - H + i / W, d.ny), gz = wrap_near(z0 - H + i % W, d.nz);
+      boff[k] = static_cast<uint32_t>((gy * d.nz + gz) * (int64_t)sizeof(T));
+    }
+  }
+
+  __device__ __forceinline__ bool valid(int k) const { return k < NPT - 1 || last_ok; }
+
+  __device__ __forceinline__ void fetch(const T* __restrict__ uc, int64_t xg, const Dims& d) {
+    const char* pl = reinterpret_cast<const char*>(uc + wrap_near(xg, d.nx) * (d.ny * d.nz));
+#pragma unroll
+    for (int k = 0; k < NPT; ++k)
+      if (valid(k)) v[k] = *reinterpret_cast<const T*>(pl + boff[k]);
+  }
+
+  __device__ __forceinline__ void commit(T* flat, int tid) const {
+#pragma unroll
+    for (int k = 0; k < NPT; ++k)
+      if (valid(k)) flat[tid + k * NT] = CLIP ? clip01(v[k]) : v[k];
+  }
+};
+
 // Ring slot of a plane at small relative index rel >= -3 (32-bit, constant modulus).
 template <int NS>
 __device__ __forceinline__ int ring(int64_t rel) { return (int)((unsigned)((int)rel + 3 * NS) % NS); }
@@ -270,7 +310,10 @@ struct ChTile {
 
 // One (y,z) tile x x-chunk of the CH right-hand side.  tid = flat thread index
 // in a 256-thread block, tile = flat tile index (see tile_origin), smem =
-// ChTile<T>::LDS_BYTES of LDS.
+// ChTile<T>::LDS_BYTES of LDS.  The contracted flavour (!STRICT) serves the
+// fully periodic semi-implicit path only: it has no ghost logic, loads
+// through PeriodicLoader and stores the plain right-hand side with 32-bit
+// in-plane offsets.
 template <typename T, bool STRICT>
 __device__ __forceinline__ void ch_rhs_tile(const T* __restrict__ u, T* __restrict__ out, const Dims& d,
                                             const Bc& bc, const Geo<T>& g, ChCoef<T> k, const Epi& ep,
@@ -294,8 +337,9 @@ __device__ __forceinline__ void ch_rhs_tile(const T* __restrict__ u, T* __restri
     const double e = chan[2 * o.ch], dd = chan[2 * o.ch + 1];
     k = ChCoef<T>{static_cast<T>(18.0 / e), static_cast<T>(2.0 * e), static_cast<T>(dd)};
   }
-  const bool xper = bc.kind[0] == EVX_BC_PERIODIC;
-  const bool yper = bc.kind[1] == EVX_BC_PERIODIC, zper = bc.kind[2] == EVX_BC_PERIODIC;
+  constexpr bool PER = !STRICT;
+  const bool xper = PER || bc.kind[0] == EVX_BC_PERIODIC;
+  const bool yper = PER || bc.kind[1] == EVX_BC_PERIODIC, zper = PER || bc.kind[2] == EVX_BC_PERIODIC;
   const int r0 = ty * PY;
   const int64_t gz = z0 + tz;
 
@@ -396,17 +440,31 @@ __device__ __forceinline__ void ch_rhs_tile(const T* __restrict__ u, T* __restri
     }
   };
 
-  PlaneLoader<T, 2, CR, CW, true> pl;
-  pl.init(y0, z0, d, bc, tid);
+  using Loader = typename std::conditional<PER, PeriodicLoader<T, 2, CR, CW, true>,
+                                           PlaneLoader<T, 2, CR, CW, true>>::type;
+  Loader pl;
+  auto pl_init = [&]() {
+    if constexpr (PER) pl.init(y0, z0, d, tid);
+    else pl.init(y0, z0, d, bc, tid);
+  };
+  auto pl_fetch = [&](int64_t xg) {
+    if constexpr (PER) pl.fetch(uc, xg, d);
+    else pl.fetch(uc, xg, d, bc);
+  };
+  auto pl_commit = [&](int slot) {
+    if constexpr (PER) pl.commit(&cs[slot][0][0], tid);
+    else pl.commit(&cs[slot][0][0], bc, tid);
+  };
+  pl_init();
   const int b = tz + 2, bm = tz + 1;
   // Warm-up: planes x0-2 .. x0+1, mu(x0-1) and mu(x0), and the flux through
   // the x face x0-1/2 (afterwards carried in registers, so neither c(x-1)
   // nor mu(x-1) is read again and the rings hold 4 c planes and 2 mu planes).
   for (int64_t xg = x0 - 2; xg <= x0 + 1; ++xg) {
-    pl.fetch(uc, xg, d, bc);
-    pl.commit(&cs[ring<NC>(xg - x0 + 2)][0][0], bc, tid);
+    pl_fetch(xg);
+    pl_commit(ring<NC>(xg - x0 + 2));
   }
-  if (x0 + 2 <= x1 + 1) pl.fetch(uc, x0 + 2, d, bc);
+  if (x0 + 2 <= x1 + 1) pl_fetch(x0 + 2);
   __syncthreads();
   compute_mu(x0 - 1);
   compute_mu(x0);
@@ -422,10 +480,16 @@ __device__ __forceinline__ void ch_rhs_tile(const T* __restrict__ u, T* __restri
       carry[q] = Op::flux(Cm[r0 + q + 2][b], C0[r0 + q + 2][b], Mm[r0 + q + 1][bm], M0[r0 + q + 1][bm], g.idx);
   }
   // plane x0+2 goes to the slot of x0-2 (last read by mu(x0-1))
-  if (x0 + 2 <= x1 + 1) pl.commit(&cs[ring<NC>(4)][0][0], bc, tid);
-  if (x0 + 3 <= x1 + 1) pl.fetch(uc, x0 + 3, d, bc);
+  if (x0 + 2 <= x1 + 1) pl_commit(ring<NC>(4));
+  if (x0 + 3 <= x1 + 1) pl_fetch(x0 + 3);
   __syncthreads();  // mu(x0+1) below reuses the slot of mu(x0-1) read above
   const int64_t plane = d.ny * d.nz;
+  // plain right-hand-side stores of the periodic flavour: 32-bit byte offset
+  // of this thread's first row in the plane, rows nz apart
+  const bool full_tile = y0 + BY <= d.ny && z0 + BZ <= d.nz;
+  const bool rhs_only = PER && ep.mode == EVX_OUT_RHS;
+  const uint32_t obyte0 = static_cast<uint32_t>(((y0 + r0) * d.nz + gz) * (int64_t)sizeof(T));
+  const uint32_t orow_bytes = static_cast<uint32_t>(d.nz * (int64_t)sizeof(T));
   for (int64_t x = x0; x < x1; ++x) {
     // mu(x+1) from c(x), c(x+1), c(x+2), into the slot of mu(x-1)
     compute_mu(x + 1);
@@ -434,30 +498,50 @@ __device__ __forceinline__ void ch_rhs_tile(const T* __restrict__ u, T* __restri
     const T(*Cp)[CW] = cs[ring<NC>(x + 1 - x0 + 2)];
     const T(*M0)[MW] = ms[ring<NM>(x - x0 + 1)];
     const T(*Mp)[MW] = ms[ring<NM>(x + 1 - x0 + 1)];
-    T cc[PY + 2], mc[PY + 2], fy[PY + 1];
+    T cc[PY + 2], mc[PY + 2], fy[PY + 1], qc[PY + 1];
 #pragma unroll
     for (int j = 0; j < PY + 2; ++j) {
       cc[j] = C0[r0 + j + 1][b];
       mc[j] = M0[r0 + j][bm];
     }
+    if constexpr (PER) {  // q = 1 - c of each own point, shared by its faces
 #pragma unroll
-    for (int j = 0; j <= PY; ++j) fy[j] = Op::flux(cc[j], cc[j + 1], mc[j], mc[j + 1], g.idy);
+      for (int j = 0; j <= PY; ++j) qc[j] = T(1) - cc[j];
+    }
+#pragma unroll
+    for (int j = 0; j <= PY; ++j) {
+      if constexpr (PER) fy[j] = Op::flux_q(qc[j], cc[j + 1], mc[j], mc[j + 1]);
+      else fy[j] = Op::flux(cc[j], cc[j + 1], mc[j], mc[j + 1], g.idy);
+    }
 #pragma unroll
     for (int q = 0; q < PY; ++q) {
       const int a = r0 + q + 2, am = r0 + q + 1;
       const T c0 = cc[q + 1], m0 = mc[q + 1];
-      const T fxp = Op::flux(c0, Cp[a][b], m0, Mp[am][bm], g.idx);
-      const T fzm = Op::flux(C0[a][b - 1], c0, M0[am][bm - 1], m0, g.idz);
-      const T fzp = Op::flux(c0, C0[a][b + 1], m0, M0[am][bm + 1], g.idz);
+      T fxp, fzm, fzp;
+      if constexpr (PER) {
+        const T q0 = qc[q + 1];
+        fxp = Op::flux_q(q0, Cp[a][b], m0, Mp[am][bm]);
+        fzm = Op::flux_q(q0, C0[a][b - 1], M0[am][bm - 1], m0);
+        fzp = Op::flux_q(q0, C0[a][b + 1], m0, M0[am][bm + 1]);
+      } else {
+        fxp = Op::flux(c0, Cp[a][b], m0, Mp[am][bm], g.idx);
+        fzm = Op::flux(C0[a][b - 1], c0, M0[am][bm - 1], m0, g.idz);
+        fzp = Op::flux(c0, C0[a][b + 1], m0, M0[am][bm + 1], g.idz);
+      }
       const T r = Op::div(carry[q], fxp, fy[q], fy[q + 1], fzm, fzp, g, k.D);
       carry[q] = fxp;
       const int64_t gy = y0 + r0 + q;
-      if (gy < d.ny && gz < d.nz) epilogue(ep, uc, oc, x * plane + gy * d.nz + gz, r);
+      if (rhs_only) {
+        if (full_tile || (gy < d.ny && gz < d.nz))
+          *reinterpret_cast<T*>(reinterpret_cast<char*>(oc + x * plane) + (obyte0 + q * orow_bytes)) = r;
+      } else if (gy < d.ny && gz < d.nz) {
+        epilogue(ep, uc, oc, x * plane + gy * d.nz + gz, r);
+      }
     }
     // plane x+3 (fetched one iteration ago) goes to the slot of c(x-1), which
     // neither the output above nor mu(x+1) reads; it feeds mu(x+2)
-    if (x + 3 <= x1 + 1) pl.commit(&cs[ring<NC>(x + 3 - x0 + 2)][0][0], bc, tid);
-    if (x + 4 <= x1 + 1) pl.fetch(uc, x + 4, d, bc);
+    if (x + 3 <= x1 + 1) pl_commit(ring<NC>(x + 3 - x0 + 2));
+    if (x + 4 <= x1 + 1) pl_fetch(x + 4);
     __syncthreads();
   }
 }
