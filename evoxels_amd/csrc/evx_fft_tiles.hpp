@@ -273,11 +273,11 @@ constexpr int pick_strided_e(int n, int l, int tl0) {
 template <typename T, int N, int MODE = 0>
 struct SGeo {
   static constexpr int TL0 = 128 / (int)sizeof(cpx<T>);
-  // The forward*prefac*inverse sweeps hold two transforms' worth of live
-  // values: from 512 points up they use half the values per thread (twice
-  // the threads per line) to stay off the register limit.
+  // From 512 points up every sweep uses half the values per thread (twice
+  // the threads per line): the forward*prefac*inverse sweeps stay off the
+  // register limit, and the one-way sweeps reach eight waves per SIMD.
   static constexpr int E0 = pick_strided_e(N, FftSpec<N>::L, TL0);
-  static constexpr int E = (MODE >= 2 && N >= 512 && E0 >= 2 * FftSpec<N>::L) ? E0 / 2 : E0;
+  static constexpr int E = (N >= 512 && E0 >= 2 * FftSpec<N>::L) ? E0 / 2 : E0;
   static constexpr int TPL = N / E;
   static constexpr int TL = TL0 > 256 / TPL ? TL0 : 256 / TPL;
   static constexpr int THREADS = TL * TPL;
