@@ -328,6 +328,202 @@ def batched_leg(steps, seq_steps, cpu_steps):
     return out
 
 
+def _time_steps(fn, steps, reps=1):
+    """Mean ms per call of fn over `steps` calls (after one warm-up call),
+    bracketed by events on the current stream."""
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = None
+    for _ in range(reps):
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        best = ms if best is None else min(best, ms)
+    return best
+
+
+def copy_ceiling():
+    """Measured copy ceiling: torch copy_ of a 1 GiB float32 buffer, read +
+    write bytes over the kernel time."""
+    a = torch.rand(2 ** 28, device="cuda")
+    b = torch.empty_like(a)
+    ms = _time_steps(lambda: b.copy_(a), 10, reps=3)
+    return {"bytes_per_copy": 2 * 4 * 2 ** 28, "ms": round(ms, 4), "gbps": round(2 * 4 * 2 ** 28 / (ms * 1e-3) / 1e9, 1)}
+
+
+def configs_leg(cpu):
+    """The other workloads of the scope: config 1 (100^3 FP64 CH semi-implicit,
+    dt 0.1, 1000 steps, graph-replayed, parity spot-checked against the FP64
+    oracle), config 2 (256^3 FP32 RD forward Euler with bc (Neumann,
+    Dirichlet(0,1), periodic), dt = 0.1*dx^2/6, 2000 steps, plus the
+    Allen-Cahn variant), and one line per next-row path: ETD1 at 512^3, the
+    staggered_x Dirichlet RD step, the fused two-species step, the
+    non-periodic (Neumann x) semi-implicit step and the generic multiphase
+    and smoothed-boundary right-hand sides.  Bytes are algorithmic: 8 B per
+    voxel for one-field explicit steps, 16 B for two species."""
+    import warnings
+    import evoxels_amd as evx
+    from evoxels_amd.problem_definition import (CahnHilliard, CoupledReactionDiffusion, MultiPhaseAllenCahn,
+                                                ReactionDiffusion, ReactionDiffusionSBM, TwoPhaseAllenCahn)
+    from evoxels_amd.timesteppers import ExponentialEuler, ForwardEuler, PseudoSpectralIMEX
+    from evoxels_amd.voxelgrid import VoxelGridHip
+    sys.path.insert(0, str(ROOT))
+    from oracle import evoxels_oracle as O
+    out = {}
+    warnings.simplefilter("ignore")
+
+    def grid(shape, precision, domain=None, convention="cell_center"):
+        vf = evx.VoxelFields(shape, domain or shape, convention=convention)
+        vf.precision = precision
+        return vf, VoxelGridHip(vf.grid_info(), precision=precision, device="cuda")
+
+    # config 1: 100^3 FP64 CH IMEX, seed 0
+    n = 100
+    vf, vg = grid((n, n, n), "float64")
+    c0 = 0.5 + 0.1 * np.random.default_rng(0).random((n, n, n))
+    st = PseudoSpectralIMEX(CahnHilliard(vg, eps=3.0, D=1.0), 0.1)
+    u = vg.init_scalar_field(c0).contiguous()
+    m0 = float(vg.channel_sums(u)[0])
+    st.step_into(u, u)  # plan and workspace
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    ug = u.clone()
+    with torch.cuda.graph(g, stream=side):
+        for _ in range(100):
+            st.step_into(ug, ug)
+    u1 = vg.init_scalar_field(c0).contiguous()
+    ug.copy_(u1)
+    ms = _time_steps(lambda: g.replay(), 10) / 100  # 1000 steps
+    m1 = float(vg.channel_sums(ug)[0])
+    # parity spot-check: 3 steps from c0 against the FP64 oracle
+    chk = vg.init_scalar_field(c0).contiguous()
+    og = O.OracleGrid((n, n, n), vf.spacing, "float64")
+    pre = O.imex_prefac(0.1, O.ch_symbol(O.rfft_k_squared((n, n, n), og.spacing), 3.0, 1.0))
+    ref = og.field(c0)
+    P3 = (("periodic", None),) * 3
+    worst = 0.0
+    for _ in range(3):
+        st.step_into(chk, chk)
+        ref = O.imex_step(ref, lambda t, x: O.ch_rhs(x, og, P3, 3.0, 1.0), pre)
+        got = chk.cpu().numpy()
+        worst = max(worst, float(np.linalg.norm(got - ref.numpy()) / np.linalg.norm(ref.numpy())))
+    out["config1_ch_imex_100cubed_fp64"] = {
+        "steps": 1000, "ms_per_step": round(ms, 4), "voxel_steps_per_s": n ** 3 / (ms * 1e-3),
+        "alg_gbps": round(89.28 * n ** 3 / (ms * 1e-3) / 1e9, 1), "mass_rel_drift": abs(m1 - m0) / abs(m0),
+        "rel_l2_vs_fp64_oracle_3_steps": worst,
+        "note": "0.8 GB working set: resident in the 256 MB last-level cache only in part; not an HBM fraction"}
+    if cpu:
+        t0 = time.perf_counter()
+        r = og.field(c0)
+        for _ in range(5):
+            r = O.imex_step(r, lambda t, x: O.ch_rhs(x, og, P3, 3.0, 1.0), pre)
+        dt = time.perf_counter() - t0
+        out["config1_ch_imex_100cubed_fp64"]["cpu_baseline"] = {
+            "value": 5 * n ** 3 / dt, "unit": "voxel-steps/s", "cores": torch.get_num_threads(), "kind": "port",
+            "sample": "5 steps of the CPU oracle"}
+    del ug, u, u1, chk, g
+    log(f"[bench] config 1: {out['config1_ch_imex_100cubed_fp64']}")
+
+    # config 2: 256^3 FP32 RD forward Euler, bc (N, D(0,1), P), seed 1; AC variant
+    n = 256
+    vf, vg = grid((n, n, n), "float32", (1, 1, 1))
+    phi0 = np.random.default_rng(1).random((n, n, n))
+    dt = 0.1 * vf.spacing[0] ** 2 / 6
+    bc = ("neumann", ("dirichlet", (0.0, 1.0)), "periodic")
+    for name, prob in [("rd", ReactionDiffusion(vg, D=1.0, bc=bc)),
+                       ("ac", TwoPhaseAllenCahn(vg, eps=2.0, curvature=0.5, bc=bc))]:
+        fe = ForwardEuler(prob, dt)
+        state = {"u": vg.init_scalar_field(phi0).contiguous()}
+
+        def one():
+            state["u"] = fe.step(0.0, state["u"])
+        ms = _time_steps(one, 2000)
+        out[f"config2_{name}_fe_256cubed_fp32"] = {
+            "steps": 2000, "ms_per_step": round(ms, 5), "voxel_steps_per_s": n ** 3 / (ms * 1e-3),
+            "gbps_at_8B": round(8 * n ** 3 / (ms * 1e-3) / 1e9, 1),
+            "note": "134 MB per step: fits the 256 MB last-level cache, so GB/s can exceed HBM peak"}
+        if cpu and name == "rd":
+            og = O.OracleGrid((n, n, n), vf.spacing, "float32")
+            bcn = O.normalize_bc(bc)
+            r = og.field(phi0)
+            t0 = time.perf_counter()
+            for _ in range(3):
+                r = O.fe_step(r, lambda t, x: O.rd_rhs(x, og, bcn, 1.0), dt)
+            tc = time.perf_counter() - t0
+            out[f"config2_{name}_fe_256cubed_fp32"]["cpu_baseline"] = {
+                "value": 3 * n ** 3 / tc, "unit": "voxel-steps/s", "cores": torch.get_num_threads(), "kind": "port",
+                "sample": "3 forward-Euler steps of the CPU oracle"}
+        log(f"[bench] config 2 {name}: {out[f'config2_{name}_fe_256cubed_fp32']}")
+        del state
+
+    # next rows
+    n = 512
+    vf, vg = grid((n, n, n), "float32")
+    c0 = 0.5 + 0.05 * np.random.default_rng(2).standard_normal((n, n, n))
+    etd = ExponentialEuler(CahnHilliard(vg, eps=3.0, D=1.0), 0.05)
+    u = vg.init_scalar_field(c0).contiguous()
+    ms = _time_steps(lambda: etd.step_into(u, u), 20)
+    out["f2_ch_etd1_512cubed_fp32"] = {"ms_per_step": round(ms, 4), "voxel_steps_per_s": n ** 3 / (ms * 1e-3),
+                                      "step_roofline_frac": round(44.125 * n ** 3 / (ms * 1e-3) / 1e9 / PEAK_GBPS, 4)}
+    del u, etd
+    torch.cuda.empty_cache()
+    n = 256
+    vf, vg = grid((n + 1, n, n), "float32", (1, 1, 1), "staggered_x")
+    rd = ReactionDiffusion(vg, D=1.0, bc=(("dirichlet", (0.0, 1.0)), "periodic", "periodic"))
+    fe = ForwardEuler(rd, 0.1 * vf.spacing[0] ** 2 / 6)
+    state = {"u": torch.rand((1, n - 1, n, n), device="cuda")}
+
+    def one_s():
+        state["u"] = fe.step(0.0, state["u"])
+    ms = _time_steps(one_s, 500)
+    vox = (n - 1) * n * n
+    out["f1_staggered_rd_fe_256cubed_fp32"] = {"ms_per_step": round(ms, 5), "voxel_steps_per_s": vox / (ms * 1e-3),
+                                              "gbps_at_8B": round(8 * vox / (ms * 1e-3) / 1e9, 1)}
+    vf, vg = grid((n, n, n), "float32", (1, 1, 1))
+    imex = PseudoSpectralIMEX(ReactionDiffusion(vg, D=1.0, bc=("neumann", "periodic", "periodic")), 1e-4)
+    state = {"u": torch.rand((1, n, n, n), device="cuda")}
+
+    def one_n():
+        state["u"] = imex.step(0.0, state["u"])
+    ms = _time_steps(one_n, 50)
+    out["f1_rd_imex_neumann_x_256cubed_fp32"] = {
+        "ms_per_step": round(ms, 4), "voxel_steps_per_s": n ** 3 / (ms * 1e-3),
+        "note": "generic path: fused RHS, device mirror extension to 512 x 256 x 256, HIP FFTs, torch multiply"}
+    gs = CoupledReactionDiffusion(vg)
+    fe = ForwardEuler(gs, 0.1)
+    state = {"u": torch.rand((2, n, n, n), device="cuda")}
+
+    def one_g():
+        state["u"] = fe.step(0.0, state["u"])
+    ms = _time_steps(one_g, 500)
+    out["f4_gray_scott_fe_256cubed_fp32"] = {"ms_per_step": round(ms, 5), "voxel_steps_per_s": n ** 3 / (ms * 1e-3),
+                                            "gbps_at_16B": round(16 * n ** 3 / (ms * 1e-3) / 1e9, 1)}
+    n = 128
+    vf, vg = grid((n, n, n), "float32")
+    mp = MultiPhaseAllenCahn(vg, bc=("neumann", "periodic", "periodic"))
+    ph = torch.rand((3, n, n, n), device="cuda")
+    ms = _time_steps(lambda: mp.rhs(0.0, ph), 50)
+    out["f4_multiphase_ac_rhs_3phases_128cubed_fp32"] = {"ms_per_call": round(ms, 4),
+                                                         "voxel_calls_per_s": n ** 3 / (ms * 1e-3),
+                                                         "note": "generic path"}
+    x, y, z = np.meshgrid(*[np.arange(n) / n] * 3, indexing="ij")
+    sbm = ReactionDiffusionSBM(vg, D=1.0, mask=0.5 + 0.3 * np.cos(4 * np.pi * x) * np.cos(2 * np.pi * y),
+                               bc=(("dirichlet", (0.0, 1.0)), "periodic", "periodic"), bc_flux=1.0)
+    us = torch.rand((1, n, n, n), device="cuda")
+    ms = _time_steps(lambda: sbm.rhs(0.0, us), 50)
+    out["f4_smoothed_boundary_rhs_128cubed_fp32"] = {"ms_per_call": round(ms, 4),
+                                                     "voxel_calls_per_s": n ** 3 / (ms * 1e-3),
+                                                     "note": "generic path"}
+    log(f"[bench] next rows: { {k: v for k, v in out.items() if k[:1] == 'f'} }")
+    return out
+
+
 def frames_leg(n, steps, every):
     """Frame output cost: run_cahn_hilliard_solver settings (eps 3, D 1,
     PseudoSpectralIMEX, dt 0.1, jit) on an n^3 float32 grid for `steps` steps
@@ -380,6 +576,8 @@ def main():
     ap.add_argument("--no-inversion", action="store_true", help="skip the forward+adjoint inversion leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-grid leg")
     ap.add_argument("--no-frames", action="store_true", help="skip the frame-output leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip configs 1-2, the next-row paths and the copy ceiling")
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--inv-steps", type=int, default=200)
     ap.add_argument("--inv-reps", type=int, default=3)
@@ -511,6 +709,15 @@ def main():
         except Exception as e:  # reported in the JSON line; the headline stands on its own
             out["batched"] = {"error": f"{type(e).__name__}: {e}"}
             log(f"[bench] batched leg failed: {e}")
+        torch.cuda.empty_cache()
+    if not args.no_configs:
+        try:
+            out["copy_ceiling"] = copy_ceiling()
+            log(f"[bench] copy ceiling: {out['copy_ceiling']}")
+            out["configs"] = configs_leg(not args.no_cpu_baseline)
+        except Exception as e:  # reported in the JSON line; the headline stands on its own
+            out["configs"] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] configs leg failed: {e}")
         torch.cuda.empty_cache()
     if not args.no_frames:
         try:
