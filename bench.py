@@ -443,7 +443,7 @@ def configs_leg(cpu):
 
         def one():
             state["u"] = fe.step(0.0, state["u"])
-        ms = _time_steps(one, 2000)
+        ms = _time_steps(one, 2000, reps=2)
         out[f"config2_{name}_fe_256cubed_fp32"] = {
             "steps": 2000, "ms_per_step": round(ms, 5), "voxel_steps_per_s": n ** 3 / (ms * 1e-3),
             "gbps_at_8B": round(8 * n ** 3 / (ms * 1e-3) / 1e9, 1),
@@ -481,7 +481,7 @@ def configs_leg(cpu):
 
     def one_s():
         state["u"] = fe.step(0.0, state["u"])
-    ms = _time_steps(one_s, 500)
+    ms = _time_steps(one_s, 500, reps=3)
     vox = (n - 1) * n * n
     out["f1_staggered_rd_fe_256cubed_fp32"] = {"ms_per_step": round(ms, 5), "voxel_steps_per_s": vox / (ms * 1e-3),
                                               "gbps_at_8B": round(8 * vox / (ms * 1e-3) / 1e9, 1)}
@@ -491,7 +491,7 @@ def configs_leg(cpu):
 
     def one_n():
         state["u"] = imex.step(0.0, state["u"])
-    ms = _time_steps(one_n, 50)
+    ms = _time_steps(one_n, 50, reps=2)
     out["f1_rd_imex_neumann_x_256cubed_fp32"] = {
         "ms_per_step": round(ms, 4), "voxel_steps_per_s": n ** 3 / (ms * 1e-3),
         "note": "generic path: fused RHS, device mirror extension to 512 x 256 x 256, HIP FFTs, torch multiply"}
@@ -501,7 +501,7 @@ def configs_leg(cpu):
 
     def one_g():
         state["u"] = fe.step(0.0, state["u"])
-    ms = _time_steps(one_g, 500)
+    ms = _time_steps(one_g, 500, reps=3)
     out["f4_gray_scott_fe_256cubed_fp32"] = {"ms_per_step": round(ms, 5), "voxel_steps_per_s": n ** 3 / (ms * 1e-3),
                                             "gbps_at_16B": round(16 * n ** 3 / (ms * 1e-3) / 1e9, 1)}
     n = 128
