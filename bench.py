@@ -81,8 +81,16 @@ def cpu_baseline(c0, steps):
     for _ in range(steps):
         u = O.imex_step(u, rhs, pre)
     dt = time.perf_counter() - t0
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": n ** 3 * steps / dt, "unit": "voxel-steps/s", "cores": torch.get_num_threads(),
-            "kind": "port", "s_per_step": dt / steps,
+            "kind": "port", "s_per_step": dt / steps, "cpu_model": model, "os_cpu_count": os.cpu_count(),
             "sample": f"{steps} timed steps (after 1 warm-up) of the same 512^3 float32 workload, "
                       f"torch {torch.__version__} eager on CPU, os.cpu_count()={os.cpu_count()}"}
 
