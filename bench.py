@@ -519,7 +519,7 @@ def configs_leg(cpu):
     ms = _time_steps(lambda: mp.rhs(0.0, ph), 50)
     out["f4_multiphase_ac_rhs_3phases_128cubed_fp32"] = {"ms_per_call": round(ms, 4),
                                                          "voxel_calls_per_s": n ** 3 / (ms * 1e-3),
-                                                         "note": "generic path"}
+                                                         "note": "projection kernel, ghost pad, padded RHS kernel"}
     x, y, z = np.meshgrid(*[np.arange(n) / n] * 3, indexing="ij")
     sbm = ReactionDiffusionSBM(vg, D=1.0, mask=0.5 + 0.3 * np.cos(4 * np.pi * x) * np.cos(2 * np.pi * y),
                                bc=(("dirichlet", (0.0, 1.0)), "periodic", "periodic"), bc_flux=1.0)
@@ -527,7 +527,7 @@ def configs_leg(cpu):
     ms = _time_steps(lambda: sbm.rhs(0.0, us), 50)
     out["f4_smoothed_boundary_rhs_128cubed_fp32"] = {"ms_per_call": round(ms, 4),
                                                      "voxel_calls_per_s": n ** 3 / (ms * 1e-3),
-                                                     "note": "generic path"}
+                                                     "note": "ghost pad (mask-scaled node values) and one RHS kernel"}
     log(f"[bench] next rows: { {k: v for k, v in out.items() if k[:1] == 'f'} }")
     return out
 

@@ -72,6 +72,9 @@ SIGNATURES = {
     "evx_ac_rhs": [_I32, _P, _P, C.POINTER(EvxGrid), C.POINTER(EvxBC), _D, _D, _D, _D, _D,
                    C.POINTER(EvxEpilogue), _P],
     "evx_axpy": [_I32, _P, _P, _D, _P, _I64, _P],
+    "evx_simplex_project": [_I32, _P, _P, _I32, _I64, _I32, _P],
+    "evx_multiphase_ac_rhs_padded": [_I32, _P, _P, _P, C.POINTER(EvxGrid), _D, _D, _D, _D, _P],
+    "evx_sbm_rhs_padded": [_I32, _P, _P, _P, _D, _P, C.POINTER(EvxGrid), _D, _D, _P],
     "evx_fft_plan_create": [C.POINTER(_P), _I32, _I64, _I64, _I64, _I64],
     "evx_fft_plan_destroy": [_P],
     "evx_fft_plan_workspace_bytes": [_P, C.POINTER(C.c_size_t)],

@@ -292,6 +292,141 @@ __global__ void padded_stencil_kernel(const T* __restrict__ p, T* __restrict__ o
   }
 }
 
+// ---------------------------------------------------------------------------
+// Multiphase Allen-Cahn (problem_definition.py:508-573) in three launches:
+// simplex projection per voxel, the ghost pad of the projected phases, and the
+// right-hand side from the padded phases.  Expressions follow the reference's
+// torch order with contraction off.
+// ---------------------------------------------------------------------------
+constexpr int MAX_PHASES = 8;
+
+template <typename T>
+__device__ __forceinline__ T clip_min0(T v) { return v != v ? v : (v < T(0) ? T(0) : v); }
+
+// Sloppy projection clip(phis, 0) / sum (problem_definition.py:508-512) or the
+// Euclidean one through sort / cumsum (problem_definition.py:514-530).
+template <typename T, bool EUCLID>
+__global__ void simplex_project_kernel(const T* __restrict__ in, T* __restrict__ out, int nph, int64_t vol) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < vol; i += (int64_t)gridDim.x * blockDim.x) {
+    T v[MAX_PHASES];
+    for (int p = 0; p < nph; ++p) v[p] = in[p * vol + i];
+    if (!EUCLID) {
+      T sum = T(0);
+      for (int p = 0; p < nph; ++p) {
+        v[p] = clip_min0(v[p]);
+        sum = p == 0 ? v[p] : sum + v[p];
+      }
+      for (int p = 0; p < nph; ++p) out[p * vol + i] = v[p] / sum;
+    } else {
+      T u[MAX_PHASES];
+      for (int p = 0; p < nph; ++p) u[p] = v[p];
+      for (int a = 1; a < nph; ++a)  // insertion sort, descending
+        for (int b = a; b > 0 && u[b - 1] < u[b]; --b) {
+          const T tmp = u[b];
+          u[b] = u[b - 1];
+          u[b - 1] = tmp;
+        }
+      T csum = T(0), theta_num = T(0);
+      int active = 0;
+      T cssv[MAX_PHASES];
+      for (int p = 0; p < nph; ++p) {
+        csum = p == 0 ? u[0] : csum + u[p];
+        cssv[p] = csum - T(1);
+        if (u[p] - cssv[p] / static_cast<T>(p + 1) > T(0)) ++active;
+      }
+      if (active < 1) active = 1;
+      theta_num = cssv[active - 1];
+      const T theta = theta_num / static_cast<T>(active);
+      for (int p = 0; p < nph; ++p) out[p * vol + i] = clip_min0(v[p] - theta);
+    }
+  }
+}
+
+template <typename T>
+struct MpCoef {
+  T neg_curv;     // -curvature
+  T one_m_curv;   // 1 - curvature
+  T pot_factor;   // 9/(2 eps^2)
+  T neg_mgab;     // -M*gab
+  T n_phases;     // the mean over phases divides by it
+  bool normal;    // curvature != 1: the normal-Laplacian term is evaluated
+};
+
+template <typename T>
+__global__ void multiphase_rhs_kernel(const T* __restrict__ phis, const T* __restrict__ pad, T* __restrict__ out,
+                                      Dims d, Geo<T> g, MpCoef<T> k) {
+  const int nph = (int)d.c;
+  const int64_t py = d.ny + 2, pz = d.nz + 2, pplane = py * pz, pvol = (d.nx + 2) * pplane, vol = d.vol();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < vol; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t z = i % d.nz, t = i / d.nz, y = t % d.ny, x = t / d.ny;
+    const int64_t c0 = (x + 1) * pplane + (y + 1) * pz + (z + 1);
+    T ph[MAX_PHASES], df[MAX_PHASES];
+    T sum_sq = T(0);
+    for (int p = 0; p < nph; ++p) {
+      ph[p] = phis[p * vol + i];
+      const T sq = ph[p] * ph[p];
+      sum_sq = p == 0 ? sq : sum_sq + sq;
+    }
+    T dsum = T(0);
+    for (int p = 0; p < nph; ++p) {
+      const T* q = pad + p * pvol + c0;
+      auto at = [&](int dx, int dy, int dz) { return q[dx * pplane + dy * pz + dz]; };
+      T grad = k.neg_curv * lap7(at(0, 0, 0), at(1, 0, 0), at(-1, 0, 0), at(0, 1, 0), at(0, -1, 0), at(0, 0, 1),
+                                 at(0, 0, -1), g);
+      if (k.normal) {
+        T Pm[3][3], P0[3][3], Pp[3][3];
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            Pm[a][b] = at(-1, a - 1, b - 1);
+            P0[a][b] = at(0, a - 1, b - 1);
+            Pp[a][b] = at(1, a - 1, b - 1);
+          }
+        grad = grad - k.one_m_curv * normal_lap_at<T, 3>(Pm, P0, Pp, 1, 1, g);
+      }
+      const T phi = ph[p];
+      const T pot = k.pot_factor * ((((T(3) * phi) * (sum_sq - phi * phi)) + (phi * phi) * phi) - phi);
+      df[p] = grad + pot;
+      dsum = p == 0 ? df[0] : dsum + df[p];
+    }
+    const T mean = dsum / k.n_phases;
+    for (int p = 0; p < nph; ++p) out[p * vol + i] = k.neg_mgab * (df[p] - mean);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Smoothed-boundary reaction-diffusion (problem_definition.py:273-287) from
+// the padded field z and padded mask m:
+//   r = D * sum_a grad_a( grad_a(z) - to_face_a(z/m) * grad_a(m) ) + norm*bc_flux
+// with face operators of fd_stencils.py:20-42, in the reference's order.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void sbm_rhs_kernel(const T* __restrict__ zp, const T* __restrict__ mp, const T* __restrict__ norm,
+                               T norm_scalar, T* __restrict__ out, Dims d, Geo<T> g, T D, T bc_flux) {
+  const int64_t py = d.ny + 2, pz = d.nz + 2, pplane = py * pz, pvol = (d.nx + 2) * pplane, vol = d.vol();
+  const int64_t n = d.c * vol;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ch = i / vol, v = i - ch * vol;
+    const int64_t z = v % d.nz, t = v / d.nz, y = t % d.ny, x = t / d.ny;
+    const int64_t c0 = (x + 1) * pplane + (y + 1) * pz + (z + 1);
+    const T* zc = zp + ch * pvol + c0;
+    const T* mc = mp + c0;
+    // face flux between padded points lo and lo + s along an axis with weight id
+    auto face = [&](int64_t lo, int64_t st, T id) {
+      const T zl = zc[lo], zh = zc[lo + st], ml = mc[lo], mh = mc[lo + st];
+      const T gz = (zh - zl) * id;
+      const T tf = T(0.5) * (zh / mh + zl / ml);
+      const T gm = (mh - ml) * id;
+      return gz - tf * gm;
+    };
+    const T dx = (face(0, pplane, g.idx) - face(-pplane, pplane, g.idx)) * g.idx;
+    const T dy = (face(0, pz, g.idy) - face(-pz, pz, g.idy)) * g.idy;
+    const T dz = (face(0, 1, g.idz) - face(-1, 1, g.idz)) * g.idz;
+    const T div = (dx + dy) + dz;
+    const T nv = norm ? norm[v] : norm_scalar;
+    out[i] = D * div + nv * bc_flux;
+  }
+}
+
 template <typename T>
 __global__ void axpy_kernel(const T* __restrict__ a, const T* __restrict__ b, T c, T* __restrict__ out,
                             int64_t n) {
@@ -546,6 +681,81 @@ extern "C" int evx_ac_rhs(int32_t dtype, const void* u, void* out, const evx_gri
   hipStream_t s = static_cast<hipStream_t>(stream);
   return dtype == EVX_F32 ? launch_ac<float>(u, out, d, b, g, eps, gab, M, force, curvature, ep, s)
                           : launch_ac<double>(u, out, d, b, g, eps, gab, M, force, curvature, ep, s);
+}
+
+extern "C" int evx_simplex_project(int32_t dtype, const void* phis, void* out, int32_t n_phases, int64_t voxels,
+                                   int32_t euclidean, void* stream) {
+  if (dtype != EVX_F32 && dtype != EVX_F64) return fail(EVX_ERR_INVALID, "bad dtype");
+  if (n_phases < 1 || n_phases > MAX_PHASES)
+    return fail(EVX_ERR_UNSUPPORTED, "the fused projection handles 1 to %d phases", MAX_PHASES);
+  if (voxels < 0 || (voxels > 0 && (!phis || !out))) return fail(EVX_ERR_INVALID, "bad projection arguments");
+  if (voxels == 0) return EVX_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 grid(flat_blocks(voxels)), block(256);
+  if (dtype == EVX_F32) {
+    if (euclidean)
+      hipLaunchKernelGGL((simplex_project_kernel<float, true>), grid, block, 0, s, static_cast<const float*>(phis),
+                         static_cast<float*>(out), (int)n_phases, voxels);
+    else
+      hipLaunchKernelGGL((simplex_project_kernel<float, false>), grid, block, 0, s, static_cast<const float*>(phis),
+                         static_cast<float*>(out), (int)n_phases, voxels);
+  } else {
+    if (euclidean)
+      hipLaunchKernelGGL((simplex_project_kernel<double, true>), grid, block, 0, s, static_cast<const double*>(phis),
+                         static_cast<double*>(out), (int)n_phases, voxels);
+    else
+      hipLaunchKernelGGL((simplex_project_kernel<double, false>), grid, block, 0, s, static_cast<const double*>(phis),
+                         static_cast<double*>(out), (int)n_phases, voxels);
+  }
+  return check_launch("simplex_project_kernel");
+}
+
+template <typename T>
+static int launch_multiphase(const void* phis, const void* padded, void* out, const Dims& d, const evx_grid* g,
+                             double eps, double gab, double M, double curvature, hipStream_t s) {
+  MpCoef<T> k{static_cast<T>(-curvature), static_cast<T>(1.0 - curvature), static_cast<T>(9.0 / (2.0 * eps * eps)),
+              static_cast<T>((-M) * gab), static_cast<T>(d.c), curvature != 1.0};
+  hipLaunchKernelGGL(multiphase_rhs_kernel<T>, dim3(flat_blocks(d.vol())), dim3(256), 0, s,
+                     static_cast<const T*>(phis), static_cast<const T*>(padded), static_cast<T*>(out), d,
+                     make_geo<T>(g), k);
+  return check_launch("multiphase_rhs_kernel");
+}
+
+extern "C" int evx_multiphase_ac_rhs_padded(int32_t dtype, const void* phis, const void* padded, void* out,
+                                            const evx_grid* g, double eps, double gab, double M, double curvature,
+                                            void* stream) {
+  Dims d;
+  Bc b;
+  EVX_TRY(check_common(dtype, phis, out, g, nullptr, &d, &b));
+  if (!padded) return fail(EVX_ERR_INVALID, "null padded field");
+  if (d.c < 1 || d.c > MAX_PHASES)
+    return fail(EVX_ERR_UNSUPPORTED, "the fused multiphase right-hand side handles 1 to %d phases", MAX_PHASES);
+  if (!(eps != 0.0)) return fail(EVX_ERR_INVALID, "eps must be nonzero");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return dtype == EVX_F32 ? launch_multiphase<float>(phis, padded, out, d, g, eps, gab, M, curvature, s)
+                          : launch_multiphase<double>(phis, padded, out, d, g, eps, gab, M, curvature, s);
+}
+
+template <typename T>
+static int launch_sbm(const void* z, const void* m, const void* norm, double norm_scalar, void* out, const Dims& d,
+                      const evx_grid* g, double D, double bc_flux, hipStream_t s) {
+  hipLaunchKernelGGL(sbm_rhs_kernel<T>, dim3(flat_blocks(d.c * d.vol())), dim3(256), 0, s,
+                     static_cast<const T*>(z), static_cast<const T*>(m), static_cast<const T*>(norm),
+                     static_cast<T>(norm_scalar), static_cast<T*>(out), d, make_geo<T>(g), static_cast<T>(D),
+                     static_cast<T>(bc_flux));
+  return check_launch("sbm_rhs_kernel");
+}
+
+extern "C" int evx_sbm_rhs_padded(int32_t dtype, const void* z_padded, const void* mask_padded, const void* norm,
+                                  double norm_scalar, void* out, const evx_grid* g, double D, double bc_flux,
+                                  void* stream) {
+  Dims d;
+  Bc b;
+  EVX_TRY(check_common(dtype, z_padded, out, g, nullptr, &d, &b));
+  if (!mask_padded) return fail(EVX_ERR_INVALID, "null padded mask");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return dtype == EVX_F32 ? launch_sbm<float>(z_padded, mask_padded, norm, norm_scalar, out, d, g, D, bc_flux, s)
+                          : launch_sbm<double>(z_padded, mask_padded, norm, norm_scalar, out, d, g, D, bc_flux, s);
 }
 
 extern "C" int evx_axpy(int32_t dtype, const void* a, const void* b, double coef, void* out, int64_t n,

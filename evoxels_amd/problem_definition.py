@@ -449,8 +449,31 @@ class MultiPhaseAllenCahn(SemiLinearODE):
         df_dphi = 3 * phis * (sum_phi_squared - phis ** 2) + phis ** 3 - phis
         return self.pot_factor * df_dphi
 
+    def _kernel_path(self, n_phases):
+        """The HIP kernels serve the built-in projections and potential for up
+        to eight phases; replaced methods take the tensor path."""
+        proj = getattr(self.project_to_simplex, "__func__", None)
+        pot = getattr(self.calc_potential_derivatives, "__func__", None)
+        return n_phases <= 8 and pot is MultiPhaseAllenCahn._calc_well_derivatives and \
+            proj in (MultiPhaseAllenCahn._sloppy_simplex_projection, MultiPhaseAllenCahn._euclidean_simplex_projection)
+
     def rhs(self, t, phis):
         phis = self._check_state(phis)
+        n = phis.shape[0]
+        if self._kernel_path(n):
+            dt_code = _lib.dtype_code(phis.dtype)
+            stream = _lib.stream_handle(phis.device)
+            proj = torch.empty_like(phis)
+            euclid = getattr(self.project_to_simplex, "__func__", None) is \
+                MultiPhaseAllenCahn._euclidean_simplex_projection
+            _lib.call("evx_simplex_project", dt_code, _lib.ptr(phis), _lib.ptr(proj), n, phis[0].numel(),
+                      int(euclid), stream)
+            padded = self.pad_bc(proj).contiguous()
+            out = torch.empty_like(phis)
+            g = self._grid(proj)
+            _lib.call("evx_multiphase_ac_rhs_padded", dt_code, _lib.ptr(proj), _lib.ptr(padded), _lib.ptr(out),
+                      C.byref(g), float(self.eps), float(self.gab), float(self.M), float(self.curvature), stream)
+            return out
         phis = self.project_to_simplex(phis)
         phi_pad = self.pad_bc(phis)
         dfgrad_dphi = -self.curvature * self.vg.laplace(phi_pad)
@@ -585,6 +608,21 @@ class ReactionDiffusionSBM(ReactionDiffusion):
         vg = self.vg
         z = self.pad_bc(u)
         m = self.mask
+        if isinstance(self.bc_flux, (int, float)) and not isinstance(self.bc_flux, bool):
+            # one HIP launch for the divergence and the boundary flux term
+            z = z.contiguous()
+            out = torch.empty_like(u)
+            g = self._grid(u)
+            norm = self.norm
+            norm_t = norm.contiguous() if isinstance(norm, torch.Tensor) else None
+            norm_s = 0.0 if norm_t is not None else float(norm)
+            _lib.call("evx_sbm_rhs_padded", _lib.dtype_code(u.dtype), _lib.ptr(z), _lib.ptr(m.contiguous()),
+                      _lib.ptr(norm_t), norm_s, _lib.ptr(out), C.byref(g), float(self.D), float(self.bc_flux),
+                      _lib.stream_handle(u.device))
+            if not _is_default(self.f):
+                inner = m[:, 1:-1, 1:-1, 1:-1]
+                out = out + inner * self._eval_f(t, u / inner, torch)
+            return out
         divergence = vg.grad_x_face(vg.grad_x_face(z) - vg.to_x_face(z / m) * vg.grad_x_face(m))[:, :, 1:-1, 1:-1]
         divergence += vg.grad_y_face(vg.grad_y_face(z) - vg.to_y_face(z / m) * vg.grad_y_face(m))[:, 1:-1, :, 1:-1]
         divergence += vg.grad_z_face(vg.grad_z_face(z) - vg.to_z_face(z / m) * vg.grad_z_face(m))[:, 1:-1, 1:-1, :]

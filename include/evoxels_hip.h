@@ -152,6 +152,33 @@ int evx_coupled_rd_rhs(int32_t dtype, const void* u, void* out, const evx_grid* 
                        const evx_bc* bc, double D_A, double D_B, double feed,
                        double kill, const evx_epilogue* ep, void* stream);
 
+/* Simplex projection of the phase fractions of MultiPhaseAllenCahn, per
+ * voxel over n_phases channels of `voxels` values each: euclidean = 0 is the
+ * clip-and-normalise projection (problem_definition.py:508-512), 1 the
+ * Euclidean projection by sort and cumulative sum (problem_definition.py:
+ * 514-530).  1 <= n_phases <= 8. */
+int evx_simplex_project(int32_t dtype, const void* phis, void* out, int32_t n_phases,
+                        int64_t voxels, int32_t euclidean, void* stream);
+
+/* MultiPhaseAllenCahn.rhs with the 'well' potential (problem_definition.py:
+ * 532-573) from the projected phases `phis` (g->c = n_phases channels) and
+ * their padded copy (C, Nx+2, Ny+2, Nz+2):
+ * df = -curv*lap - (1-curv)*normal_lap + 9/(2eps^2)*(3phi(sum phi^2 - phi^2)
+ * + phi^3 - phi), r = -M*gab*(df - mean_phases(df)).  The normal Laplacian is
+ * skipped when curvature == 1. */
+int evx_multiphase_ac_rhs_padded(int32_t dtype, const void* phis, const void* padded,
+                                 void* out, const evx_grid* g, double eps, double gab,
+                                 double M, double curvature, void* stream);
+
+/* ReactionDiffusionSBM.rhs without the source term (problem_definition.py:
+ * 273-287) from the padded field and the padded (clipped) mask:
+ * r = D * sum_axes grad_face(grad_face(z) - to_face(z/m)*grad_face(m)) + norm*bc_flux,
+ * with `norm` an interior field (Nx, Ny, Nz) or NULL for the scalar
+ * norm_scalar.  The mask has one channel, shared by the g->c channels of z. */
+int evx_sbm_rhs_padded(int32_t dtype, const void* z_padded, const void* mask_padded,
+                       const void* norm, double norm_scalar, void* out, const evx_grid* g,
+                       double D, double bc_flux, void* stream);
+
 /* Elementwise out = a + coef*b (coef rounded to the working precision).
  * Used for generic-path Euler / RK4 stage updates (timesteppers.py:42-61). */
 int evx_axpy(int32_t dtype, const void* a, const void* b, double coef,

@@ -230,3 +230,47 @@ def test_sbm_euler_steps():
     for _ in range(4):
         ur = O.fe_step(ur, lambda t, x: O.rd_sbm_rhs(x, og, bcm, 1.0, m, norm, 0.1), 0.05)
     assert rel_l2(u.cpu().numpy(), ur.numpy()) < 1e-12
+
+
+# ------------------------------------------------- kernel path vs tensor path
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+@pytest.mark.parametrize("fast", [True, False])
+@pytest.mark.parametrize("curvature", [1.0, 0.3])
+def test_multiphase_kernels_equal_tensor_path(precision, fast, curvature):
+    """The projection + padded right-hand-side kernels against the tensor
+    path of the same class (taken when the projection is replaced)."""
+    shape = (10, 12, 66)
+    _, vg, _ = setup(shape, precision)
+    ph = torch.tensor(phases(shape, 4, 12), dtype=vg.precision, device="cuda")
+    bc = BCS["npp"]
+    k = problem(MultiPhaseAllenCahn, vg, eps=2.0, curvature=curvature, fast=fast, bc=bc)
+    t = problem(MultiPhaseAllenCahn, vg, eps=2.0, curvature=curvature, fast=fast, bc=bc)
+    builtin = t.project_to_simplex
+    t.project_to_simplex = lambda x: builtin(x)  # replaced: the tensor path runs
+    assert k._kernel_path(4) and not t._kernel_path(4)
+    a, b = k.rhs(0.0, ph).cpu().numpy(), t.rhs(0.0, ph).cpu().numpy()
+    assert rel_l2(a, b) < TOL[precision]
+
+
+def test_multiphase_nine_phases_take_the_tensor_path():
+    shape = (6, 8, 10)
+    _, vg, og = setup(shape, "float64")
+    arr = phases(shape, 9, 13)
+    p = problem(MultiPhaseAllenCahn, vg, eps=3.0, bc=BCS["ppp"])
+    assert not p._kernel_path(9)
+    got = p.rhs(0.0, torch.tensor(arr, dtype=torch.float64, device="cuda")).cpu().numpy()
+    want = O.multiphase_ac_rhs(og.field(arr)[0], og, BCS["ppp"], eps=3.0).numpy()
+    assert rel_l2(got, want) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_sbm_kernel_equals_tensor_path(precision):
+    """A float bc_flux takes the kernel, a tensor bc_flux the tensor path."""
+    shape = (10, 12, 66)
+    _, vg, _ = setup(shape, precision)
+    mask = smooth_mask(shape, 14)
+    u = torch.tensor(0.2 + 0.6 * np.random.default_rng(15).random((1,) + shape), dtype=vg.precision, device="cuda")
+    k = problem(ReactionDiffusionSBM, vg, D=0.7, bc=BCS["d_n_p"], mask=mask, bc_flux=0.3)
+    t = problem(ReactionDiffusionSBM, vg, D=0.7, bc=BCS["d_n_p"], mask=mask,
+                bc_flux=torch.tensor(0.3, dtype=vg.precision, device="cuda"))
+    assert rel_l2(k.rhs(0.0, u).cpu(), t.rhs(0.0, u).cpu()) < TOL[precision]
