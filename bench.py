@@ -502,7 +502,8 @@ def configs_leg(cpu):
     ms = _time_steps(one_n, 50, reps=2)
     out["f1_rd_imex_neumann_x_256cubed_fp32"] = {
         "ms_per_step": round(ms, 4), "voxel_steps_per_s": n ** 3 / (ms * 1e-3),
-        "note": "generic path: fused RHS, device mirror extension to 512 x 256 x 256, HIP FFTs, torch multiply"}
+        "note": "mirrored pipeline: RHS with the Neumann ghosts, extension to 512 x 256 x 256 in the staging "
+                "buffer, five sweeps over the extended grid, inverse sweeps on the 256 physical planes only"}
     gs = CoupledReactionDiffusion(vg)
     fe = ForwardEuler(gs, 0.1)
     state = {"u": torch.rand((2, n, n, n), device="cuda")}

@@ -82,6 +82,8 @@ SIGNATURES = {
     "evx_rfftn": [_P, _P, _P, _P],
     "evx_irfftn": [_P, _P, _P, _P],
     "evx_ch_imex_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
+    "evx_ch_imex_step_mirror": [_P, _P, _P, _P, C.POINTER(EvxImexParams), C.POINTER(EvxBC), _I32, _I32, _P],
+    "evx_rd_imex_step_mirror": [_P, _P, _P, _P, C.POINTER(EvxImexParams), C.POINTER(EvxBC), _I32, _I32, _P],
     "evx_rd_imex_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
     "evx_ch_etd1_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],
     "evx_rd_etd1_step": [_P, _P, _P, _P, C.POINTER(EvxImexParams), _P],

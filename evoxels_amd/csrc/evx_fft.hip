@@ -17,6 +17,7 @@
 //   x : forward C2C along x, times prefac(k)/N, inverse C2C along x
 //   y : inverse C2C along y
 //   z : C2R of line pairs, u_out = u + update
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -878,10 +879,120 @@ int rd_spectral_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, cons
                              : spectral_update<double>(p, rhs, spec, u, u_out, pf, s);
 }
 
+// Mirror extension of the right-hand side along x (boundary_conditions.py:
+// 61-71,133-144): planes [0, n) hold the field; the kernel writes the
+// extension planes after them.  mode 0: [f, flip f] (zero flux), 1:
+// [f, -flip f] (Dirichlet), 2: [f, 0, -flip f, 0] (staggered_x Dirichlet).
+template <typename T>
+__global__ void mirror_x_kernel(T* __restrict__ buf, int64_t n, int64_t plane, int mode) {
+  const int64_t planes = mode == 2 ? n + 2 : n;
+  const int64_t total = planes * plane;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = i / plane, r = i - j * plane;
+    T v;
+    if (mode < 2) {
+      const T f = buf[(n - 1 - j) * plane + r];
+      v = mode == 0 ? f : -f;
+    } else {
+      v = (j == 0 || j == n + 1) ? T(0) : -buf[(n - j) * plane + r];
+    }
+    buf[(n + j) * plane + r] = v;
+  }
+}
+
+// Semi-implicit / exponential step with one non-periodic axis, x
+// (timesteppers.py:85-89,195-199 with pad_fft_bc): the plan covers the
+// mirror-extended x length (2n, or 2n+2 on staggered_x grids); the
+// right-hand side of the n physical planes is evaluated with the ghost rules
+// of `bc`, extended in the staging buffer, transformed over the extended
+// grid, and only the n physical planes are transformed back and added to u.
+int mirror_spectral_step(evx_fft_plan* p, int ch, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
+                         const evx_bc* bc, int staggered, hipStream_t s, int etd) {
+  EVX_TRY(imex_common(p, u, u_out, ws, prm));
+  if (!bc) return fail(EVX_ERR_INVALID, "null bc");
+  if (prm->chan_params) return fail(EVX_ERR_UNSUPPORTED, "per-channel parameters need a periodic grid");
+  if (p->d.c != 1) return fail(EVX_ERR_UNSUPPORTED, "the mirrored spectral step handles one channel");
+  if (ch && !(prm->eps != 0.0)) return fail(EVX_ERR_INVALID, "eps must be nonzero");
+  const int kx = bc->kind[0];
+  const bool dir = kx == EVX_BC_DIRICHLET || kx == EVX_BC_NODE_VALUE;
+  const bool neu = kx == EVX_BC_NEUMANN || kx == EVX_BC_NODE_EXTRAP;
+  if (bc->kind[1] != EVX_BC_PERIODIC || bc->kind[2] != EVX_BC_PERIODIC || !(dir || neu))
+    return fail(EVX_ERR_UNSUPPORTED, "the mirrored spectral step needs x non-periodic and y, z periodic");
+  if (staggered && !dir)
+    return fail(EVX_ERR_UNSUPPORTED, "staggered_x spectral steps implement the Dirichlet extension only");
+  const int64_t NX = p->d.nx;
+  const int64_t n = staggered ? (NX - 2) / 2 : NX / 2;
+  if (n < 1 || (staggered ? NX != 2 * n + 2 : NX != 2 * n))
+    return fail(EVX_ERR_INVALID, "the plan's x length must be the mirror extension of the field");
+  double dd[3], dd2[3];
+  geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
+  void* rhs = ws;
+  void* spec = static_cast<char*>(ws) + real_bytes(p);
+  const evx_grid g{1, n, p->d.ny, p->d.nz, {dd[0], dd[1], dd[2]}, {dd2[0], dd2[1], dd2[2]}};
+  const evx_epilogue e{EVX_OUT_RHS, 0.0, nullptr, nullptr, nullptr};
+  EVX_TRY(trace_start(p, s));
+  EVX_TRY(ch ? evx_ch_rhs(p->dtype, u, rhs, &g, bc, prm->eps, prm->D, &e, s)
+             : evx_rd_rhs(p->dtype, u, rhs, &g, bc, prm->D, &e, s));
+  const int64_t plane = p->d.ny * p->d.nz;
+  const int mode = staggered ? 2 : (dir ? 1 : 0);
+  const int64_t total = (staggered ? n + 2 : n) * plane;
+  const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 65535LL * 16);
+  if (p->dtype == EVX_F32)
+    hipLaunchKernelGGL(mirror_x_kernel<float>, dim3(blocks), dim3(256), 0, s, static_cast<float*>(rhs), n, plane, mode);
+  else
+    hipLaunchKernelGGL(mirror_x_kernel<double>, dim3(blocks), dim3(256), 0, s, static_cast<double*>(rhs), n, plane,
+                       mode);
+  EVX_TRY(check_launch("mirror_x_kernel"));
+  EVX_TRY(trace_tick(p, TK_RHS, s));
+  // problem_definition.py:206,303 -> (-D)*A or ((-2*eps)*D)*A as Python doubles
+  Prefac pf = make_prefac(p, ch, ch ? ((-2.0 * prm->eps) * prm->D) * prm->A : (-prm->D) * prm->A, prm->dt,
+                          prm->spacing);
+  pf.A = prm->A;
+  pf.etd = etd;
+  if (etd) pf.exact = 0;
+  const int64_t pt = p->pitch;
+  if (p->dtype == EVX_F32) {
+    EVX_TRY(run_z_r2c<float>(p, rhs, spec, pt, s));
+    EVX_TRY(trace_tick(p, TK_Z_R2C, s));
+    EVX_TRY((run_strided<float, 0>(p, 1, spec, pf, pt, s)));
+    EVX_TRY(trace_tick(p, TK_Y_FWD, s));
+    EVX_TRY(run_x_multiplier<float>(p, spec, pf, pt, s));
+    EVX_TRY(trace_tick(p, TK_X, s));
+    EVX_TRY((run_strided<float, 1>(p, 1, spec, pf, pt, s, 0, n)));
+    EVX_TRY(trace_tick(p, TK_Y_INV, s));
+    EVX_TRY(run_z_c2r<float>(p, spec, u_out, u, 1.0, pt, s, 0, n));
+  } else {
+    EVX_TRY(run_z_r2c<double>(p, rhs, spec, pt, s));
+    EVX_TRY(trace_tick(p, TK_Z_R2C, s));
+    EVX_TRY((run_strided<double, 0>(p, 1, spec, pf, pt, s)));
+    EVX_TRY(trace_tick(p, TK_Y_FWD, s));
+    EVX_TRY(run_x_multiplier<double>(p, spec, pf, pt, s));
+    EVX_TRY(trace_tick(p, TK_X, s));
+    EVX_TRY((run_strided<double, 1>(p, 1, spec, pf, pt, s, 0, n)));
+    EVX_TRY(trace_tick(p, TK_Y_INV, s));
+    EVX_TRY(run_z_c2r<double>(p, spec, u_out, u, 1.0, pt, s, 0, n));
+  }
+  EVX_TRY(trace_tick(p, TK_Z_C2R, s));
+  trace_end(p);
+  return EVX_OK;
+}
+
 }  // namespace
 }  // namespace evx
 
 using namespace evx;
+
+extern "C" int evx_ch_imex_step_mirror(evx_fft_plan* p, const void* u, void* u_out, void* ws,
+                                       const evx_imex_params* prm, const evx_bc* bc, int32_t staggered, int32_t etd,
+                                       void* stream) {
+  return mirror_spectral_step(p, 1, u, u_out, ws, prm, bc, staggered, static_cast<hipStream_t>(stream), etd);
+}
+
+extern "C" int evx_rd_imex_step_mirror(evx_fft_plan* p, const void* u, void* u_out, void* ws,
+                                       const evx_imex_params* prm, const evx_bc* bc, int32_t staggered, int32_t etd,
+                                       void* stream) {
+  return mirror_spectral_step(p, 0, u, u_out, ws, prm, bc, staggered, static_cast<hipStream_t>(stream), etd);
+}
 
 extern "C" int evx_ch_imex_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
                                 void* stream) {

@@ -4,7 +4,8 @@
 problem with a fused kernel (default closures) the state update is fused into
 the right-hand-side kernel: ForwardEuler is one launch, RungeKutta4 four, and
 PseudoSpectralIMEX on a fully periodic Cahn-Hilliard or reaction-diffusion
-problem is the five-sweep spectral pipeline of evx_*_imex_step.  Otherwise
+problem is the five-sweep spectral pipeline of evx_*_imex_step, and with x
+non-periodic the mirrored pipeline of evx_*_imex_step_mirror.  Otherwise
 the reference's composition runs over the HIP operators.
 """
 from __future__ import annotations
@@ -109,11 +110,19 @@ class PseudoSpectralIMEX(TimeStepper):
         self.pad = self.problem.pad_fft_bc
         p = self.problem
         self._kind = None
+        self._mirror = None
         if p.fused and p.bc_type == ("periodic", "periodic", "periodic"):
             if isinstance(p, CahnHilliard):
                 self._kind = "ch"
             elif isinstance(p, ReactionDiffusion):
                 self._kind = "rd"
+        elif p.fused and p.bc_type[1:] == ("periodic", "periodic"):
+            # one non-periodic axis, x: the mirrored pipeline for one channel
+            staggered = p.vg.convention == "staggered_x"
+            if isinstance(p, CahnHilliard) and not staggered:
+                self._mirror = "ch"
+            elif isinstance(p, ReactionDiffusion) and (not staggered or p.bc_type[0] == "dirichlet"):
+                self._mirror = "rd"
         if self._kind is None:
             self._fft_prefac = self._multiplier()
         self._ws = None
@@ -140,7 +149,7 @@ class PseudoSpectralIMEX(TimeStepper):
         p = self.problem
         prm.D = float(p.D)
         prm.A = float(p.A)
-        prm.eps = float(p.eps) if self._kind == "ch" else 0.0
+        prm.eps = float(p.eps) if "ch" in (self._kind, self._mirror) else 0.0
         return prm
 
     def workspace(self, u):
@@ -158,10 +167,33 @@ class PseudoSpectralIMEX(TimeStepper):
                   _lib.stream_handle(u.device))
         return out
 
+    _ETD = 0
+
+    def _mirror_step(self, u):
+        """Fused step with x non-periodic: the plan covers the mirror-extended
+        x length (2n, or 2n+2 on staggered_x grids)."""
+        p = self.problem
+        staggered = p.vg.convention == "staggered_x"
+        n = u.shape[1]
+        ext = (1, 2 * n + 2 if staggered else 2 * n) + tuple(u.shape[2:])
+        plan = p.vg.plan(ext, u.dtype)
+        if self._ws is None or self._ws.numel() < plan.workspace_bytes or self._ws.device != u.device:
+            self._ws = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device=u.device)
+        out = torch.empty_like(u)
+        prm = self._params()
+        bc = _lib.make_bc(p.bc, p.vg.convention)
+        fn = "evx_ch_imex_step_mirror" if self._mirror == "ch" else "evx_rd_imex_step_mirror"
+        _lib.call(fn, plan.handle, _lib.ptr(u), _lib.ptr(out), _lib.ptr(self._ws), C.byref(prm), C.byref(bc),
+                  int(staggered), self._ETD, _lib.stream_handle(u.device))
+        return out
+
     def step(self, t: float, u: State) -> State:
         if self._kind is not None:
             u = self.problem._check_state(u)
             return self.step_into(u, torch.empty_like(u))
+        if self._mirror is not None and u.shape[0] == 1 and \
+                not any(isinstance(v, torch.Tensor) for v in (self.problem.bc[0][1] or ())):
+            return self._mirror_step(self.problem._check_state(u))
         vg = self.problem.vg
         dc = self.pad(self.problem.rhs(t, u))
         dc_fft = self._fft_prefac * vg.rfftn(dc, dc.shape)
@@ -180,6 +212,7 @@ class ExponentialEuler(PseudoSpectralIMEX):
     five-sweep pipeline (evx_*_etd1_step)."""
 
     _FUSED = {"ch": "evx_ch_etd1_step", "rd": "evx_rd_etd1_step"}
+    _ETD = 1
     _PADE_N = (1, 1 / 26, 5 / 156, 1 / 858, 1 / 5720, 1 / 205920, 1 / 8648640)
     _PADE_D = (1, -6 / 13, 5 / 52, -5 / 429, 1 / 1144, -1 / 25740, 1 / 1235520)
 

@@ -243,6 +243,24 @@ int evx_ch_etd1_step(evx_fft_plan* plan, const void* u, void* u_out,
 int evx_rd_etd1_step(evx_fft_plan* plan, const void* u, void* u_out,
                      void* workspace, const evx_imex_params* p, void* stream);
 
+/* PseudoSpectralIMEX.step / ExponentialEuler.step (etd = 1) of CahnHilliard
+ * (ch entry) or ReactionDiffusion (rd entry) with default closures, one
+ * channel, x non-periodic and y, z periodic (timesteppers.py:85-89,195-199,
+ * problem_definition.py:138-177): the right-hand side of u (1, n, Ny, Nz) is
+ * evaluated with the ghost rules of `bc`, mirror-extended along x as
+ * pad_fft_bc does (boundary_conditions.py:61-71,133-144: [f, flip f] zero
+ * flux, [f, -flip f] Dirichlet, [f, 0, -flip f, 0] staggered_x Dirichlet),
+ * multiplied in k-space on the extended grid and the first n planes are
+ * added to u.  The plan is made for (1, 2n, Ny, Nz) (cell_center) or
+ * (1, 2n+2, Ny, Nz) (staggered = 1, Dirichlet only; the CH entry needs
+ * cell_center); its workspace as for the periodic steps. */
+int evx_ch_imex_step_mirror(evx_fft_plan* plan, const void* u, void* u_out, void* workspace,
+                            const evx_imex_params* p, const evx_bc* bc, int32_t staggered,
+                            int32_t etd, void* stream);
+int evx_rd_imex_step_mirror(evx_fft_plan* plan, const void* u, void* u_out, void* workspace,
+                            const evx_imex_params* p, const evx_bc* bc, int32_t staggered,
+                            int32_t etd, void* stream);
+
 /* ---- adjoints (reverse mode) ----------------------------------------------
  * The reference differentiates its solvers with JAX (inversion.py:20-233);
  * these entry points give the vector-Jacobian products of the torch path's

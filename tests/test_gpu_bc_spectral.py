@@ -241,3 +241,43 @@ def test_nonperiodic_cahn_hilliard_imex_step(kind):
                               O.imex_prefac(0.1, sym), kind).numpy()
     a32 = arr.astype(np.float32).astype(np.float64)
     assert rel_l2(got - a32, want - arr) < 5e-5
+
+
+@pytest.mark.parametrize("conv,kind,nx", [("cell_center", "neumann", 64), ("cell_center", "dirichlet", 50),
+                                          ("staggered_x", "dirichlet", 65)])
+@pytest.mark.parametrize("stepper", ["imex", "etd1"])
+def test_mirrored_step_equals_generic_path(conv, kind, nx, stepper):
+    """One channel takes the fused mirrored pipeline, two identical channels
+    the generic composition (device extension, HIP FFTs, torch multiply)."""
+    shape = (nx, 16, 24)
+    vf = evx.VoxelFields(shape, (1.0, 0.6, 0.8), convention=conv)
+    vf.precision = "float64"
+    vg = VoxelGridHip(vf.grid_info(), precision="float64", device="cuda")
+    bc = ((kind, None if kind == "neumann" else (0.2, 0.7)), P, P)
+    p = quiet(ReactionDiffusion, vg, D=0.8, bc=bc)
+    st = (PseudoSpectralIMEX if stepper == "imex" else ExponentialEuler)(p, 0.01)
+    assert st._mirror == "rd"
+    n_int = nx if conv == "cell_center" else nx - 2
+    arr = np.random.default_rng(20).random((1, n_int, 16, 24))
+    one = st.step(0.0, torch.tensor(arr, device="cuda")).cpu().numpy()
+    two = st.step(0.0, torch.tensor(np.concatenate([arr, arr]), device="cuda")).cpu().numpy()
+    assert rel_l2(one - arr, two[:1] - arr) < 1e-11
+
+
+@pytest.mark.parametrize("kind", ["neumann", "dirichlet"])
+def test_mirrored_cahn_hilliard_float64_matches_oracle(kind):
+    shape = (32, 20, 18)
+    vf = evx.VoxelFields(shape, shape)
+    vf.precision = "float64"
+    vg = VoxelGridHip(vf.grid_info(), precision="float64", device="cuda")
+    og = O.OracleGrid(shape, vf.spacing, "float64")
+    bc = ((kind, None if kind == "neumann" else (0.3, 0.6)), P, P)
+    p = quiet(CahnHilliard, vg, eps=3.0, D=1.0, bc=bc)
+    st = PseudoSpectralIMEX(p, 0.1)
+    assert st._mirror == "ch"
+    arr = 0.5 + 0.05 * np.random.default_rng(21).standard_normal((1,) + shape)
+    got = st.step(0.0, torch.tensor(arr, device="cuda")).cpu().numpy()
+    sym = O.ch_symbol(O.rfft_k_squared_nonperiodic(shape, og.spacing), 3.0, 1.0)
+    want = O.spectral_step_bc(og.field(arr)[0], lambda t, x: O.ch_rhs(x, og, bc, 3.0, 1.0),
+                              O.imex_prefac(0.1, sym), kind).numpy()
+    assert rel_l2(got - arr, want - arr) < 1e-10
