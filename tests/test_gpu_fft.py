@@ -170,6 +170,40 @@ def test_rd_imex_512_single_mode_closed_form():
     assert (u1 - expect).abs().max().item() < 2e-6
 
 
+def test_rd_imex_1024_single_mode_closed_form():
+    """The largest specialised size: RD + IMEX at 1024^3 FP32 (4 GiB field,
+    8.6 GB workspace) on one Fourier mode, u1 = u0*(1 + prefac*D*lam_h), and
+    the mean (the k = 0 mode) untouched."""
+    n, D, dt = 1024, 0.7, 0.05
+    vf = evx.VoxelFields((n, n, n), (n, n, n))
+    vg = VoxelGridHip(vf.grid_info(), precision="float32", device="cuda")
+    step = PseudoSpectralIMEX(ReactionDiffusion(vg, D=D), dt)
+    a = (5, 3, 11)
+    ix = torch.arange(n, device="cuda", dtype=torch.float32)
+    phase = [2 * np.pi * a[i] / n * ix for i in range(3)]
+    # cos(p0 + p1 + p2) built plane by plane to keep the float64 temporaries small
+    u0 = torch.empty((1, n, n, n), dtype=torch.float32, device="cuda")
+    pyz = (phase[1].double()[:, None] + phase[2].double()[None, :])
+    for x in range(n):
+        u0[0, x] = (0.5 + 0.25 * torch.cos(phase[0][x].double() + pyz)).float()
+    del pyz
+    lam = -sum(4 * np.sin(np.pi * a[i] / n) ** 2 for i in range(3))
+    k = [np.float32(2 * np.pi) * (np.float32(a[i]) * np.float32(1.0 / n)) for i in range(3)]
+    k2 = (k[0] * k[0] + k[1] * k[1]) + k[2] * k[2]
+    pre = (np.float32(1) / (np.float32(1) - np.float32(dt) * (np.float32(-D * 0.25) * k2))) * np.float32(dt)
+    u1 = step.step(0.0, u0)
+    fac = float(1 + float(pre) * D * lam)
+    err = 0.0
+    for x0 in range(0, n, 128):
+        want = 0.5 + (u0[:, x0:x0 + 128] - 0.5) * fac
+        err = max(err, float((u1[:, x0:x0 + 128] - want).abs().max()))
+    print(f"1024^3 single mode: max abs err {err:.3e}", flush=True)
+    assert err < 2e-6
+    m0 = float(vg.channel_sums(u0)[0]) / n ** 3
+    m1 = float(vg.channel_sums(u1)[0]) / n ** 3
+    assert abs(m1 - m0) < 1e-6
+
+
 @pytest.mark.parametrize("precision", ["float32", "float64"])
 def test_specialised_and_generic_kernels_agree_with_two_channels(precision):
     """Two channels through the size-specialised kernels, against the
