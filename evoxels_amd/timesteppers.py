@@ -196,8 +196,9 @@ class PseudoSpectralIMEX(TimeStepper):
             return self._mirror_step(self.problem._check_state(u))
         vg = self.problem.vg
         dc = self.pad(self.problem.rhs(t, u))
-        dc_fft = self._fft_prefac * vg.rfftn(dc, dc.shape)
-        update = vg.irfftn(dc_fft, dc.shape)[:, :u.shape[1]]
+        dc_fft = vg.rfftn(dc, dc.shape)
+        dc_fft.mul_(self._fft_prefac)  # the spectrum is a temporary: multiplied and transformed in place
+        update = vg.irfftn(dc_fft, dc.shape, overwrite=True)[:, :u.shape[1]]
         return u + update
 
 

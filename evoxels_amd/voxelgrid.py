@@ -336,13 +336,15 @@ class VoxelGridHip(FDStencils):
         _lib.call("evx_rfftn", p.handle, _lib.ptr(field), _lib.ptr(spec), _lib.stream_handle(field.device))
         return spec
 
-    def irfftn(self, spec, shape):
-        """Per-channel inverse real 3-D FFT (voxelgrid.py:206-207)."""
+    def irfftn(self, spec, shape, overwrite=False):
+        """Per-channel inverse real 3-D FFT (voxelgrid.py:206-207).  The
+        transform runs in place on its input, so the spectrum is copied first
+        unless the caller passes ``overwrite=True`` for a temporary."""
         c, nx, ny, nz = (int(v) for v in shape)
         if spec.shape != (c, nx, ny, nz // 2 + 1):
             raise NotImplementedError("irfftn with a spectrum shape that does not match `shape`")
         rdt = torch.float32 if spec.dtype == torch.complex64 else torch.float64
-        work = spec.contiguous().clone()
+        work = spec if overwrite and spec.is_contiguous() else spec.contiguous().clone()
         p = self.plan((c, nx, ny, nz), rdt)
         out = torch.empty((c, nx, ny, nz), dtype=rdt, device=spec.device)
         _lib.call("evx_irfftn", p.handle, _lib.ptr(work), _lib.ptr(out), _lib.stream_handle(spec.device))
