@@ -563,7 +563,7 @@ class SmoothedBoundaryODE(ODE):
 
 
 @dataclass
-class ReactionDiffusionSBM(ReactionDiffusion):
+class ReactionDiffusionSBM(ReactionDiffusion, SmoothedBoundaryODE):
     """problem_definition.py:231-287: reaction-diffusion on a smoothed
     boundary (diffuse domain) given by ``mask``; ``bc_flux`` is the flux
     imposed through the diffuse boundary.  cell_center convention."""
