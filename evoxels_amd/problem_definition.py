@@ -449,6 +449,11 @@ class MultiPhaseAllenCahn(SemiLinearODE):
         df_dphi = 3 * phis * (sum_phi_squared - phis ** 2) + phis ** 3 - phis
         return self.pot_factor * df_dphi
 
+    def _calc_obstacle_derivatives(self, phis):
+        """problem_definition.py:536-538 (the reference keeps the obstacle
+        potential unselectable; kept for API parity)."""
+        return -self.pot_factor * phis
+
     def _kernel_path(self, n_phases):
         """The HIP kernels serve the built-in projections and potential for up
         to eight phases; replaced methods take the tensor path."""
