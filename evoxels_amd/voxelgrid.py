@@ -336,6 +336,36 @@ class VoxelGridHip(FDStencils):
         _lib.call("evx_rfftn", p.handle, _lib.ptr(field), _lib.ptr(spec), _lib.stream_handle(field.device))
         return spec
 
+    @staticmethod
+    def _negate_freq(a, dims):
+        """a at the negated frequency index (-k mod n) along ``dims``."""
+        return torch.roll(torch.flip(a, dims), shifts=[1] * len(dims), dims=dims)
+
+    def fftn(self, field, shape):
+        """Full complex 3-D FFT per channel of a real field (voxelgrid.py:
+        200-201): the half spectrum of the HIP R2C, completed by Hermitian
+        symmetry X[k] = conj(X[-k])."""
+        if field.is_complex():
+            raise NotImplementedError("fftn of a complex field: the 'hip' transforms are real-to-complex")
+        c, nx, ny, nz = (int(v) for v in shape)
+        half = self.rfftn(field, shape)
+        nh = nz // 2 + 1
+        full = torch.empty((c, nx, ny, nz), dtype=half.dtype, device=half.device)
+        full[..., :nh] = half
+        if nz > nh:
+            mirrored = torch.conj(self._negate_freq(half[..., 1:nz - nh + 1], [1, 2]))
+            full[..., nh:] = torch.flip(mirrored, [3])
+        return full
+
+    def real_of_ifftn(self, field, shape):
+        """real(ifftn(F)) per channel (voxelgrid.py:209-210) as the C2R of the
+        Hermitian part (F + conj(F(-k)))/2, which has the same real inverse."""
+        c, nx, ny, nz = (int(v) for v in shape)
+        if tuple(field.shape) != (c, nx, ny, nz):
+            raise NotImplementedError("real_of_ifftn with a spectrum shape that does not match `shape`")
+        herm = 0.5 * (field + torch.conj(self._negate_freq(field, [1, 2, 3])))
+        return self.irfftn(herm[..., :nz // 2 + 1].contiguous(), shape, overwrite=True)
+
     def irfftn(self, spec, shape, overwrite=False):
         """Per-channel inverse real 3-D FFT (voxelgrid.py:206-207).  The
         transform runs in place on its input, so the spectrum is copied first
