@@ -94,6 +94,8 @@ __device__ __forceinline__ void strided_dispatch(const BInst& b, C* spec, const 
     lm.es = (int64_t)b.ny * b.nh;
     n = b.nx;
   }
+  // Grids of at most 128 points per axis keep every brick offset far below
+  // the 4 GiB of strided_tile's 32-bit lane offsets.
   C* s = spec + b.s_off;
   const C* t = tw + b.tw[axis];
   switch (n) {

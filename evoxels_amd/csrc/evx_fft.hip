@@ -346,6 +346,7 @@ bool fast_strided(int n, void* spec, int64_t nlines, const LineMap& lm, const vo
 #define X(NN)                                                                                               \
   case NN: {                                                                                                \
     using G = SGeo<T, NN, MODE>;                                                                            \
+    if (!brick_offsets_fit<T>(nlines, lm, NN, G::TPL, G::TL)) return false;                                 \
     const int64_t blocks = (nlines + G::TL - 1) / G::TL;                                                    \
     hipLaunchKernelGGL((strided_fast<T, NN, MODE>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,         \
                        static_cast<cpx<T>*>(spec), nlines, lm, static_cast<const cpx<T>*>(tw), pf);          \

@@ -24,6 +24,113 @@ __device__ __forceinline__ int64_t line_start(const LineMap& m, int64_t L) {
   return (outer / m.G) * m.GM + (outer % m.G) * m.OM + k;
 }
 
+// Line L0 + t of a brick whose first line L0 is block-uniform: the divisions
+// resolve L0 once per wave; a lane's line carries k into o and o into g
+// (k = L % nh, o = outer % G, g = outer / G, outer = L / nh), which takes a
+// division only when nh is smaller than the lane offset t.  rel is the
+// element offset of the lane's line from the brick's first line (>= 0:
+// line starts grow with L since GM >= G*OM and OM >= nh or G == 1).
+struct BrickLine {
+  int64_t start0;
+  uint32_t k, o, g, rel;
+};
+
+__host__ __device__ __forceinline__ BrickLine brick_line(const LineMap& m, uint32_t L0, uint32_t t) {
+  const uint32_t nh = (uint32_t)m.nh, G = (uint32_t)m.G;
+  const uint32_t outer0 = L0 / nh, k0 = L0 - outer0 * nh;
+  const uint32_t g0 = outer0 / G, o0 = outer0 - g0 * G;
+  BrickLine b;
+  b.start0 = (int64_t)g0 * m.GM + (int64_t)o0 * m.OM + k0;
+  uint32_t k = k0 + t, o = o0, g = g0;
+  if (k >= nh) {
+    uint32_t q = 1;
+    k -= nh;
+    if (k >= nh) {
+      const uint32_t r = k / nh;
+      q += r;
+      k -= r * nh;
+    }
+    o += q;
+    if (G == 1) {
+      g += o;
+      o = 0;
+    } else if (o >= G) {
+      const uint32_t r = o / G;
+      g += r;
+      o -= r * G;
+    }
+  }
+  b.k = k;
+  b.o = o;
+  b.g = g;
+  b.rel = (uint32_t)((int64_t)(g - g0) * m.GM + ((int64_t)o - (int64_t)o0) * m.OM + ((int64_t)k - (int64_t)k0));
+  return b;
+}
+
+// A block-uniform pointer moved to scalar registers (the buffer descriptors
+// below are scalar operands).
+template <typename P>
+__device__ __forceinline__ P* uniform_ptr(P* p) {
+  const uint64_t a = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(a >> 32));
+  return reinterpret_cast<P*>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Raw (stride 0) buffer descriptor over the 4 GiB from p; the fourth word is
+// the gfx9-family default (32-bit data format, no swizzle).  Loads and stores
+// through it take a 32-bit per-lane offset plus a 32-bit scalar offset, so a
+// brick needs one vector register of addressing for all its elements.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(p, 0, -1, 0x00020000);
+}
+
+// (The words are moved with memcpy: a bit_cast of a vector element reads the
+// wrong element with this compiler.)
+template <typename T>
+__device__ __forceinline__ cpx<T> buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  cpx<T> v;
+  if constexpr (sizeof(cpx<T>) == 8) {
+    const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    __builtin_memcpy(&v, &w, sizeof(v));
+  } else {
+    const auto w = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    __builtin_memcpy(&v, &w, sizeof(v));
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ void buf_store(cpx<T> v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(cpx<T>) == 8) {
+    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+    u2 w;
+    __builtin_memcpy(&w, &v, sizeof(v));
+    __builtin_amdgcn_raw_buffer_store_b64(w, r, voff, soff, 0);
+  } else {
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    u4 w;
+    __builtin_memcpy(&w, &v, sizeof(v));
+    __builtin_amdgcn_raw_buffer_store_b128(w, r, voff, soff, 0);
+  }
+}
+
+// Host-side condition of the brick addressing: line indices in 32 bits, and
+// every element reached through one of the brick's two descriptors (elements
+// q < E/2 from the brick's first element, the rest from element E/2) at a
+// lane offset rel + tl*es plus a scalar offset below E/2 * TPL*es, the sum
+// below 4 GiB.  A brick of TL lines wraps k at most TL/nh + 1 times and g at
+// most TL/(G*nh) + 1 times (never when all lines share one g).
+template <typename T>
+inline bool brick_offsets_fit(int64_t nlines, const LineMap& m, int n, int tpl, int tl) {
+  if (nlines >= (int64_t(1) << 31) || m.nh >= (int64_t(1) << 31) || m.G >= (int64_t(1) << 31)) return false;
+  const int64_t wraps_o = tl / m.nh + 1, wraps_g = nlines > m.G * m.nh ? tl / (m.G * m.nh) + 1 : 0;
+  const int64_t lane = wraps_g * m.GM + wraps_o * m.OM + tl + (int64_t)(tpl - 1) * m.es;
+  const int64_t e = n / tpl, half = (e + 1) / 2;
+  const int64_t scalar = (half - 1) * (int64_t)tpl * m.es;
+  return (lane + scalar) * (int64_t)sizeof(cpx<T>) < (int64_t(1) << 32);
+}
+
 // float32 IMEX multiplier of the reference: k^2 from fftfreq (float32), the
 // symbol sc*k^2 (RD) or sc*(k^2)^2 (CH), prefac = reciprocal(1 - dt*sym)*dt.
 struct Prefac {
@@ -295,40 +402,71 @@ struct STile {
       sizeof(cpx<T>) * N + (SPLIT ? sizeof(T) : sizeof(cpx<T>)) * SGeo<T, N, MODE>::TL * N;
 };
 
-// Strided sweep of brick `tile` (TL adjacent lines).  MODE 0: forward,
-// 1: inverse, 2: forward * prefac * inverse (x axis only) with the float32
-// multiplier, 3: the same with the exact-symbol float64 multiplier, 4: the
-// same with the exponential Euler multiplier.
+// Global side of one brick (TL adjacent lines): lane t serves line L0 + t,
+// lanes past the last line read line L0 (always present) and store nothing.
+// Element q of a lane sits at descriptor q < H ? lo : hi, scalar offset
+// (q mod H)*TPL*es and lane offset rel + tl*es (bytes; the host checks the
+// 4 GiB bound with brick_offsets_fit).
 template <typename T, int N, int MODE>
-__device__ __forceinline__ void strided_tile(cpx<T>* __restrict__ spec, int64_t nlines, const LineMap& lm,
-                                             const cpx<T>* __restrict__ twg, const Prefac& pf, int64_t tile,
-                                             int tid, unsigned char* smem) {
+struct BrickIo {
   using G = SGeo<T, N, MODE>;
-  constexpr int TPL = G::TPL, TL = G::TL, E = N / TPL;
-  cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem);
-  cpx<T>* lds = tw + N;
-  const int t = tid % TL, tl = tid / TL;
-  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
-  const int64_t L = tile * TL + t;
-  const bool active = L < nlines;
-  const int64_t start = active ? line_start(lm, L) : 0;
-  cpx<T> v[E];
+  static constexpr int TPL = G::TPL, TL = G::TL, E = N / TPL, H = (E + 1) / 2;
+  __amdgpu_buffer_rsrc_t rlo, rhi;
+  uint32_t voff, qstep;
+  bool active;
+  BrickLine bl;
+
+  __device__ __forceinline__ void setup(cpx<T>* spec, int64_t nlines, const LineMap& lm, uint32_t tile, int t,
+                                        int tl) {
+    const uint32_t L0 = tile * TL;
+    active = (int64_t)L0 + t < nlines;
+    bl = brick_line(lm, L0, active ? t : 0);
+    cpx<T>* const base = uniform_ptr(spec + bl.start0);
+    qstep = (uint32_t)TPL * (uint32_t)lm.es * (uint32_t)sizeof(cpx<T>);
+    rlo = raw_rsrc(base);
+    rhi = raw_rsrc(base + (int64_t)H * TPL * lm.es);
+    voff = (bl.rel + (uint32_t)tl * (uint32_t)lm.es) * (uint32_t)sizeof(cpx<T>);
+  }
+  __device__ __forceinline__ void load(cpx<T> (&v)[E]) const {
 #pragma unroll
-  for (int q = 0; q < E; ++q)
-    v[q] = active ? spec[start + (int64_t)(tl + TPL * q) * lm.es] : cpx<T>{T(0), T(0)};
-  __syncthreads();
-  constexpr bool SPLIT = STile<T, N, MODE>::SPLIT;
-  using Lay = typename std::conditional<SPLIT, BrickLaySplit<T, TL>, BrickLay<T, TL>>::type;
-  Lay lay;
-  if constexpr (SPLIT) lay.base = reinterpret_cast<T*>(lds) + t;
+    for (int q = 0; q < E; ++q) v[q] = buf_load<T>(q < H ? rlo : rhi, voff, (uint32_t)(q < H ? q : q - H) * qstep);
+  }
+  __device__ __forceinline__ void store(const cpx<T> (&v)[E]) const {
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) buf_store<T>(v[q], q < H ? rlo : rhi, voff, (uint32_t)(q < H ? q : q - H) * qstep);
+    }
+  }
+};
+
+// LDS exchange layout of a brick: line t's elements TL apart.
+template <typename T, int N, int MODE>
+using BrickLayout = typename std::conditional<STile<T, N, MODE>::SPLIT, BrickLaySplit<T, SGeo<T, N, MODE>::TL>,
+                                              BrickLay<T, SGeo<T, N, MODE>::TL>>::type;
+
+template <typename T, int N, int MODE>
+__device__ __forceinline__ BrickLayout<T, N, MODE> brick_layout(cpx<T>* lds, int t) {
+  BrickLayout<T, N, MODE> lay;
+  if constexpr (STile<T, N, MODE>::SPLIT) lay.base = reinterpret_cast<T*>(lds) + t;
   else lay.base = lds + t;
+  return lay;
+}
+
+// In-register transform of a brick: forward (MODE 0, 2-4) or inverse (1);
+// MODE >= 2 multiplies by the step's multiplier and transforms back.  The
+// caller has synchronised the block since the exchange buffer's last use.
+template <typename T, int N, int MODE, int E, typename Lay>
+__device__ __forceinline__ void brick_transform(cpx<T> (&v)[E], const Lay& lay, int tl, const cpx<T>* tw,
+                                                const BrickLine& bl, const Prefac& pf) {
+  constexpr int TPL = SGeo<T, N, MODE>::TPL;
+  static_assert(E == N / TPL, "one brick's values per thread");
   fft_regs<T, N, TPL, MODE == 1>(v, lay, tl, tw, typename FftSpec<N>::R{});
   if constexpr (MODE >= 2) {
-    const int64_t outer = L / lm.nh;
-    const int jz = (int)(L - outer * lm.nh);
-    const int jy = (int)(outer % pf.ny);
+    // x lines: G = ny, so (k, o, g) = (jz, jy, channel)
+    const int jz = (int)bl.k;
+    const int jy = (int)bl.o;
     const T sc = static_cast<T>(pf.scale);
-    const int64_t ch = active ? outer / pf.ny : 0;
+    const int64_t ch = bl.g;
     if constexpr (MODE == 3) {
 #pragma unroll
       for (int q = 0; q < E; ++q)
@@ -344,10 +482,29 @@ __device__ __forceinline__ void strided_tile(cpx<T>* __restrict__ spec, int64_t 
     }
     fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
   }
-  if (active) {
-#pragma unroll
-    for (int q = 0; q < E; ++q) spec[start + (int64_t)(tl + TPL * q) * lm.es] = v[q];
-  }
+}
+
+// Strided sweep of brick `tile` (TL adjacent lines).  MODE 0: forward,
+// 1: inverse, 2: forward * prefac * inverse (x axis only) with the float32
+// multiplier, 3: the same with the exact-symbol float64 multiplier, 4: the
+// same with the exponential Euler multiplier.
+template <typename T, int N, int MODE>
+__device__ __forceinline__ void strided_tile(cpx<T>* __restrict__ spec, int64_t nlines, const LineMap& lm,
+                                             const cpx<T>* __restrict__ twg, const Prefac& pf, int64_t tile,
+                                             int tid, unsigned char* smem) {
+  using G = SGeo<T, N, MODE>;
+  constexpr int TL = G::TL, E = N / G::TPL;
+  cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* lds = tw + N;
+  const int t = tid % TL, tl = tid / TL;
+  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
+  BrickIo<T, N, MODE> io;
+  io.setup(spec, nlines, lm, (uint32_t)tile, t, tl);
+  cpx<T> v[E];
+  io.load(v);
+  __syncthreads();
+  brick_transform<T, N, MODE>(v, brick_layout<T, N, MODE>(lds, t), tl, tw, io.bl, pf);
+  io.store(v);
 }
 
 }  // namespace evx
