@@ -470,11 +470,18 @@ int check_epi(const evx_epilogue* e, const void* u, const void* out, Epi* ep) {
   return EVX_OK;
 }
 
+// Blocks per launch of the single-field streaming kernels (about four per CU,
+// i.e. one resident round: longer x-chunks, fewer warm-up planes), and of
+// the two-species kernel, whose blocks hold twice the LDS and are better
+// served by two rounds of shorter chunks (measured on MI355X at 256^3).
+constexpr int64_t XCHUNK_BLOCKS = 1024;
+constexpr int64_t XCHUNK_BLOCKS_TWO_SPECIES = 2048;
+
 template <typename T>
-dim3 stencil_grid(const Dims& d, TileGrid* tg) {
+dim3 stencil_grid(const Dims& d, TileGrid* tg, int64_t target = XCHUNK_BLOCKS) {
   constexpr int BY = NTY * TileCfg<T>::PY;
   const int64_t ty = (d.ny + BY - 1) / BY, tzn = (d.nz + BZ - 1) / BZ;
-  tg->xc = pick_xchunk(d, ty * tzn);
+  tg->xc = pick_xchunk(d, ty * tzn, target);
   const int64_t nxc = (d.nx + tg->xc - 1) / tg->xc;
   tg->ntz = (int)tzn;
   tg->nty = (int)ty;
@@ -498,7 +505,7 @@ int launch_gs(const void* u, void* out, const Dims& d, const Bc& bc, const evx_g
   TileGrid tg;
   Dims d1 = d;
   d1.c = 1;  // one block covers both species of its tile
-  dim3 grid = stencil_grid<T>(d1, &tg);
+  dim3 grid = stencil_grid<T>(d1, &tg, XCHUNK_BLOCKS_TWO_SPECIES);
   GsCoef<T> k{static_cast<T>(Da), static_cast<T>(Db), static_cast<T>(feed), static_cast<T>(kill)};
   hipLaunchKernelGGL(gs_rhs_kernel<T>, grid, dim3(BZ, NTY), 0, s, static_cast<const T*>(u),
                      static_cast<T*>(out), d1, bc, make_geo<T>(g), k, ep, tg);

@@ -271,10 +271,9 @@ __device__ __forceinline__ TileOrigin tile_origin(const TileGrid& tg, const Dims
   return o;
 }
 
-__host__ inline int pick_xchunk(const Dims& d, int64_t tiles_yz) {
-  // Enough blocks to fill 256 CUs a few times over, but at least 8 planes
+__host__ inline int pick_xchunk(const Dims& d, int64_t tiles_yz, int64_t target = 2048) {
+  // About `target` blocks (a few per CU of the 256), but at least 8 planes
   // per block so the warm-up planes stay a small fraction of the traffic.
-  const int64_t target = 2048;
   int64_t chunks = (target + tiles_yz * d.c - 1) / (tiles_yz * d.c);
   if (chunks < 1) chunks = 1;
   int64_t xc = (d.nx + chunks - 1) / chunks;
