@@ -384,7 +384,11 @@ struct SGeo {
   // the threads per line): the forward*prefac*inverse sweeps stay off the
   // register limit, and the one-way sweeps reach eight waves per SIMD.
   static constexpr int E0 = pick_strided_e(N, FftSpec<N>::L, TL0);
-  static constexpr int E = (N >= 512 && E0 >= 2 * FftSpec<N>::L) ? E0 / 2 : E0;
+  static constexpr int E1 = (N >= 512 && E0 >= 2 * FftSpec<N>::L) ? E0 / 2 : E0;
+  // The float32 512-point forward*multiplier*inverse sweep halves it once
+  // more (eight values, 1024 threads per brick): 56 VGPRs, eight waves per
+  // SIMD instead of four, 2.7% faster on MI355X.
+  static constexpr int E = (MODE == 2 && N == 512 && sizeof(T) == 4 && E1 >= 2 * FftSpec<N>::L) ? E1 / 2 : E1;
   static constexpr int TPL = N / E;
   static constexpr int TL = TL0 > 256 / TPL ? TL0 : 256 / TPL;
   static constexpr int THREADS = TL * TPL;
