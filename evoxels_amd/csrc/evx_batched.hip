@@ -236,9 +236,9 @@ namespace {
 
 void free_plan(evx_batch_plan* p) {
   if (!p) return;
-  hipFree(p->d_inst);
-  hipFree(p->d_items);
-  hipFree(p->d_tw);
+  (void)hipFree(p->d_inst);
+  (void)hipFree(p->d_items);
+  (void)hipFree(p->d_tw);
   delete p;
 }
 
@@ -279,7 +279,7 @@ int build_items(evx_batch_plan* p) {
   }
   p->stage_first[NSTAGE] = (int)items.size();
   p->n_items = (int)items.size();
-  hipFree(p->d_items);
+  (void)hipFree(p->d_items);
   p->d_items = nullptr;
   if (hipMalloc(&p->d_items, sizeof(int4) * items.size()) != hipSuccess)
     return fail(EVX_ERR_NOMEM, "hipMalloc of the batched item list failed");
@@ -481,8 +481,11 @@ extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* 
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int cands[] = {4, 8, 16, 32, 64};
   hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+  if (hipEventCreate(&e0) != hipSuccess) return fail(EVX_ERR_HIP, "hipEventCreate failed");
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
     return fail(EVX_ERR_HIP, "hipEventCreate failed");
+  }
   double best = 1e300;
   int pick = p->xchunk;
   int st = EVX_OK;
@@ -490,20 +493,20 @@ extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* 
     if (hipStreamSynchronize(s) != hipSuccess) { st = fail(EVX_ERR_HIP, "stream synchronisation failed"); break; }
     if ((st = evx_batch_set_tiles(p, c)) != EVX_OK) break;
     if ((st = launch_step(p, scratch_state, workspace, s)) != EVX_OK) break;
-    hipEventRecord(e0, s);
+    if (hipEventRecord(e0, s) != hipSuccess) { st = fail(EVX_ERR_HIP, "hipEventRecord failed"); break; }
     for (int i = 0; i < steps && st == EVX_OK; ++i) st = launch_step(p, scratch_state, workspace, s);
     if (st != EVX_OK) break;
-    hipEventRecord(e1, s);
+    if (hipEventRecord(e1, s) != hipSuccess) { st = fail(EVX_ERR_HIP, "hipEventRecord failed"); break; }
     if (hipEventSynchronize(e1) != hipSuccess) { st = fail(EVX_ERR_HIP, "event synchronisation failed"); break; }
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { st = fail(EVX_ERR_HIP, "hipEventElapsedTime failed"); break; }
     if (ms / steps < best) {
       best = ms / steps;
       pick = c;
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (st != EVX_OK) return st;
   EVX_TRY(evx_batch_set_tiles(p, pick));
   if (best_ms) *best_ms = best;
