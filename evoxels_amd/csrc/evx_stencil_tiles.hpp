@@ -312,7 +312,7 @@ struct ChTile {
 // ChTile<T>::LDS_BYTES of LDS.  The contracted flavour (!STRICT) serves the
 // fully periodic semi-implicit path only: it has no ghost logic, loads
 // through PeriodicLoader and stores the plain right-hand side with 32-bit
-// in-plane offsets.
+// in-plane offsets whatever ep says (its callers pass EVX_OUT_RHS).
 template <typename T, bool STRICT>
 __device__ __forceinline__ void ch_rhs_tile(const T* __restrict__ u, T* __restrict__ out, const Dims& d,
                                             const Bc& bc, const Geo<T>& g, ChCoef<T> k, const Epi& ep,
@@ -486,7 +486,6 @@ __device__ __forceinline__ void ch_rhs_tile(const T* __restrict__ u, T* __restri
   // plain right-hand-side stores of the periodic flavour: 32-bit byte offset
   // of this thread's first row in the plane, rows nz apart
   const bool full_tile = y0 + BY <= d.ny && z0 + BZ <= d.nz;
-  const bool rhs_only = PER && ep.mode == EVX_OUT_RHS;
   const uint32_t obyte0 = static_cast<uint32_t>(((y0 + r0) * d.nz + gz) * (int64_t)sizeof(T));
   const uint32_t orow_bytes = static_cast<uint32_t>(d.nz * (int64_t)sizeof(T));
   for (int64_t x = x0; x < x1; ++x) {
@@ -530,7 +529,7 @@ __device__ __forceinline__ void ch_rhs_tile(const T* __restrict__ u, T* __restri
       const T r = Op::div(carry[q], fxp, fy[q], fy[q + 1], fzm, fzp, g, k.D);
       carry[q] = fxp;
       const int64_t gy = y0 + r0 + q;
-      if (rhs_only) {
+      if constexpr (PER) {  // the contracted flavour's callers all take the plain right-hand side
         if (full_tile || (gy < d.ny && gz < d.nz))
           *reinterpret_cast<T*>(reinterpret_cast<char*>(oc + x * plane) + (obyte0 + q * orow_bytes)) = r;
       } else if (gy < d.ny && gz < d.nz) {
