@@ -470,12 +470,32 @@ int check_epi(const evx_epilogue* e, const void* u, const void* out, Epi* ep) {
   return EVX_OK;
 }
 
-// Blocks per launch of the single-field streaming kernels (about four per CU,
-// i.e. one resident round: longer x-chunks, fewer warm-up planes), and of
-// the two-species kernel, whose blocks hold twice the LDS and are better
-// served by two rounds of shorter chunks (measured on MI355X at 256^3).
-constexpr int64_t XCHUNK_BLOCKS = 1024;
+// Blocks per launch.  The single-field streaming kernels aim at one resident
+// round, as many blocks as the device holds at once (blocks per CU from the
+// occupancy query for the kernel, times the CU count): the longest x-chunks,
+// the fewest warm-up planes and no tail round.  The two-species kernel,
+// whose blocks hold twice the LDS, is better served by about two rounds of
+// shorter chunks (measured on MI355X at 256^3).
+constexpr int64_t XCHUNK_BLOCKS = 1024;  // fallback when the occupancy query fails
 constexpr int64_t XCHUNK_BLOCKS_TWO_SPECIES = 2048;
+
+template <typename K>
+int64_t resident_round(K kernel) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, NT, 0) != hipSuccess || per < 1 || cus < 1)
+    return XCHUNK_BLOCKS;
+  return (int64_t)per * cus;
+}
+
+// Cached per kernel instantiation (the first launch queries the device; all
+// devices of a process are taken to be alike).
+#define EVX_RESIDENT_ROUND(kernel)                                   \
+  ([]() {                                                            \
+    static const int64_t r = resident_round(kernel);                 \
+    return r;                                                        \
+  }())
 
 template <typename T>
 dim3 stencil_grid(const Dims& d, TileGrid* tg, int64_t target = XCHUNK_BLOCKS) {
@@ -493,7 +513,7 @@ template <typename T>
 int launch_rd(const void* u, void* out, const Dims& d, const Bc& bc, const evx_grid* g, double D,
               const Epi& ep, hipStream_t s) {
   TileGrid tg;
-  dim3 grid = stencil_grid<T>(d, &tg);
+  dim3 grid = stencil_grid<T>(d, &tg, EVX_RESIDENT_ROUND(rd_rhs_kernel<T>));
   hipLaunchKernelGGL(rd_rhs_kernel<T>, grid, dim3(BZ, NTY), 0, s, static_cast<const T*>(u),
                      static_cast<T*>(out), d, bc, make_geo<T>(g), static_cast<T>(D), ep, tg);
   return check_launch("rd_rhs_kernel");
@@ -519,7 +539,7 @@ int launch_ch(const void* u, void* out, const Dims& d, const Bc& bc, const evx_g
               double D, const Epi& ep, hipStream_t s, const double* chan = nullptr, int64_t xa = 0,
               int64_t xb = -1) {
   TileGrid tg;
-  dim3 grid = stencil_grid<T>(d, &tg);
+  dim3 grid = stencil_grid<T>(d, &tg, EVX_RESIDENT_ROUND((ch_rhs_kernel<T, STRICT>)));
   unsigned base = 0;
   if (xb >= 0) {  // x-slab [xa, xb) of a one-channel field, at tile-chunk granularity
     tg.xc = SLAB_XCHUNK;
@@ -539,7 +559,7 @@ template <typename T>
 int launch_ac(const void* u, void* out, const Dims& d, const Bc& bc, const evx_grid* g, double eps,
               double gab, double M, double force, double curv, const Epi& ep, hipStream_t s) {
   TileGrid tg;
-  dim3 grid = stencil_grid<T>(d, &tg);
+  dim3 grid = stencil_grid<T>(d, &tg, EVX_RESIDENT_ROUND(ac_rhs_kernel<T>));
   AcCoef<T> k{static_cast<T>(18.0 / eps), static_cast<T>(eps), static_cast<T>(curv),
               static_cast<T>(1.0 - curv), static_cast<T>(gab), static_cast<T>(M),
               static_cast<T>(force), static_cast<T>(3.0 / eps)};
