@@ -470,14 +470,12 @@ int check_epi(const evx_epilogue* e, const void* u, const void* out, Epi* ep) {
   return EVX_OK;
 }
 
-// Blocks per launch.  The single-field streaming kernels aim at one resident
-// round, as many blocks as the device holds at once (blocks per CU from the
-// occupancy query for the kernel, times the CU count): the longest x-chunks,
-// the fewest warm-up planes and no tail round.  The two-species kernel,
-// whose blocks hold twice the LDS, is better served by about two rounds of
-// shorter chunks (measured on MI355X at 256^3).
+// Blocks per launch of the streaming kernels: one resident round, as many
+// blocks as the device holds at once (blocks per CU from the occupancy query
+// for the kernel, times the CU count).  That gives the longest x-chunks, the
+// fewest warm-up planes and no partial tail round; a fractional number of
+// rounds leaves CUs idle at the end (measured on MI355X).
 constexpr int64_t XCHUNK_BLOCKS = 1024;  // fallback when the occupancy query fails
-constexpr int64_t XCHUNK_BLOCKS_TWO_SPECIES = 2048;
 
 template <typename K>
 int64_t resident_round(K kernel) {
@@ -525,7 +523,7 @@ int launch_gs(const void* u, void* out, const Dims& d, const Bc& bc, const evx_g
   TileGrid tg;
   Dims d1 = d;
   d1.c = 1;  // one block covers both species of its tile
-  dim3 grid = stencil_grid<T>(d1, &tg, XCHUNK_BLOCKS_TWO_SPECIES);
+  dim3 grid = stencil_grid<T>(d1, &tg, EVX_RESIDENT_ROUND(gs_rhs_kernel<T>));
   GsCoef<T> k{static_cast<T>(Da), static_cast<T>(Db), static_cast<T>(feed), static_cast<T>(kill)};
   hipLaunchKernelGGL(gs_rhs_kernel<T>, grid, dim3(BZ, NTY), 0, s, static_cast<const T*>(u),
                      static_cast<T*>(out), d1, bc, make_geo<T>(g), k, ep, tg);
