@@ -8,7 +8,9 @@ library is missing or cannot be loaded, every entry point raises.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
+import gc
 import os
 from pathlib import Path
 
@@ -211,3 +213,18 @@ def make_epilogue(mode=OUT_RHS, coef=0.0, base=None, acc_in=None, acc_out=None) 
     e.acc_in = acc_in.data_ptr() if acc_in is not None else None
     e.acc_out = acc_out.data_ptr() if acc_out is not None else None
     return e
+
+
+@contextlib.contextmanager
+def gc_paused():
+    """No automatic garbage collection inside the block.  Used around HIP
+    graph captures: a collected reference cycle may run a finalizer that
+    frees device or pinned memory (an FFT plan's tables), and such a call
+    invalidates the capture.  torch.cuda.graph itself collects on entry."""
+    was_on = gc.isenabled()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was_on:
+            gc.enable()

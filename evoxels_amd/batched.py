@@ -104,7 +104,7 @@ class BatchedCahnHilliard:
         side = torch.cuda.Stream(device=self.device)
         side.wait_stream(torch.cuda.current_stream(self.device))
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g, stream=side):
+        with _lib.gc_paused(), torch.cuda.graph(g, stream=side):
             self.step(state, n_steps)
         return g
 

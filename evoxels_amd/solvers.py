@@ -21,7 +21,6 @@ those of the synchronous loop; only their timing differs.
 """
 from __future__ import annotations
 
-import gc
 import sys
 import warnings
 from abc import ABC, abstractmethod
@@ -33,6 +32,7 @@ from typing import Any, Callable, Type
 import numpy as np
 import torch
 
+from . import _lib
 from .problem_definition import ODE
 from .timesteppers import TimeStepper
 from .vtk_io import AsyncWriter
@@ -258,21 +258,11 @@ class TimeDependentSolver(BaseSolver):
                 step(0.0, self._graph_in.clone())
             torch.cuda.current_stream(u.device).wait_stream(s)
             self._graph = torch.cuda.CUDAGraph()
-            # No automatic garbage collection inside the capture: a collected
-            # cycle may run a finalizer that frees device or pinned memory (an
-            # FFT plan's tables, a previous solver's buffers), and such a call
-            # invalidates the capture.  torch.cuda.graph collects on entry.
-            gc_on = gc.isenabled()
-            gc.disable()
-            try:
-                with torch.cuda.graph(self._graph):
-                    v = self._graph_in
-                    for _ in range(n):
-                        v = step(0.0, v)
-                    self._graph_out = v
-            finally:
-                if gc_on:
-                    gc.enable()
+            with _lib.gc_paused(), torch.cuda.graph(self._graph):
+                v = self._graph_in
+                for _ in range(n):
+                    v = step(0.0, v)
+                self._graph_out = v
             self._graph_key = key
         self._graph_in.copy_(u)
         self._graph.replay()

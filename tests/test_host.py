@@ -148,3 +148,27 @@ def test_batched_plan_rejects_unsupported_shapes_and_dtype_before_any_gpu_work()
         == _lib.EVX_ERR_UNSUPPORTED
     assert lib.evx_batch_plan_create(C.byref(h), _lib.EVX_F32, 0, shp.ctypes.data, sp.ctypes.data) \
         == _lib.EVX_ERR_INVALID
+
+
+def test_gc_paused_restores_the_collector_state():
+    """Graph captures run with automatic garbage collection paused; the
+    previous state comes back afterwards, also after an exception."""
+    import gc
+    from evoxels_amd import _lib
+    assert gc.isenabled()
+    with _lib.gc_paused():
+        assert not gc.isenabled()
+    assert gc.isenabled()
+    try:
+        with _lib.gc_paused():
+            raise KeyError("inside")
+    except KeyError:
+        pass
+    assert gc.isenabled()
+    gc.disable()
+    try:
+        with _lib.gc_paused():
+            assert not gc.isenabled()
+        assert not gc.isenabled()
+    finally:
+        gc.enable()
