@@ -100,13 +100,16 @@ class BatchedCahnHilliard:
         return state
 
     def capture(self, state, n_steps: int):
-        """A graph whose replay advances ``state`` by ``n_steps`` steps."""
+        """A graph whose replay advances ``state`` by ``n_steps`` steps.  It
+        holds the tile schedule of the moment: after ``set_tiles`` or
+        ``autotune`` change it, ``replay`` raises instead of running the old
+        launches against the new item list."""
         side = torch.cuda.Stream(device=self.device)
         side.wait_stream(torch.cuda.current_stream(self.device))
         g = torch.cuda.CUDAGraph()
         with _lib.gc_paused(), torch.cuda.graph(g, stream=side):
             self.step(state, n_steps)
-        return g
+        return BatchGraph(self, g, self._generation)
 
     # ------------------------------------------------------------ tiles
     @property
@@ -115,14 +118,19 @@ class BatchedCahnHilliard:
         _lib.check(_lib.load().evx_batch_get_tiles(self.handle, C.byref(x), C.byref(ni)), "evx_batch_get_tiles")
         return {"launches_per_step": 6, "xchunk": x.value, "items": ni.value}
 
+    _generation = 0
+
     def set_tiles(self, xchunk=8):
         torch.cuda.synchronize(self.device)
+        if int(xchunk) != self.schedule["xchunk"]:
+            self._generation += 1
         _lib.check(_lib.load().evx_batch_set_tiles(self.handle, int(xchunk)), "evx_batch_set_tiles")
 
     def autotune(self, state, steps: int = 5):
         """Times the candidate tile shapes on a copy of ``state`` and keeps the
         fastest; returns the choice with its time per step."""
         scratch = state.clone()
+        self._generation += 1
         best = C.c_double()
         _lib.check(_lib.load().evx_batch_autotune(self.handle, _lib.ptr(scratch), _lib.ptr(self.ws), int(steps),
                                                   _lib.stream_handle(state.device), C.byref(best)),
@@ -130,3 +138,23 @@ class BatchedCahnHilliard:
         out = self.schedule
         out["ms_per_step"] = best.value
         return out
+
+
+class BatchGraph:
+    """A captured batch of steps (``BatchedCahnHilliard.capture``).  ``replay``
+    checks that the plan's tile schedule is still the one recorded."""
+
+    def __init__(self, runner: BatchedCahnHilliard, graph, generation: int):
+        self.runner = runner
+        self.graph = graph
+        self.generation = generation
+
+    @property
+    def stale(self) -> bool:
+        return self.generation != self.runner._generation
+
+    def replay(self):
+        if self.stale:
+            raise RuntimeError("the tile schedule changed (set_tiles/autotune) after this graph was captured; "
+                               "capture again")
+        self.graph.replay()

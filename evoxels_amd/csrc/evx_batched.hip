@@ -223,6 +223,7 @@ struct evx_batch_plan {
   // schedule
   int xchunk = 8;   // x planes per CH tile
   int n_items = 0;
+  int items_cap = 0;  // capacity of d_items: the item count of the finest tiling (one x plane per CH tile)
   int stage_first[evx::NSTAGE + 1] = {0};
   bool params_set = false;
   // device
@@ -279,10 +280,22 @@ int build_items(evx_batch_plan* p) {
   }
   p->stage_first[NSTAGE] = (int)items.size();
   p->n_items = (int)items.size();
-  (void)hipFree(p->d_items);
-  p->d_items = nullptr;
-  if (hipMalloc(&p->d_items, sizeof(int4) * items.size()) != hipSuccess)
-    return fail(EVX_ERR_NOMEM, "hipMalloc of the batched item list failed");
+  if (p->d_items == nullptr) {
+    // Allocated once, for the finest tiling, and kept until the plan is
+    // destroyed: a later evx_batch_set_tiles rewrites the list in place, so a
+    // graph captured before it never reads freed memory (it does replay the
+    // old schedule against the new list, which the caller must not do).
+    int64_t cap = (int64_t)items.size();
+    for (int i = 0; i < p->n; ++i) {
+      const BInst& b = p->inst[i];
+      cap += (int64_t)b.tg.ntz * b.tg.nty * (b.nx - b.tg.nxc);
+    }
+    if (cap > (1 << 30)) return fail(EVX_ERR_UNSUPPORTED, "too many tiles in one sweep");
+    if (hipMalloc(&p->d_items, sizeof(int4) * cap) != hipSuccess)
+      return fail(EVX_ERR_NOMEM, "hipMalloc of the batched item list failed");
+    p->items_cap = (int)cap;
+  }
+  if (p->n_items > p->items_cap) return fail(EVX_ERR_INVALID, "batched item list exceeds its capacity");
   if (hipMemcpy(p->d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice) != hipSuccess)
     return fail(EVX_ERR_HIP, "upload of the batched item list failed");
   if (hipMemcpy(p->d_inst, p->inst.data(), sizeof(BInst) * p->n, hipMemcpyHostToDevice) != hipSuccess)

@@ -122,6 +122,14 @@ class ODE(ABC):
     # ---- fused-kernel protocol used by the time steppers -----------------
     fused = False
 
+    def _own_rhs(self, cls) -> bool:
+        """True while ``rhs`` is ``cls.rhs`` itself: neither replaced on the
+        instance (the manufactured-solution driver assigns ``problem.rhs``,
+        utils.py:285) nor overridden by a subclass.  A fused step evaluates the
+        built-in right-hand side in its kernel, so it must not run when the
+        user's ``rhs`` is another function."""
+        return "rhs" not in vars(self) and type(self).rhs is cls.rhs
+
     def launch(self, u, out, epilogue):  # pragma: no cover - overridden
         raise NotImplementedError
 
@@ -201,8 +209,12 @@ class ReactionDiffusion(SemiLinearODE):
         self.initialize_boundary_conditions()
 
     @property
-    def fused(self):
+    def _kernel_ok(self):
         return _is_default(self.f) and self._kernel_bcs()
+
+    @property
+    def fused(self):
+        return self._kernel_ok and self._own_rhs(ReactionDiffusion)
 
     @property
     def order(self):
@@ -231,7 +243,7 @@ class ReactionDiffusion(SemiLinearODE):
 
     def rhs(self, t, u):
         u = self._check_state(u)
-        if self.fused:
+        if self._kernel_ok:
             out = torch.empty_like(u)
             self.launch(u, out, _lib.make_epilogue())
             return out
@@ -256,8 +268,12 @@ class CahnHilliard(SemiLinearODE):
             self.mu_hom = _default(lambda c, lib=None: 18 / self.eps * c * (1 - c) * (1 - 2 * c))
 
     @property
-    def fused(self):
+    def _kernel_ok(self):
         return _is_default(self.mu_hom) and self.vg.convention == "cell_center"
+
+    @property
+    def fused(self):
+        return self._kernel_ok and self._own_rhs(CahnHilliard)
 
     @property
     def order(self):
@@ -288,7 +304,7 @@ class CahnHilliard(SemiLinearODE):
 
     def rhs(self, t, c):
         c = self._check_state(c)
-        if self.fused:
+        if self._kernel_ok:
             out = torch.empty_like(c)
             self.launch(c, out, _lib.make_epilogue())
             return out
@@ -323,8 +339,12 @@ class TwoPhaseAllenCahn(SemiLinearODE):
             self.potential = _default(lambda u, lib=None: 18 / self.eps * u * (1 - u) * (1 - 2 * u))
 
     @property
-    def fused(self):
+    def _kernel_ok(self):
         return _is_default(self.potential) and self._kernel_bcs()
+
+    @property
+    def fused(self):
+        return self._kernel_ok and self._own_rhs(TwoPhaseAllenCahn)
 
     @property
     def order(self):
@@ -362,7 +382,7 @@ class TwoPhaseAllenCahn(SemiLinearODE):
 
     def rhs(self, t, phi):
         phi = self._check_state(phi)
-        if self.fused:
+        if self._kernel_ok:
             out = torch.empty_like(phi)
             self.launch(phi, out, _lib.make_epilogue())
             return out
@@ -519,8 +539,12 @@ class CoupledReactionDiffusion(SemiLinearODE):
         return self._fourier_symbol
 
     @property
-    def fused(self):
+    def _kernel_ok(self):
         return _is_default(self.interaction) and self._kernel_bcs()
+
+    @property
+    def fused(self):
+        return self._kernel_ok and self._own_rhs(CoupledReactionDiffusion)
 
     def _eval_interaction(self, u, lib):
         try:
@@ -547,7 +571,7 @@ class CoupledReactionDiffusion(SemiLinearODE):
         u = self._check_state(u)
         if u.shape[0] != 2:
             raise ValueError(f"CoupledReactionDiffusion needs two channels (A, B), got {u.shape[0]}")
-        if self.fused:
+        if self._kernel_ok:
             out = torch.empty_like(u)
             self.launch(u, out, _lib.make_epilogue())
             return out
@@ -597,6 +621,10 @@ class ReactionDiffusionSBM(ReactionDiffusion, SmoothedBoundaryODE):
                 self.bc = (("dirichlet", (x_bc[1][0] * self.mask[:, 0, :, :], x_bc[1][1] * self.mask[:, -1, :, :])),
                            y_bc, z_bc)
                 self.initialize_boundary_conditions()
+
+    @property
+    def _kernel_ok(self):
+        return False
 
     @property
     def fused(self):

@@ -167,7 +167,15 @@ class BaseSolver(ABC):
             step = self.timestepper.step
         self._capture = bool(jit) and self.timestepper is not None and \
             getattr(self.timestepper, "capturable", False)
+        # A captured graph holds the kernel arguments (dt, parameters) and the
+        # workspace of the stepper it was recorded from: a new stepper needs a
+        # new capture.
+        self._drop_graph()
         return step
+
+    def _drop_graph(self):
+        self._graph = self._graph_in = self._graph_out = self._graph_step = None
+        self._graph_key = None
 
     @abstractmethod
     def _run_loop(self, u, step, time_increment, frames, max_iters, vtk_out, verbose, plot_bounds, colormap):
@@ -248,7 +256,7 @@ class TimeDependentSolver(BaseSolver):
     def _captured(self, u, step, n):
         """Replays n steps captured in a hipGraph; returns the new state."""
         key = (n, tuple(u.shape), u.dtype)
-        if getattr(self, "_graph_key", None) != key:
+        if getattr(self, "_graph_key", None) != key or getattr(self, "_graph_step", None) is not step:
             self._graph_in = torch.empty_like(u)
             self._graph_in.copy_(u)
             # Warm-up on a copy: creates FFT plans and workspaces outside capture.
@@ -264,6 +272,8 @@ class TimeDependentSolver(BaseSolver):
                     v = step(0.0, v)
                 self._graph_out = v
             self._graph_key = key
+            # keeps the stepper, and with it the workspace the graph writes, alive
+            self._graph_step = step
         self._graph_in.copy_(u)
         self._graph.replay()
         return self._graph_out

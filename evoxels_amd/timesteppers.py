@@ -108,24 +108,37 @@ class PseudoSpectralIMEX(TimeStepper):
     def __post_init__(self):
         self.problem.verify_fft_bc_config()
         self.pad = self.problem.pad_fft_bc
+        self._ws = None
+
+    @property
+    def _kind(self):
+        """'ch' or 'rd' when the fully periodic fused pipeline applies to the
+        problem as it stands (its ``rhs`` may be replaced after construction)."""
         p = self.problem
-        self._kind = None
-        self._mirror = None
         if p.fused and p.bc_type == ("periodic", "periodic", "periodic"):
             if isinstance(p, CahnHilliard):
-                self._kind = "ch"
-            elif isinstance(p, ReactionDiffusion):
-                self._kind = "rd"
-        elif p.fused and p.bc_type[1:] == ("periodic", "periodic"):
-            # one non-periodic axis, x: the mirrored pipeline for one channel
+                return "ch"
+            if isinstance(p, ReactionDiffusion):
+                return "rd"
+        return None
+
+    @property
+    def _mirror(self):
+        """'ch' or 'rd' when the mirrored pipeline applies (x non-periodic)."""
+        p = self.problem
+        if p.fused and p.bc_type[0] != "periodic" and p.bc_type[1:] == ("periodic", "periodic"):
             staggered = p.vg.convention == "staggered_x"
             if isinstance(p, CahnHilliard) and not staggered:
-                self._mirror = "ch"
-            elif isinstance(p, ReactionDiffusion) and (not staggered or p.bc_type[0] == "dirichlet"):
-                self._mirror = "rd"
-        if self._kind is None:
+                return "ch"
+            if isinstance(p, ReactionDiffusion) and (not staggered or p.bc_type[0] == "dirichlet"):
+                return "rd"
+        return None
+
+    @property
+    def _fft_prefac_generic(self):
+        if self._fft_prefac is None:
             self._fft_prefac = self._multiplier()
-        self._ws = None
+        return self._fft_prefac
 
     _FUSED = {"ch": "evx_ch_imex_step", "rd": "evx_rd_imex_step"}
 
@@ -197,7 +210,7 @@ class PseudoSpectralIMEX(TimeStepper):
         vg = self.problem.vg
         dc = self.pad(self.problem.rhs(t, u))
         dc_fft = vg.rfftn(dc, dc.shape)
-        dc_fft.mul_(self._fft_prefac)  # the spectrum is a temporary: multiplied and transformed in place
+        dc_fft.mul_(self._fft_prefac_generic)  # the spectrum is a temporary: multiplied and transformed in place
         update = vg.irfftn(dc_fft, dc.shape, overwrite=True)[:, :u.shape[1]]
         return u + update
 

@@ -265,3 +265,77 @@ def test_inversion_model_reduces_misfit():
     err0 = np.abs(np.log(np.array([[3.6, 0.8], [2.0, 1.6]]) / truth))
     err1 = np.abs(np.log(np.stack([p["eps"].cpu().numpy(), p["D"].cpu().numpy()], 1) / truth))
     assert err1.sum() < 0.5 * err0.sum()
+
+
+def config5_draw(n=128, B=16):
+    """The inversion workload: truth (M, kappa) ~ U[0.5, 2] per trajectory,
+    D = M and eps = kappa, the evaluation point truth * U[0.8, 1.25], and
+    c0 = 0.5 + 0.05*N(0,1) per trajectory, numpy default_rng(4) in this
+    order."""
+    rng = np.random.default_rng(4)
+    truth_mk = rng.uniform(0.5, 2.0, (B, 2))
+    truth = np.stack([truth_mk[:, 1], truth_mk[:, 0]], 1)
+    guess = truth * rng.uniform(0.8, 1.25, (B, 2))
+    c0 = 0.5 + 0.05 * rng.standard_normal((B, n, n, n))
+    return truth, guess, c0
+
+
+def exact_k2(n, d=1.0):
+    """|k|^2 on the rfft grid from float64 wavenumbers 2*pi*fftfreq(n, d),
+    the exact-symbol multiplier's table."""
+    k = [2 * np.pi * torch.fft.fftfreq(n, d=d, dtype=torch.float64) for _ in range(2)]
+    k.append(2 * np.pi * torch.fft.rfftfreq(n, d=d, dtype=torch.float64))
+    kx, ky, kz = torch.meshgrid(*k, indexing="ij")
+    return kx ** 2 + ky ** 2 + kz ** 2
+
+
+def test_config5_two_trajectories_128_adjoint_matches_oracle_autograd():
+    """Trajectories 0 and 1 of the inversion workload (trajectory 1 has the
+    smallest eps of the draw, 0.57, where the clip of CahnHilliard.rhs is
+    active) on 128^3 in float64 with the exact-symbol multiplier: 40 steps,
+    observations every 8 steps, checkpoints every 20.  The HIP adjoint
+    gradient against torch autograd of the CPU oracle (the same discrete
+    function, clip included, float64 wavenumbers), recomputed segment by
+    segment with torch.utils.checkpoint; relative to each trajectory's
+    largest gradient component, <= 1e-8."""
+    import torch.utils.checkpoint as tuc
+    n, dt, steps, ce = 128, 0.01, 40, 20
+    obs_steps = [8, 16, 24, 32, 40]
+    truth, guess, c0 = config5_draw(n)
+    pick = [0, 1]
+    truth, guess, c0 = truth[pick], guess[pick], c0[pick]
+    assert guess[1, 0] < 0.6
+    vf, vg = grid_for((n, n, n), "float64")
+    eng = CHImexEngine(vg, 2, dt, exact_symbol=True)
+    u0 = torch.tensor(c0, dtype=torch.float64, device="cuda")
+    obs = solve(eng, u0, truth, steps, obs_steps)
+    states = solve(eng, u0, guess, steps, obs_steps)
+    clipped = [int(((s < 0) | (s > 1)).sum(dim=(1, 2, 3))[1]) for s in states]
+    print(f"trajectory 1 (eps {guess[1, 0]:.3f}): voxels outside [0, 1] at the observations {clipped}", flush=True)
+    loss, grad = loss_and_grad(eng, u0, guess, steps, obs_steps, obs, checkpoint_every=ce)
+    grad = grad.cpu().numpy()
+    k2 = exact_k2(n)
+    og = O.OracleGrid((n, n, n), (1.0, 1.0, 1.0), "float64")
+    for c in range(2):
+        e64 = torch.tensor(guess[c, 0], dtype=torch.float64, requires_grad=True)
+        d64 = torch.tensor(guess[c, 1], dtype=torch.float64, requires_grad=True)
+        ob = [o[c:c + 1].cpu() for o in obs]
+
+        def segment(v, e, d):
+            pre = O.imex_prefac(dt, O.ch_symbol(k2, e, d))
+            for _ in range(8):
+                v = O.imex_step(v, lambda t, w: O.ch_rhs(w, og, P3, e, d), pre)
+            return v
+
+        v = torch.tensor(c0[c:c + 1], dtype=torch.float64)
+        J = 0
+        for s in range(0, steps, 8):  # checkpointed segments of 8 steps: one per observation
+            v = tuc.checkpoint(segment, v, e64, d64, use_reentrant=False)
+            J = J + 0.5 * ((v - ob[obs_steps.index(s + 8)]) ** 2).sum()
+        ge, gd = torch.autograd.grad(J, (e64, d64))
+        want = np.array([float(ge), float(gd)])
+        err = np.abs(grad[c] - want).max() / np.abs(want).max()
+        print(f"trajectory {c}: adjoint {grad[c]}, oracle autograd {want}, rel {err:.3e}; "
+              f"loss {float(loss[c]):.6e} vs {float(J):.6e}", flush=True)
+        assert abs(float(loss[c]) - float(J)) <= 1e-10 * float(J)
+        assert err <= 1e-8

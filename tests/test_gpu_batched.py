@@ -127,3 +127,64 @@ def test_batched_grid_against_float64_oracle():
         a32 = a.astype(np.float32).astype(np.float64)
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
         assert np.linalg.norm((got - a32) - (ref - a)) / np.linalg.norm(ref - a) < 5e-4
+
+
+def config4_draw():
+    """The 48-grid workload: edges from {32, 48, 64, 96, 128} per axis and
+    c0 = U[0.3, 0.7] + 0.05*N(0,1) per grid, numpy default_rng(3), in this
+    order; dt 0.1, eps 3, D 1, unit spacing."""
+    rng = np.random.default_rng(3)
+    shapes = [tuple(int(v) for v in s) for s in rng.choice([32, 48, 64, 96, 128], (48, 3))]
+    arrs = [rng.uniform(0.3, 0.7) + 0.05 * rng.standard_normal(s) for s in shapes]
+    return shapes, arrs
+
+
+def test_config4_48_uneven_grids_three_steps():
+    """The 48-grid draw for three autotuned, graph-replayed steps: every grid
+    against the single-grid fused step, and three grids (one with a 128
+    edge) against three steps of the float64 oracle (field relative L2
+    <= 1e-5, update <= 5e-4 per step)."""
+    shapes, arrs = config4_draw()
+    assert len(shapes) == 48 and any(128 in s for s in shapes)
+    b = BatchedCahnHilliard(shapes, 0.1, eps=3.0, D=1.0)
+    state = b.pack(arrs)
+    sched = b.autotune(state, steps=2)
+    print(f"config 4: {len(shapes)} grids, schedule {sched}", flush=True)
+    g = b.capture(state, 3)
+    g.replay()
+    torch.cuda.synchronize()
+    worst = 0.0
+    for i, (v, a, s) in enumerate(zip(b.views(state), arrs, shapes)):
+        vf = evx.VoxelFields(s, s)
+        vg = VoxelGridHip(vf.grid_info(), precision="float32", device="cuda")
+        st = PseudoSpectralIMEX(CahnHilliard(vg, eps=3.0, D=1.0), 0.1)
+        u = vg.init_scalar_field(a).contiguous()
+        for _ in range(3):
+            st.step_into(u, u)
+        d = float((v - u[0]).abs().max() / u[0].abs().max())
+        worst = max(worst, d)
+    print(f"config 4: largest batched vs single-grid difference {worst:.3e}", flush=True)
+    assert worst < 2e-6
+    P3 = (("periodic", None),) * 3
+    pick = [next(i for i, s in enumerate(shapes) if 128 in s)]
+    pick += [i for i in (0, 47) if i not in pick][:2]
+    views = b.views(state)
+    for i in pick:
+        s, a = shapes[i], arrs[i]
+        og = O.OracleGrid(s, (1.0, 1.0, 1.0), "float64")
+        pre = O.imex_prefac(0.1, O.ch_symbol(O.rfft_k_squared(s, og.spacing), 3.0, 1.0))
+        ref = og.field(a.astype(np.float32).astype(np.float64))
+        bb = BatchedCahnHilliard([s], 0.1, eps=3.0, D=1.0)
+        st1 = bb.pack([a])
+        for k in range(3):
+            prev = ref
+            ref = O.imex_step(ref, lambda t, x: O.ch_rhs(x, og, P3, 3.0, 1.0), pre)
+            prev_got = bb.views(st1)[0].cpu().double().numpy()
+            bb.step(st1)
+            got = bb.views(st1)[0].cpu().double().numpy()
+            r = ref[0].numpy()
+            ferr = np.linalg.norm(got - r) / np.linalg.norm(r)
+            uerr = np.linalg.norm((got - prev_got) - (r - prev[0].numpy())) / np.linalg.norm(r - prev[0].numpy())
+            print(f"config 4 grid {i} {s} step {k + 1}: field {ferr:.3e} update {uerr:.3e}", flush=True)
+            assert ferr <= 1e-5 and uerr <= 5e-4
+        np.testing.assert_allclose(got, views[i].cpu().double().numpy(), rtol=0, atol=2e-6)

@@ -22,9 +22,9 @@ import sympy.vector as spv
 import torch
 
 import evoxels_amd as evx
-from evoxels_amd.problem_definition import CoupledReactionDiffusion
+from evoxels_amd.problem_definition import CoupledReactionDiffusion, ReactionDiffusion
 from evoxels_amd.solvers import TimeDependentSolver
-from evoxels_amd.timesteppers import PseudoSpectralIMEX
+from evoxels_amd.timesteppers import ForwardEuler, PseudoSpectralIMEX
 from evoxels_amd.voxelgrid import VoxelGridHip
 
 pytestmark = pytest.mark.gpu
@@ -36,8 +36,18 @@ FUNS = (0.5 + 0.4 * sp.cos(2 * sp.pi * CS.x) * sp.cos(2 * sp.pi * CS.y) ** 3 * s
         0.4 + 0.1 * sp.sin(4 * sp.pi * CS.x) * sp.exp(-t / 10))
 
 
-def mms_errors(mode, g_powers, t_powers, t_final=1.0, precision="float32"):
-    """errors[j, k, i] for function j, time power k, grid power i."""
+# the diffusion study of the same notebook (cells 21-23, results1-results4)
+FUNS_RD = ((0.5 + 0.4 * sp.cos(4 * sp.pi * CS.x) * sp.cos(2 * sp.pi * CS.y) ** 3 * sp.cos(t * 2 * sp.pi))
+           * sp.exp(-t),)
+
+
+def mms_errors(mode, g_powers, t_powers, t_final=1.0, precision="float32", funs=FUNS,
+               make_ode=lambda vg: CoupledReactionDiffusion(vg, D_A=1.0, D_B=0.5), stepper_cls=PseudoSpectralIMEX):
+    """errors[j, k, i] for function j, time power k, grid power i.  The
+    forcing is installed the way the reference's driver does it, by assigning
+    ``ode.rhs`` on the instance (utils.py:285), so the steppers must take the
+    replaced right-hand side instead of their fused kernels."""
+    FUNS = funs
     args = (t, CS.x, CS.y, CS.z)
     u_t = [sp.lambdify(args, sp.N(sp.diff(f, t)), "torch") for f in FUNS]
     u_ex = [sp.lambdify(args, sp.N(f), "torch") for f in FUNS]
@@ -49,7 +59,7 @@ def mms_errors(mode, g_powers, t_powers, t_final=1.0, precision="float32"):
         vf.precision = precision
         dx[i] = vf.spacing[0]
         vg = VoxelGridHip(vf.grid_info(), precision=precision, device="cuda")
-        ode = CoupledReactionDiffusion(vg, D_A=1.0, D_B=0.5)
+        ode = make_ode(vg)
         rhs_orig = ode.rhs
         grid = vg.meshgrid()
         if mode == "spatial":
@@ -82,7 +92,8 @@ def mms_errors(mode, g_powers, t_powers, t_final=1.0, precision="float32"):
                 vf.add_field(f"u{j}_final", np.broadcast_to(fun(t_final, *mesh), vf.shape).copy())
                 names.append(f"u{j}")
             dt = t_final / 2 ** q
-            st = PseudoSpectralIMEX(ode, dt)
+            st = stepper_cls(ode, dt)
+            assert not ode.fused, "a replaced rhs must not run on the fused kernels"
             solver = TimeDependentSolver(vf, names, backend="hip", step_fn=st.step, device="cuda")
             solver.solve(dt, 8, int(2 ** q), None, verbose=False)
             for j in range(len(FUNS)):
@@ -111,3 +122,32 @@ def test_coupled_rd_imex_temporal_slopes_match_notebook():
     ts = [slope(dts, err[j, :, 0]) for j in range(2)]
     print("temporal slopes", ts, "notebook", KAT["temporal_slopes"], flush=True)
     np.testing.assert_allclose(ts, KAT["temporal_slopes"], atol=3e-3)
+
+
+def _rd(vg):
+    return ReactionDiffusion(vg, D=1.0)
+
+
+@pytest.mark.parametrize("stepper_cls", [PseudoSpectralIMEX, ForwardEuler])
+def test_rd_temporal_slope_with_replaced_rhs(stepper_cls):
+    """results3/results4 (diffusion, temporal mode) on 8^3 in float64 (dt
+    within forward Euler's stability limit dx^2/6): the forcing enters
+    through an instance-level ``rhs``; both first-order steppers must show
+    order 1 in time.  Dropping the forcing (the fused kernel's default
+    right-hand side) leaves an O(1) error that does not converge."""
+    _, err = mms_errors("temporal", [3], [9, 10, 11], precision="float64", funs=FUNS_RD, make_ode=_rd,
+                        stepper_cls=stepper_cls)
+    dts = np.array([1 / 512, 1 / 1024, 1 / 2048])
+    ts = slope(dts, err[0, :, 0])
+    print(stepper_cls.__name__, "temporal errors", err[0, :, 0], "slope", ts, flush=True)
+    assert err[0, -1, 0] < 1e-2
+    assert 0.9 < ts < 1.15
+
+
+def test_rd_imex_spatial_slope_with_replaced_rhs():
+    """results1 (diffusion, semi-implicit, spatial mode) on 8^3..32^3 at
+    dt = 1/1024: second order in space."""
+    dx, err = mms_errors("spatial", [3, 4, 5], [10], precision="float64", funs=FUNS_RD, make_ode=_rd)
+    g = slope(dx, err[0, -1, :])
+    print("RD IMEX spatial errors", err[0, -1, :], "slope", g, flush=True)
+    assert 1.8 < g < 2.3

@@ -103,6 +103,32 @@ def test_cahn_hilliard_solver_frames_match_oracle(jit):
         assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-12
 
 
+def test_second_solve_recaptures_with_new_dt():
+    """Two solve() calls on one solver with different time increments and
+    parameters: with jit=True each solve records its own graph (the kernel
+    arguments hold dt, eps and the stepper's workspace), so both results equal
+    the uncaptured runs."""
+    n = 20
+    c0 = 0.5 + 0.1 * np.random.default_rng(11).random((n, n, n))
+    results = {}
+    for jit in (False, True):
+        vf = evx.VoxelFields((n, n, n), (n, n, n))
+        vf.precision = "float64"
+        vf.add_field("c", c0.copy())
+        solver = TimeDependentSolver(vf, "c", backend="hip", problem_cls=CahnHilliard,
+                                     timestepper_cls=PseudoSpectralIMEX, device="cuda")
+        out = []
+        for dt, eps in ((0.1, 3.0), (0.05, 2.0), (0.1, 3.0)):
+            vf.set_field("c", c0.copy())
+            solver.solve(time_increment=dt, frames=2, max_iters=6, problem_kwargs={"eps": eps}, jit=jit,
+                         verbose=False)
+            out.append(vf.fields["c"].copy())
+        results[jit] = out
+    assert not np.array_equal(results[False][0], results[False][1])
+    for a, b in zip(results[False], results[True]):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("jit", [False, True])
 def test_allen_cahn_solver_frames_match_oracle(jit):
     shape, dt, iters, frames = (20, 18, 70), 0.2, 12, 3

@@ -186,3 +186,29 @@ def test_explicit_diffusion_benchmark_config_steps():
         err = rel_l2(u.cpu(), u_ref)
         print(f"step {i + 1}: rel_l2 vs float64 oracle {err:.3e}")
         assert err < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_ch_rhs_face_mobility_kat_by_hand(axis, precision):
+    """The hand-worked CH profile of the oracle pins (0/1 values with -0.1 and
+    1.1 entries, face mobility cbar(1 - cbar), clip first) through the fused
+    HIP right-hand side, exactly, and through the semi-implicit step, whose
+    contracted periodic right-hand side must reproduce it to rounding."""
+    from tests.test_oracle_pins import ch_kat_case
+    shape, c, want = ch_kat_case(axis, other=4)
+    vf = evx.VoxelFields(shape, tuple(float(n) for n in shape))
+    vg = VoxelGridHip(vf.grid_info(), precision=precision, device="cuda")
+    prob = CahnHilliard(vg, eps=3.0, D=1.0)
+    got = prob.rhs(0.0, vg.init_scalar_field(c)).cpu()[0].double().numpy()
+    np.testing.assert_array_equal(got, want)
+    from evoxels_amd.timesteppers import PseudoSpectralIMEX
+    dt = 0.05
+    step = PseudoSpectralIMEX(prob, dt).step(0.0, vg.init_scalar_field(c)).cpu()
+    og = O.OracleGrid(shape, vf.spacing, precision)
+    pre = O.imex_prefac(dt, O.ch_symbol(O.rfft_k_squared(shape, vf.spacing), 3.0, 1.0))
+    hand = torch.tensor(want[None], dtype=og.dtype)
+    ref = O.imex_step(og.field(c), lambda t, v: hand, pre)
+    upd, upd_ref = (step - og.field(c)).double(), (ref - og.field(c)).double()
+    err = float(torch.linalg.norm(upd - upd_ref) / torch.linalg.norm(upd_ref))
+    assert err < (5e-6 if precision == "float32" else 1e-13), err

@@ -172,3 +172,44 @@ def test_gc_paused_restores_the_collector_state():
         assert not gc.isenabled()
     finally:
         gc.enable()
+
+
+def test_batch_graph_refuses_replay_after_schedule_change():
+    """A batched graph records the tile schedule it was captured with; after
+    set_tiles/autotune bump the plan's generation, replay raises instead of
+    launching the old schedule (no GPU: stand-ins for the plan and graph)."""
+    from evoxels_amd.batched import BatchGraph
+    runner = types.SimpleNamespace(_generation=3)
+    calls = []
+    g = BatchGraph(runner, types.SimpleNamespace(replay=lambda: calls.append(1)), runner._generation)
+    g.replay()
+    assert calls == [1] and not g.stale
+    runner._generation += 1
+    assert g.stale
+    with pytest.raises(RuntimeError, match="capture again"):
+        g.replay()
+    assert calls == [1]
+
+
+def test_replaced_rhs_disables_fused_steps():
+    """The manufactured-solution driver assigns ``problem.rhs`` on the
+    instance (utils.py:285); the fused kernels evaluate the built-in
+    right-hand side, so such a problem must not take them (no GPU: a
+    stand-in grid)."""
+    vg = _stub_vg()
+    for cls, kw in ((ReactionDiffusion, {"D": 1.0}), (CahnHilliard, {}), (TwoPhaseAllenCahn, {})):
+        p = cls(vg, **kw)
+        assert p.fused and p._kernel_ok
+        st = PseudoSpectralIMEX(p, 0.1) if cls is not TwoPhaseAllenCahn else None
+        if st is not None:
+            assert st._kind in ("rd", "ch") and st.capturable
+        p.rhs = lambda t, u: u
+        assert not p.fused and p._kernel_ok
+        if st is not None:  # the stepper was built before the replacement
+            assert st._kind is None and not st.capturable
+
+    class MyCH(CahnHilliard):
+        def rhs(self, t, c):
+            return super().rhs(t, c)
+
+    assert not MyCH(vg).fused

@@ -182,6 +182,50 @@ def test_ch_rhs_single_mode_linearisation():
 
 
 # ---------------------------------------------------------- normal Laplacian --
+# A 1-D periodic profile of 0/1 values with two out-of-range entries, constant
+# in the other two axes.  Worked by hand (dx = 1, eps = 3, D = 1):
+#   clip:            c  = [0, 0, 1, 1, 0, 1, 0, 1]       (-0.1 -> 0, 1.1 -> 1)
+#   c(1-c) = 0 everywhere, so mu = -2*eps*lap(c) = -6*(c[i+1] + c[i-1] - 2 c[i])
+#                    mu = [-6, -6, 6, 6, -12, 12, -12, 12]
+#   face i+1/2 mobility cbar(1-cbar), cbar = (c[i]+c[i+1])/2:
+#                    m  = [0, 1/4, 0, 1/4, 1/4, 1/4, 1/4, 1/4]
+#   flux m*(mu[i+1]-mu[i]) = [0, 3, 0, -4.5, 6, -6, 6, -4.5]
+#   rhs = flux[i+1/2] - flux[i-1/2]
+# A mobility averaged from the cells, avg(c(1-c)), is 0 on every face and
+# gives rhs = 0; a clip after the mobility gives 0.2475 on the 1|-0.1 face.
+CH_KAT_RAW = [0.0, 0.0, 1.0, 1.0, -0.1, 1.1, 0.0, 1.0]
+CH_KAT_RHS = [4.5, 3.0, -3.0, -4.5, 10.5, -12.0, 12.0, -10.5]
+
+
+def ch_kat_case(axis, other=3):
+    shape = [other, other, other]
+    shape[axis] = len(CH_KAT_RAW)
+    prof = np.asarray(CH_KAT_RAW).reshape([-1 if a == axis else 1 for a in range(3)])
+    want = np.asarray(CH_KAT_RHS).reshape([-1 if a == axis else 1 for a in range(3)])
+    return tuple(shape), np.broadcast_to(prof, shape).copy(), np.broadcast_to(want, shape).copy()
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_ch_rhs_face_mobility_and_clip_by_hand(axis):
+    """CahnHilliard.rhs (problem_definition.py:350-371): mobility
+    cbar_f(1 - cbar_f) on faces of the clipped field, against the worked
+    profile above, along each axis; D scales it, eps scales the gradient
+    term, and a spacing of 2 divides it by 2^4."""
+    shape, c, want = ch_kat_case(axis)
+    for precision in ("float64", "float32"):
+        g = grid(shape, precision=precision)
+        got = O.ch_rhs(g.field(c), g, P3, eps=3.0, D=1.0)[0].double().numpy()
+        np.testing.assert_array_equal(got, want)
+        got = O.ch_rhs(g.field(c), g, P3, eps=3.0, D=2.0)[0].double().numpy()
+        np.testing.assert_array_equal(got, 2 * want)
+    g = grid(shape)
+    np.testing.assert_allclose(O.ch_rhs(g.field(c), g, P3, eps=1.5)[0].numpy(), want / 2, rtol=0, atol=1e-13)
+    dom = [float(n) for n in shape]
+    dom[axis] *= 2
+    g2 = grid(shape, tuple(dom))
+    np.testing.assert_allclose(O.ch_rhs(g2.field(c), g2, P3)[0].numpy(), want / 16, rtol=0, atol=1e-14)
+
+
 def test_normal_laplace_planar_profile_and_bulk():
     g = grid((12, 5, 6), (12.0, 5.0, 6.0))
     x = np.arange(12) + 0.5
