@@ -63,7 +63,9 @@ def mms_errors(mode, g_powers, t_powers, t_final=1.0, precision="float32", funs=
         rhs_orig = ode.rhs
         grid = vg.meshgrid()
         if mode == "spatial":
-            rhs_an = [sp.lambdify(args, sp.N(f), "torch") for f in ode.rhs_analytic(t, FUNS)]
+            # one function: the scalar form, as utils.py:266-268 passes it
+            an = ode.rhs_analytic(t, FUNS) if len(FUNS) > 1 else (ode.rhs_analytic(t, FUNS[0]),)
+            rhs_an = [sp.lambdify(args, sp.N(f), "torch") for f in an]
 
             def mms_rhs(tt, u):
                 rhs = rhs_orig(tt, u)
