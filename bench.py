@@ -66,14 +66,28 @@ def traffic_from_profiles(kernel):
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(c0, steps):
-    """Times the CPU oracle on `steps` steps (after one warm-up step)."""
+def cpu_baseline(c0, steps, gpu_step1=None):
+    """Times the CPU oracle on `steps` steps (after one warm-up step).  With
+    ``gpu_step1`` (the HIP step of c0, on the host) one float64 oracle step
+    from the same float32 values checks it: field and update relative L2."""
     import torch
     sys.path.insert(0, str(ROOT))
     from oracle import evoxels_oracle as O
     n = c0.shape[0]
-    g = O.OracleGrid((n, n, n), (1.0, 1.0, 1.0), "float32")
     P3 = (("periodic", None),) * 3
+    parity = None
+    if gpu_step1 is not None:
+        g64 = O.OracleGrid((n, n, n), (1.0, 1.0, 1.0), "float64")
+        pre64 = O.imex_prefac(0.05, O.ch_symbol(O.rfft_k_squared((n, n, n), g64.spacing), 3.0, 1.0))
+        u0 = g64.field(c0.astype(np.float64))
+        ref = O.imex_step(u0, lambda t, v: O.ch_rhs(v, g64, P3, 3.0, 1.0), pre64)
+        got = torch.as_tensor(gpu_step1, dtype=torch.float64).reshape(ref.shape)
+        parity = {"field_rel_l2": float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref)),
+                  "update_rel_l2": float(torch.linalg.norm((got - u0) - (ref - u0)) / torch.linalg.norm(ref - u0)),
+                  "tolerance": {"field": 1e-5, "update": 5e-4},
+                  "what": "one float32 HIP step from c0 against one float64 oracle step from the same values"}
+        del g64, pre64, u0, ref, got
+    g = O.OracleGrid((n, n, n), (1.0, 1.0, 1.0), "float32")
     pre = O.imex_prefac(0.05, O.ch_symbol(O.rfft_k_squared((n, n, n), g.spacing), 3.0, 1.0))
     rhs = lambda t, v: O.ch_rhs(v, g, P3, 3.0, 1.0)  # noqa: E731
     u = g.field(c0)
@@ -92,6 +106,7 @@ def cpu_baseline(c0, steps):
         pass
     return {"value": n ** 3 * steps / dt, "unit": "voxel-steps/s", "cores": torch.get_num_threads(),
             "kind": "port", "s_per_step": dt / steps, "cpu_model": model, "os_cpu_count": os.cpu_count(),
+            "rel_l2_vs_fp64_oracle": parity,
             "sample": f"{steps} timed steps (after 1 warm-up) of the same 512^3 float32 workload, "
                       f"torch {torch.__version__} eager on CPU, os.cpu_count()={os.cpu_count()}"}
 
@@ -225,12 +240,38 @@ def inversion_leg(steps, reps, torch_reps):
     obsx = solve(engx, u64, truth, steps, obs_steps)
     _, gx = loss_and_grad(engx, u64, guess, steps, obs_steps, obsx, ce)
     gx = gx.cpu().numpy()
-    fd = np.zeros_like(gx)
-    for j in range(2):
+
+    def central(h):
+        out = np.zeros_like(gx)
+        for j in range(2):
+            pp, pm = guess.copy(), guess.copy()
+            pp[:, j] *= 1 + h
+            pm[:, j] *= 1 - h
+            out[:, j] = (traj_loss(engx, pp, obsx) - traj_loss(engx, pm, obsx)) / (2 * h * guess[:, j])
+        return out
+
+    def clip_flips(h):
+        """Per trajectory, the (voxel, step) pairs whose clip state (c < 0 or
+        c > 1 entering the right-hand side) differs between the runs at
+        eps*(1+h) and eps*(1-h): each is a kink of the loss that the central
+        difference straddles."""
         pp, pm = guess.copy(), guess.copy()
-        pp[:, j] *= 1 + 1e-4
-        pm[:, j] *= 1 - 1e-4
-        fd[:, j] = (traj_loss(engx, pp, obsx) - traj_loss(engx, pm, obsx)) / (2e-4 * guess[:, j])
+        pp[:, 0] *= 1 + h
+        pm[:, 0] *= 1 - h
+        a, b = u64.clone(), u64.clone()
+        ta, tb = torch.empty_like(a), torch.empty_like(b)
+        pa, pb = engx.params(pp), engx.params(pm)
+        flips = torch.zeros(B, dtype=torch.int64, device="cuda")
+        for _ in range(steps):
+            flips += (((a < 0) | (a > 1)) ^ ((b < 0) | (b > 1))).sum(dim=(1, 2, 3))
+            engx.step(a, pa, out=ta)
+            engx.step(b, pb, out=tb)
+            a, ta, b, tb = ta, a, tb, b
+        return flips.cpu().numpy()
+
+    fd = central(1e-4)
+    flips = clip_flips(1e-4)
+    fd_small = central(1e-10)
     del engx, obsx
     torch.cuda.empty_cache()
 
@@ -241,6 +282,7 @@ def inversion_leg(steps, reps, torch_reps):
         return float(rel_traj(a, ref).max())
 
     fd_traj = rel_traj(gx, fd)
+    fd_small_traj = rel_traj(gx, fd_small)
 
     return {"workload": f"ch_inversion_{B}x{n}^3_fp32_{steps}_steps", "trajectories": B, "grid": [n, n, n],
             "steps": steps, "dt": dt, "checkpoint_every": ce, "obs_steps": obs_steps,
@@ -257,10 +299,16 @@ def inversion_leg(steps, reps, torch_reps):
                          "fd_per_trajectory": [float(f"{v:.2e}") for v in fd_traj],
                          "trajectories_within_1e-4_of_fd": int((fd_traj <= 1e-4).sum()),
                          "eps_per_trajectory": [round(float(v), 3) for v in guess[:, 0]],
+                         "clip_flips_between_fd_runs_h_1e-4": [int(v) for v in flips],
+                         "fp64_exact_symbol_vs_central_differences_h_1e-10": rel_err(gx, fd_small),
+                         "fd_h_1e-10_per_trajectory": [float(f"{v:.2e}") for v in fd_small_traj],
+                         "trajectories_within_1e-4_of_fd_h_1e-10": int((fd_small_traj <= 1e-4).sum()),
                          "note": "relative to each trajectory's largest gradient component; central "
-                                 "differences with relative step 1e-4 in float64. Where the clip of "
-                                 "CahnHilliard.rhs is active (small eps: interfaces overshoot [0,1]) the "
-                                 "loss has kinks in eps and differences straddle them"},
+                                 "differences in float64 with relative steps 1e-4 and 1e-10. Where the clip "
+                                 "of CahnHilliard.rhs is active (small eps: interfaces overshoot [0,1]) the "
+                                 "loss has kinks in eps (clip-state flips between the two runs) and, between "
+                                 "them, curvature of order 1e15 in eps, so the 1e-4 difference is a secant; "
+                                 "the 1e-10 difference resolves the derivative"},
             "loss_rel_diff_vs_torch": loss_rel}
 
 
@@ -426,7 +474,19 @@ def configs_leg(cpu):
         "steps": 1000, "ms_per_step": round(ms, 4), "voxel_steps_per_s": n ** 3 / (ms * 1e-3),
         "alg_gbps": round(89.28 * n ** 3 / (ms * 1e-3) / 1e9, 1), "mass_rel_drift": abs(m1 - m0) / abs(m0),
         "rel_l2_vs_fp64_oracle_3_steps": worst,
-        "note": "0.8 GB working set: resident in the 256 MB last-level cache only in part; not an HBM fraction"}
+        "note": "24 MB working set (state, right-hand-side staging, half spectrum): resident in the 256 MB "
+                "last-level cache, so not an HBM fraction"}
+    # free-energy decay over the 1000 steps, frames every 100
+    ue = vg.init_scalar_field(c0).contiguous()
+    en = [float(vg.ch_energy(ue, 3.0)[0])]
+    for k in range(1, 1001):
+        st.step_into(ue, ue)
+        if k % 100 == 0:
+            en.append(float(vg.ch_energy(ue, 3.0)[0]))
+    out["config1_ch_imex_100cubed_fp64"]["energy_decay"] = {
+        "free_energy": [round(e, 4) for e in en], "non_increasing": bool(np.all(np.diff(en) <= 0)),
+        "largest_frame_change": float(np.diff(en).max())}
+    del ue
     if cpu:
         t0 = time.perf_counter()
         r = og.field(c0)
@@ -630,6 +690,8 @@ def main():
     _lib.check(lib.evx_fft_plan_set_option(plan.handle, _lib.OPT_SLABS, args.slabs), "slabs")
 
     mass0 = float(vg.channel_sums(u)[0])
+    # one step from c0, kept on the host for the float64 oracle check of the CPU leg
+    step1 = None if args.no_cpu_baseline else stepper.step(0.0, u).cpu().numpy()
     log(f"[bench] warmup {W} steps")
     for _ in range(W):
         stepper.step_into(u, u)
@@ -712,6 +774,20 @@ def main():
         "wall_ms_per_step": round(1e3 * wall / K, 4),
         "mass_rel_drift": abs(mass1 - mass0) / abs(mass0),
     }
+    # free-energy decay over the workload's 500 steps (frames every 50)
+    u = vg.init_scalar_field(c0).contiguous()
+    energies, masses = [float(vg.ch_energy(u, 3.0)[0])], [float(vg.channel_sums(u)[0])]
+    for k in range(1, 501):
+        stepper.step_into(u, u)
+        if k % 50 == 0:
+            energies.append(float(vg.ch_energy(u, 3.0)[0]))
+            masses.append(float(vg.channel_sums(u)[0]))
+    de = np.diff(energies)
+    out["energy_decay"] = {"steps": 500, "frame_every": 50, "free_energy": [round(e, 3) for e in energies],
+                           "non_increasing": bool(np.all(de <= 0)), "largest_frame_change": float(de.max()),
+                           "mass_rel_drift": float(np.abs(np.array(masses) - masses[0]).max() / abs(masses[0])),
+                           "functional": "int eps|grad c|^2 + 9/eps c^2(1-c)^2 dV, central gradients, float64 sums"}
+    log(f"[bench] energy decay: {out['energy_decay']}")
     del u
     torch.cuda.empty_cache()
     if not args.no_batched:
@@ -746,7 +822,8 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline: oracle on {args.cpu_steps} steps (+1 warm-up)")
-        out["cpu_baseline"] = cpu_baseline(c0.astype(np.float32), args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(c0.astype(np.float32), args.cpu_steps, step1)
+        out["rel_l2_vs_fp64_oracle"] = out["cpu_baseline"].pop("rel_l2_vs_fp64_oracle")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

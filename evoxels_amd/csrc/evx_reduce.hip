@@ -56,23 +56,94 @@ __device__ Pair block_reduce(Pair p) {
 }
 
 template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+  static constexpr int W = 4;
+  __device__ static double at(const float4& v, int j) { return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w; }
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int W = 2;
+  __device__ static double at(const double2& v, int j) { return j == 0 ? v.x : v.y; }
+};
+
+struct Flags {
+  bool nan = false, inf = false;
+  __device__ void see(double v) {
+    nan |= v != v;
+    inf |= fabs(v) == __builtin_inf();
+  }
+};
+
+// Per channel, NBLK blocks stream the field with 16-byte loads (U vectors in
+// flight per thread) into W independent Neumaier pairs, one per vector lane,
+// merged in a fixed order at the end; the few elements before the first
+// 16-byte boundary and after the last whole vector are added by the first
+// threads of the grid.  The split of elements over threads depends only on n
+// and the channel's address modulo 16, so results are reproducible.
+template <typename T>
 __global__ __launch_bounds__(RB) void sum_partial_kernel(const T* __restrict__ x, int64_t n, Pair* partial,
                                                          int32_t* nonfinite) {
+  using VT = Vec16<T>;
+  using V = typename VT::type;
+  constexpr int W = VT::W, U = 4;
   const int64_t ch = blockIdx.y;
   const T* xc = x + ch * n;
-  Pair p{0.0, 0.0};
-  bool has_nan = false, has_inf = false;
-  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)NBLK * RB) {
-    const double v = static_cast<double>(xc[i]);
-    has_nan |= isnan(v);
-    has_inf |= isinf(v);
-    neumaier(p, v);
+  const int64_t mis = (int64_t)((reinterpret_cast<uintptr_t>(xc) & 15u) / sizeof(T));
+  const int64_t head = mis ? (W - mis < n ? W - mis : n) : 0;
+  const int64_t nv = (n - head) / W;
+  const V* xv = reinterpret_cast<const V*>(xc + head);
+  Pair p[W];
+#pragma unroll
+  for (int j = 0; j < W; ++j) p[j] = Pair{0.0, 0.0};
+  Flags f;
+  const int64_t stride = (int64_t)NBLK * RB;
+  const int64_t g = (int64_t)blockIdx.x * RB + threadIdx.x;
+  int64_t i = g;
+  for (; i + (U - 1) * stride < nv; i += U * stride) {
+    V v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = xv[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const double d = VT::at(v[u], j);
+        f.see(d);
+        neumaier(p[j], d);
+      }
   }
+  for (; i < nv; i += stride) {
+    const V v = xv[i];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const double d = VT::at(v, j);
+      f.see(d);
+      neumaier(p[j], d);
+    }
+  }
+  const int64_t tail0 = head + nv * W;
+  if (g < head) {
+    const double d = static_cast<double>(xc[g]);
+    f.see(d);
+    neumaier(p[0], d);
+  }
+  if (g < n - tail0) {
+    const double d = static_cast<double>(xc[tail0 + g]);
+    f.see(d);
+    neumaier(p[W - 1], d);
+  }
+  Pair q = p[0];
+#pragma unroll
+  for (int j = 1; j < W; ++j) q = merge(q, p[j]);
   // Every writer stores the same value, so plain stores suffice.
-  if (nonfinite && has_nan) nonfinite[0] = 1;
-  if (nonfinite && has_inf) nonfinite[1] = 1;
-  p = block_reduce(p);
-  if (threadIdx.x == 0) partial[ch * NBLK + blockIdx.x] = p;
+  if (nonfinite && f.nan) nonfinite[0] = 1;
+  if (nonfinite && f.inf) nonfinite[1] = 1;
+  q = block_reduce(q);
+  if (threadIdx.x == 0) partial[ch * NBLK + blockIdx.x] = q;
 }
 
 __global__ __launch_bounds__(RB) void sum_final_kernel(const Pair* partial, double scale, double* out) {

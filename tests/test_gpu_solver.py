@@ -44,6 +44,35 @@ def test_compensated_sum_and_flags(precision):
 
 
 @pytest.mark.parametrize("precision", ["float32", "float64"])
+@pytest.mark.parametrize("shape", [(3, 7, 9, 11), (2, 1, 1, 1), (3, 1, 1, 3), (4, 1, 5, 1), (2, 64, 64, 65)])
+def test_compensated_sum_ragged_channels(precision, shape):
+    """Channel lengths that are not a multiple of the 16-byte vector width:
+    every channel but the first starts off the vector boundary, so the sum
+    runs through its head, vector and tail parts; NaN/Inf flags from the head
+    and the tail elements."""
+    vf = evx.VoxelFields(shape[1:])
+    vg = VoxelGridHip(vf.grid_info(), precision=precision, device="cuda")
+    rng = np.random.default_rng(sum(shape))
+    x = rng.standard_normal(shape) * 10.0 ** rng.integers(-6, 6, shape)
+    xt = torch.tensor(x, dtype=vg.precision, device="cuda")
+    flags = torch.zeros(2, dtype=torch.int32, device="cuda")
+    s = vg.channel_sums(xt, flags).cpu().numpy()
+    xs = xt.cpu().double().numpy()
+    exact = [math.fsum(xs[c].ravel()) for c in range(shape[0])]
+    np.testing.assert_allclose(s, exact, rtol=1e-15, atol=1e-15 * np.abs(xs).max())
+    assert flags.tolist() == [0, 0]
+    flat = xt.view(shape[0], -1)
+    flat[1, 0] = float("nan")
+    vg.channel_sums(xt, flags)
+    assert flags.tolist() == [1, 0]
+    flat[1, 0] = 1.0
+    flags.zero_()
+    flat[-1, -1] = float("-inf")
+    vg.channel_sums(xt, flags)
+    assert flags.tolist() == [0, 1]
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
 def test_ch_energy_matches_oracle(precision):
     shape = (20, 17, 66)
     vf = evx.VoxelFields(shape, (20.0, 17.0, 66.0))
