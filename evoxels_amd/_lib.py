@@ -107,6 +107,7 @@ SIGNATURES = {
     "evx_fft_plan_set_option": [_P, C.c_int, C.c_int],
     "evx_reduce_workspace_bytes": [_I64, C.POINTER(C.c_size_t)],
     "evx_sum_f64": [_I32, _P, _I64, _I64, _P, _P, _P, _P],
+    "evx_sum_f64_noalloc": [_I32, _P, _I64, _I64, _P, _P, _P],
     "evx_ch_energy": [_I32, _P, C.POINTER(EvxGrid), C.POINTER(EvxBC), _D, _D, _P, _P, _P],
 }
 _RESTYPE = {"evx_last_error": C.c_char_p}

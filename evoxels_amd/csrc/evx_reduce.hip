@@ -6,6 +6,8 @@
 // across the wavefronts of a block through LDS, and finally across a fixed
 // number of per-block partials by one block per channel.  The order of all
 // merges is fixed, so results are bitwise reproducible run to run.
+#include <mutex>
+
 #include "evx_common.hpp"
 
 #pragma clang fp contract(off)
@@ -184,6 +186,37 @@ __global__ __launch_bounds__(RB) void energy_partial_kernel(const T* __restrict_
 }
 
 }  // namespace
+
+// Scratch of evx_sum_f64_noalloc: the per-block partials of up to
+// EVX_NOALLOC_MAX_CHANNELS channels (1 MiB), held by each GPU the library
+// runs on until the process ends.  Host code never passes `scratch` itself
+// to a kernel: its device address on the current GPU comes from
+// hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) Pair scratch[EVX_NOALLOC_MAX_CHANNELS * NBLK];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current GPU, looked up once per device
+// (with that device current) and kept.
+int noalloc_scratch(Pair** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return fail(EVX_ERR_HIP, "hipGetDevice failed");
+  if (dev >= kMaxDevices) return fail(EVX_ERR_UNSUPPORTED, "device index %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(EVX_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = static_cast<Pair*>(g_scratch_ptr[dev]);
+  return EVX_OK;
+}
+}  // namespace
 }  // namespace evx
 
 using namespace evx;
@@ -210,6 +243,20 @@ extern "C" int evx_sum_f64(int32_t dtype, const void* x, int64_t channels, int64
   EVX_TRY(check_launch("sum_partial_kernel"));
   hipLaunchKernelGGL(sum_final_kernel, dim3((unsigned)channels), dim3(RB), 0, s, part, 1.0, out);
   return check_launch("sum_final_kernel");
+}
+
+extern "C" int evx_sum_f64_noalloc(int32_t dtype, const void* x, int64_t channels, int64_t n, double* out,
+                                   int32_t* nonfinite, void* stream) {
+  if (dtype != EVX_F32 && dtype != EVX_F64) return fail(EVX_ERR_INVALID, "bad dtype");
+  if (!x || !out || channels < 1 || n < 1) return fail(EVX_ERR_INVALID, "bad arguments");
+  if (channels > EVX_NOALLOC_MAX_CHANNELS)
+    return fail(EVX_ERR_INVALID, "evx_sum_f64_noalloc: %lld channels exceed its static scratch of %d channels",
+                (long long)channels, EVX_NOALLOC_MAX_CHANNELS);
+  Pair* part = nullptr;
+  EVX_TRY(noalloc_scratch(&part));
+  // Every block of every channel overwrites its partial, so the scratch left
+  // by an earlier call needs no clearing.
+  return evx_sum_f64(dtype, x, channels, n, out, nonfinite, part, stream);
 }
 
 extern "C" int evx_ch_energy(int32_t dtype, const void* c, const evx_grid* g, const evx_bc* bc, double eps,

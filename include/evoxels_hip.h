@@ -381,6 +381,20 @@ int evx_sum_f64(int32_t dtype, const void* x, int64_t channels,
                 int64_t elems_per_channel, double* out, int32_t* nonfinite,
                 void* workspace, void* stream);
 
+/* evx_sum_f64 without a workspace argument and without any allocation: the
+ * per-block partials live in a static device array of the library (1 MiB per
+ * GPU, held until the process ends), so at most EVX_NOALLOC_MAX_CHANNELS
+ * channels; more returns EVX_ERR_INVALID before any launch.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Results equal evx_sum_f64's bit for bit.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (two streams,
+ * two threads) race and must be serialised by the caller. */
+#define EVX_NOALLOC_MAX_CHANNELS 64
+int evx_sum_f64_noalloc(int32_t dtype, const void* x, int64_t channels,
+                        int64_t elems_per_channel, double* out,
+                        int32_t* nonfinite, void* stream);
+
 /* Discrete CH free energy per channel in float64:
  *   sum_voxels (eps*|grad_center c|^2 + 9/eps*c^2(1-c)^2) * dv
  * with centred gradients (fd_stencils.py:44-60) on pad_bc(c).  The reference
