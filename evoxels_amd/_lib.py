@@ -34,6 +34,7 @@ OUT_RHS, OUT_EULER, OUT_RK4_FIRST, OUT_RK4_MID, OUT_RK4_LAST = range(5)
 OPT_SPECIALIZED = 1
 OPT_EXACT_SYMBOL = 2
 OPT_SLABS = 3
+OPT_FUSED_RHS = 4
 
 
 class EvxBC(C.Structure):

@@ -25,6 +25,7 @@
 #include "evx_common.hpp"
 #include "evx_fft_engine.hpp"
 #include "evx_fft_tiles.hpp"
+#include "evx_rz_tile.hpp"
 
 namespace evx {
 
@@ -292,6 +293,16 @@ __global__ __launch_bounds__((ZGeo<T, N>::THREADS)) void z_c2r_fast(const cpx<T>
   z_c2r_tile<T, N>(spec, out, base, scale, nlines, pitch, twg, blockIdx.x, threadIdx.x, smem);
 }
 
+// Fused CH right-hand side + z R2C, float32, nz = 512 (evx_rz_tile.hpp).
+__global__ __launch_bounds__(RzGeo::THREADS) void ch_rhs_z_r2c_512(const float* __restrict__ u,
+                                                                   cpx<float>* __restrict__ spec, RzMap m,
+                                                                   Geo<float> g, ChCoef<float> k,
+                                                                   const double* __restrict__ chan,
+                                                                   const cpx<float>* __restrict__ twg) {
+  __shared__ __align__(16) float lds[RzGeo::LDS_FLOATS];
+  rz_tile_body(u, spec, m, g, k, chan, twg, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, lds);
+}
+
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__((SGeo<T, N, MODE>::THREADS)) void strided_fast(cpx<T>* __restrict__ spec, int64_t nlines,
                                                                     LineMap lm, const cpx<T>* __restrict__ twg,
@@ -306,7 +317,7 @@ bool fast_z_r2c(int n, const void* x, void* spec, int64_t nlines, int64_t pitch,
 #define X(NN)                                                                                               \
   case NN: {                                                                                                \
     using G = ZGeo<T, NN>;                                                                                  \
-    const int64_t blocks = ((nlines + 1) / 2 + G::LP - 1) / G::LP;                                          \
+    const int64_t blocks = G::blocks(nlines);                                                               \
     hipLaunchKernelGGL((z_r2c_fast<T, NN>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,                 \
                        static_cast<const T*>(x), static_cast<cpx<T>*>(spec), nlines, pitch,                 \
                        static_cast<const cpx<T>*>(tw));                                                     \
@@ -326,7 +337,7 @@ bool fast_z_c2r(int n, const void* spec, void* out, const void* base, double sca
 #define X(NN)                                                                                               \
   case NN: {                                                                                                \
     using G = ZGeo<T, NN>;                                                                                  \
-    const int64_t blocks = ((nlines + 1) / 2 + G::LP - 1) / G::LP;                                          \
+    const int64_t blocks = G::blocks(nlines);                                                               \
     hipLaunchKernelGGL((z_c2r_fast<T, NN>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,                 \
                        static_cast<const cpx<T>*>(spec), static_cast<T*>(out), static_cast<const T*>(base), \
                        static_cast<T>(scale), nlines, pitch, static_cast<const cpx<T>*>(tw));               \
@@ -423,6 +434,7 @@ struct evx_fft_plan {
   bool fast = true;  // use the size-specialised kernels where one exists
   bool exact_symbol = false;  // float64 multiplier from exact wavenumbers
   int slabs = 0;               // > 1: semi-implicit CH steps run x-slab by x-slab (one-channel plans)
+  bool fused_rhs = true;       // CH steps: fused right-hand side + z R2C where it applies
 };
 
 namespace evx {
@@ -614,7 +626,8 @@ int run_x_multiplier(evx_fft_plan* p, void* spec, const Prefac& pf, int64_t pt, 
 }
 
 // Launch trace (armed by evx_fft_plan_trace_next_step).  Kernel kinds:
-enum { TK_RHS = 0, TK_Z_R2C = 1, TK_Y_FWD = 2, TK_X = 3, TK_Y_INV = 4, TK_Z_C2R = 5, TK_OTHER = 6 };
+enum { TK_RHS = 0, TK_Z_R2C = 1, TK_Y_FWD = 2, TK_X = 3, TK_Y_INV = 4, TK_Z_C2R = 5, TK_RHS_Z_R2C = 6,
+       TK_OTHER = 7 };
 
 int trace_start(evx_fft_plan* p, hipStream_t s) {
   p->tpos = 0;
@@ -641,12 +654,16 @@ void trace_end(evx_fft_plan* p) {
   p->tpos = 0;
 }
 
+// rhs == nullptr: the z half spectrum of the right-hand side is already in
+// spec (the fused first sweep).
 template <typename T>
 int spectral_update(evx_fft_plan* p, const void* rhs, void* spec, const void* u, void* u_out, const Prefac& pf,
                     hipStream_t s) {
   const int64_t pt = p->pitch;
-  EVX_TRY(run_z_r2c<T>(p, rhs, spec, pt, s));
-  EVX_TRY(trace_tick(p, TK_Z_R2C, s));
+  if (rhs) {
+    EVX_TRY(run_z_r2c<T>(p, rhs, spec, pt, s));
+    EVX_TRY(trace_tick(p, TK_Z_R2C, s));
+  }
   EVX_TRY((run_strided<T, 0>(p, 1, spec, pf, pt, s)));
   EVX_TRY(trace_tick(p, TK_Y_FWD, s));
   EVX_TRY(run_x_multiplier<T>(p, spec, pf, pt, s));
@@ -700,6 +717,45 @@ int ch_slab_step(evx_fft_plan* p, const void* u, void* u_out, void* rhs, void* s
 int check_plan(const evx_fft_plan* p) {
   if (!p) return fail(EVX_ERR_INVALID, "null plan");
   return EVX_OK;
+}
+
+// Launch geometry of the fused right-hand side + z R2C sweep: about one
+// resident round of blocks (one per CU), tiles of 16 rows by x-chunks.
+RzMap rz_map(const evx_fft_plan* p, unsigned* blocks) {
+  static const int resident = [] {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, ch_rhs_z_r2c_512, RzGeo::THREADS, 0) != hipSuccess ||
+        per < 1 || cus < 1)
+      return 256;
+    return per * cus;
+  }();
+  return rz_make_map(p->d, p->pitch, resident, blocks);
+}
+
+// Whether a CH step of this plan takes the fused first sweep: float32,
+// nz = 512 with the specialised kernels, ny a multiple of the tile's 16
+// rows, a 16-byte aligned state and index ranges that fit the kernel's
+// 32-bit tile numbering.
+bool rz_applies(const evx_fft_plan* p, const void* u) {
+  if (!p->fused_rhs || !p->fast || p->dtype != EVX_F32 || p->d.nz != RzGeo::N) return false;
+  if (p->d.ny % RzGeo::BY != 0 || (reinterpret_cast<uintptr_t>(u) & 15u) != 0) return false;
+  const int64_t tiles = p->d.c * (p->d.ny / RzGeo::BY) * p->d.nx;
+  return tiles < (int64_t(1) << 31) && p->ax[2].tw != nullptr;
+}
+
+int run_rz(evx_fft_plan* p, const void* u, void* spec, const double dd[3], const double dd2[3],
+           const evx_imex_params* prm, hipStream_t s) {
+  unsigned blocks = 0;
+  const RzMap m = rz_map(p, &blocks);
+  const Geo<float> g = make_geo_from<float>(dd, dd2);
+  const ChCoef<float> k{static_cast<float>(18.0 / prm->eps), static_cast<float>(2.0 * prm->eps),
+                        static_cast<float>(prm->D)};
+  hipLaunchKernelGGL(ch_rhs_z_r2c_512, dim3(blocks), dim3(RzGeo::THREADS), 0, s, static_cast<const float*>(u),
+                     static_cast<cpx<float>*>(spec), m, g, k, prm->chan_params,
+                     static_cast<const cpx<float>*>(p->ax[2].tw));
+  return check_launch("ch_rhs_z_r2c_512");
 }
 
 int imex_common(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm) {
@@ -855,6 +911,11 @@ int ch_spectral_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, cons
     return p->dtype == EVX_F32 ? ch_slab_step<float>(p, u, u_out, rhs, spec, pf, dd, dd2, prm, s)
                                : ch_slab_step<double>(p, u, u_out, rhs, spec, pf, dd, dd2, prm, s);
   EVX_TRY(trace_start(p, s));
+  if (rz_applies(p, u)) {
+    EVX_TRY(run_rz(p, u, spec, dd, dd2, prm, s));
+    EVX_TRY(trace_tick(p, TK_RHS_Z_R2C, s));
+    return spectral_update<float>(p, nullptr, spec, u, u_out, pf, s);
+  }
   EVX_TRY(stencil_rhs_ch_periodic(p->dtype, u, rhs, p->d, dd, dd2, prm->eps, prm->D, prm->chan_params, s));
   EVX_TRY(trace_tick(p, TK_RHS, s));
   return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
@@ -1075,6 +1136,7 @@ extern "C" int evx_fft_plan_set_option(evx_fft_plan* p, int option, int value) {
       if (value < 0 || value > 4096) return fail(EVX_ERR_INVALID, "slab count must be in [0, 4096]");
       p->slabs = value;
       return EVX_OK;
+    case EVX_OPT_FUSED_RHS: p->fused_rhs = value != 0; return EVX_OK;
     default: return fail(EVX_ERR_INVALID, "unknown plan option %d", option);
   }
 }

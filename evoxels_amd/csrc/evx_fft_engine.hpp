@@ -99,6 +99,22 @@ __device__ __forceinline__ void dft_fixed(cpx<T>* x) {
 template <int... R>
 struct Rad {};
 
+// Synchronisation of an exchange.  When every line's threads lie inside one
+// wavefront (TPL divides 64 and a line's threads are contiguous), the
+// exchange is private to that wavefront: a wavefront-scope fence and a
+// code-motion barrier order its LDS writes before its reads, with no
+// workgroup barrier.  Otherwise the whole workgroup synchronises.
+template <bool WAVE>
+__device__ __forceinline__ void xsync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 template <typename T, int N, int TPL, bool INV, int P, typename Lay, int R, int... Rest>
 __device__ __forceinline__ void stockham(cpx<T> (&v)[N / TPL], const Lay& lay, int tl, const cpx<T>* __restrict__ tw) {
   constexpr int E = N / TPL;
@@ -132,7 +148,7 @@ __device__ __forceinline__ void stockham(cpx<T> (&v)[N / TPL], const Lay& lay, i
   } else {
     if constexpr (Lay::SPLIT) {
       // Half-size exchange buffer: real parts, then imaginary parts.
-      __syncthreads();  // previous readers of the exchange buffer are done
+      xsync<Lay::WAVE>();  // previous readers of the exchange buffer are done
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         const int i = tl + s * TPL;
@@ -141,10 +157,10 @@ __device__ __forceinline__ void stockham(cpx<T> (&v)[N / TPL], const Lay& lay, i
 #pragma unroll
         for (int r = 0; r < R; ++r) *lay.at(j + r * P) = y[s][r].x;
       }
-      __syncthreads();
+      xsync<Lay::WAVE>();
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q].x = *lay.at(tl + TPL * q);
-      __syncthreads();
+      xsync<Lay::WAVE>();
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         const int i = tl + s * TPL;
@@ -153,11 +169,11 @@ __device__ __forceinline__ void stockham(cpx<T> (&v)[N / TPL], const Lay& lay, i
 #pragma unroll
         for (int r = 0; r < R; ++r) *lay.at(j + r * P) = y[s][r].y;
       }
-      __syncthreads();
+      xsync<Lay::WAVE>();
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q].y = *lay.at(tl + TPL * q);
     } else {
-      __syncthreads();  // previous readers of the exchange buffer are done
+      xsync<Lay::WAVE>();  // previous readers of the exchange buffer are done
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         const int i = tl + s * TPL;
@@ -166,7 +182,7 @@ __device__ __forceinline__ void stockham(cpx<T> (&v)[N / TPL], const Lay& lay, i
 #pragma unroll
         for (int r = 0; r < R; ++r) *lay.at(j + r * P) = y[s][r];
       }
-      __syncthreads();
+      xsync<Lay::WAVE>();
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = *lay.at(tl + TPL * q);
     }
@@ -182,7 +198,7 @@ __device__ __forceinline__ void fft_regs(cpx<T> (&v)[N / TPL], const Lay& lay, i
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; remap
 // the block index so that each XCD (and its L2) owns a contiguous range of
 // tiles, keeping halos and shared cache lines inside one L2.
-__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
+__host__ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
   const unsigned xcd = b & 7u, local = b >> 3;
   const unsigned per = nb >> 3, rem = nb & 7u;
   return xcd * per + (xcd < rem ? xcd : rem) + local;

@@ -246,11 +246,17 @@ struct ZGeo {
   static constexpr int LP = TPL >= 256 ? 1 : 256 / TPL;  // line pairs per workgroup
   static constexpr int THREADS = LP * TPL;
   static constexpr int LS = N + N / 16;                   // padded LDS line
+  // a line pair's TPL threads (tid / TPL) share one wavefront
+  static constexpr bool WAVE = 64 % TPL == 0;
+  // first line (a) of pair pr of tile `tile`; b = a + 1
+  __host__ __device__ static int64_t first_line(int64_t tile, int pr) { return (tile * LP + pr) * 2; }
+  __host__ __device__ static int64_t blocks(int64_t nlines) { return ((nlines + 1) / 2 + LP - 1) / LP; }
 };
 
-template <typename T, int LS>
+template <typename T, int LS, bool W = false>
 struct LineLay {
   static constexpr bool SPLIT = false;
+  static constexpr bool WAVE = W;
   cpx<T>* base;
   __device__ __forceinline__ cpx<T>* at(int e) const { return base + e + (e >> 4); }
 };
@@ -258,6 +264,7 @@ struct LineLay {
 template <typename T, int TL>
 struct BrickLay {
   static constexpr bool SPLIT = false;
+  static constexpr bool WAVE = false;
   cpx<T>* base;
   __device__ __forceinline__ cpx<T>* at(int e) const { return base + e * TL; }
 };
@@ -266,6 +273,7 @@ struct BrickLay {
 template <typename T, int TL>
 struct BrickLaySplit {
   static constexpr bool SPLIT = true;
+  static constexpr bool WAVE = false;
   T* base;
   __device__ __forceinline__ T* at(int e) const { return base + e * TL; }
 };
@@ -287,7 +295,7 @@ __device__ __forceinline__ void z_r2c_tile(const T* __restrict__ x, cpx<T>* __re
   cpx<T>* lds = tw + N;
   const int pr = tid / TPL, tl = tid - (tid / TPL) * TPL;
   for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
-  const int64_t la = (tile * G::LP + pr) * 2;
+  const int64_t la = G::first_line(tile, pr);
   const bool has_a = la < nlines, has_b = la + 1 < nlines;
   const T* xa = x + la * N;
   cpx<T> v[E];
@@ -297,13 +305,13 @@ __device__ __forceinline__ void z_r2c_tile(const T* __restrict__ x, cpx<T>* __re
     v[q].x = has_a ? xa[e] : T(0);
     v[q].y = has_b ? xa[N + e] : T(0);
   }
-  __syncthreads();
-  const LineLay<T, G::LS> lay{lds + pr * G::LS};
+  __syncthreads();  // twiddles
+  const LineLay<T, G::LS, G::WAVE> lay{lds + pr * G::LS};
   fft_regs<T, N, TPL, false>(v, lay, tl, tw, typename FftSpec<N>::R{});
-  __syncthreads();
+  xsync<G::WAVE>();
 #pragma unroll
   for (int q = 0; q < E; ++q) *lay.at(tl + TPL * q) = v[q];
-  __syncthreads();
+  xsync<G::WAVE>();
   cpx<T>* sa = spec + la * pitch;
 #pragma unroll
   for (int q = 0; q < (NH + TPL - 1) / TPL; ++q) {
@@ -337,7 +345,7 @@ __device__ __forceinline__ void z_c2r_tile(const cpx<T>* __restrict__ spec, T* o
   cpx<T>* lds = tw + N;
   const int pr = tid / TPL, tl = tid - (tid / TPL) * TPL;
   for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
-  const int64_t la = (tile * G::LP + pr) * 2;
+  const int64_t la = G::first_line(tile, pr);
   const bool has_a = la < nlines, has_b = la + 1 < nlines;
   const cpx<T>* sa = spec + la * pitch;
   cpx<T> v[E];
@@ -357,8 +365,8 @@ __device__ __forceinline__ void z_c2r_tile(const cpx<T>* __restrict__ spec, T* o
     bv[q].x = (ba && has_a) ? ba[e] : T(0);
     bv[q].y = (ba && has_b) ? ba[N + e] : T(0);
   }
-  __syncthreads();
-  const LineLay<T, G::LS> lay{lds + pr * G::LS};
+  __syncthreads();  // twiddles
+  const LineLay<T, G::LS, G::WAVE> lay{lds + pr * G::LS};
   fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
   T* oa = out + la * N;
 #pragma unroll

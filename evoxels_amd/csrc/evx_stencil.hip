@@ -497,14 +497,7 @@ int64_t resident_round(K kernel) {
 
 template <typename T>
 dim3 stencil_grid(const Dims& d, TileGrid* tg, int64_t target = XCHUNK_BLOCKS) {
-  constexpr int BY = NTY * TileCfg<T>::PY;
-  const int64_t ty = (d.ny + BY - 1) / BY, tzn = (d.nz + BZ - 1) / BZ;
-  tg->xc = pick_xchunk(d, ty * tzn, target);
-  const int64_t nxc = (d.nx + tg->xc - 1) / tg->xc;
-  tg->ntz = (int)tzn;
-  tg->nty = (int)ty;
-  tg->nxc = (int)nxc;
-  return dim3((unsigned)(tzn * ty * nxc * d.c));
+  return dim3(stencil_tiles<T>(d, tg, target));
 }
 
 template <typename T>

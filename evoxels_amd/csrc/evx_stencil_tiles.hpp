@@ -217,14 +217,18 @@ struct PeriodicLoader {
   T v[NPT];
   bool last_ok;  // whether this thread's last element lies in the tile
 
+  // Byte offset in its x-plane of the tile point this thread loads as
+  // element k (clamped to the tile's last point past the end).
+  __host__ __device__ static uint32_t src_byte(int tid, int k, int64_t y0, int64_t z0, const Dims& d) {
+    const int i = tid + k * NT < ROWS * W - 1 ? tid + k * NT : ROWS * W - 1;
+    const int64_t gy = wrap_near(y0 - H + i / W, d.ny), gz = wrap_near(z0 - H + i % W, d.nz);
+    return static_cast<uint32_t>((gy * d.nz + gz) * (int64_t)sizeof(T));
+  }
+
   __device__ __forceinline__ void init(int64_t y0, int64_t z0, const Dims& d, int tid) {
     last_ok = tid + (NPT - 1) * NT < ROWS * W;
 #pragma unroll
-    for (int k = 0; k < NPT; ++k) {
-      const int i = min(tid + k * NT, ROWS * W - 1);
-      const int64_t gy = wrap_near(y0 - H + i / W, d.ny), gz = wrap_near(z0 - H + i % W, d.nz);
-      boff[k] = static_cast<uint32_t>((gy * d.nz + gz) * (int64_t)sizeof(T));
-    }
+    for (int k = 0; k < NPT; ++k) boff[k] = src_byte(tid, k, y0, z0, d);
   }
 
   __device__ __forceinline__ bool valid(int k) const { return k < NPT - 1 || last_ok; }
@@ -259,7 +263,7 @@ struct TileOrigin {
 };
 
 template <int BY>
-__device__ __forceinline__ TileOrigin tile_origin(const TileGrid& tg, const Dims& d, unsigned t) {
+__host__ __device__ __forceinline__ TileOrigin tile_origin(const TileGrid& tg, const Dims& d, unsigned t) {
   TileOrigin o;
   o.z0 = (int64_t)(t % tg.ntz) * BZ;
   t /= tg.ntz;
@@ -267,7 +271,7 @@ __device__ __forceinline__ TileOrigin tile_origin(const TileGrid& tg, const Dims
   t /= tg.nty;
   o.x0 = (int64_t)(t % tg.nxc) * tg.xc;
   o.ch = t / tg.nxc;
-  o.x1 = min(o.x0 + (int64_t)tg.xc, d.nx);
+  o.x1 = o.x0 + (int64_t)tg.xc < d.nx ? o.x0 + (int64_t)tg.xc : d.nx;
   return o;
 }
 
@@ -280,6 +284,20 @@ __host__ inline int pick_xchunk(const Dims& d, int64_t tiles_yz, int64_t target 
   if (xc < 8) xc = 8;
   if (xc > d.nx) xc = d.nx;
   return (int)xc;
+}
+
+// Tile grid of a streaming stencil launch of about `target` blocks; returns
+// the block count (tiles over z, y, x-chunks and channels).
+template <typename T>
+__host__ inline unsigned stencil_tiles(const Dims& d, TileGrid* tg, int64_t target) {
+  constexpr int BY = NTY * TileCfg<T>::PY;
+  const int64_t ty = (d.ny + BY - 1) / BY, tzn = (d.nz + BZ - 1) / BZ;
+  tg->xc = pick_xchunk(d, ty * tzn, target);
+  const int64_t nxc = (d.nx + tg->xc - 1) / tg->xc;
+  tg->ntz = (int)tzn;
+  tg->nty = (int)ty;
+  tg->nxc = (int)nxc;
+  return (unsigned)(tzn * ty * nxc * d.c);
 }
 
 // ---------------------------------------------------------------------------

@@ -362,8 +362,14 @@ int evx_fft_plan_trace_kinds(const evx_fft_plan* plan, int32_t* kinds,
  *                         right-hand side, z and y sweeps run x-slab by
  *                         x-slab so a slab's intermediates stay in the
  *                         last-level cache (trace events then mark the four
- *                         phase boundaries). */
-enum evx_plan_option { EVX_OPT_SPECIALIZED = 1, EVX_OPT_EXACT_SYMBOL = 2, EVX_OPT_SLABS = 3 };
+ *                         phase boundaries).
+ *   EVX_OPT_FUSED_RHS     1 (default): float32 Cahn-Hilliard steps with
+ *                         nz = 512 and ny a multiple of 16 compute the
+ *                         right-hand side inside the z R2C sweep (five
+ *                         sweeps per step); 0: separate right-hand-side
+ *                         kernel and z R2C (six sweeps). */
+enum evx_plan_option { EVX_OPT_SPECIALIZED = 1, EVX_OPT_EXACT_SYMBOL = 2, EVX_OPT_SLABS = 3,
+                       EVX_OPT_FUSED_RHS = 4 };
 int evx_fft_plan_set_option(evx_fft_plan* plan, int option, int value);
 
 /* ---- reductions ----------------------------------------------------------- */
