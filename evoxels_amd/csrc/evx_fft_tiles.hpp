@@ -245,7 +245,7 @@ struct ZGeo {
   static constexpr int TPL = N / FftSpec<N>::L;
   static constexpr int LP = TPL >= 256 ? 1 : 256 / TPL;  // line pairs per workgroup
   static constexpr int THREADS = LP * TPL;
-  static constexpr int LS = N + N / 16;                   // padded LDS line
+  static constexpr int LS = N + N / 16;                   // padded LDS line (line_slot)
   // a line pair's TPL threads (tid / TPL) share one wavefront
   static constexpr bool WAVE = 64 % TPL == 0;
   // first line (a) of pair pr of tile `tile`; b = a + 1
@@ -253,12 +253,17 @@ struct ZGeo {
   __host__ __device__ static int64_t blocks(int64_t nlines) { return ((nlines + 1) / 2 + LP - 1) / LP; }
 };
 
+// Slot of element e in a padded exchange line: one pad element after every
+// 16 (a pad after every 8 measured no faster).  A line of N elements takes
+// N + N/16 slots.
+__host__ __device__ __forceinline__ int line_slot(int e) { return e + (e >> 4); }
+
 template <typename T, int LS, bool W = false>
 struct LineLay {
   static constexpr bool SPLIT = false;
   static constexpr bool WAVE = W;
   cpx<T>* base;
-  __device__ __forceinline__ cpx<T>* at(int e) const { return base + e + (e >> 4); }
+  __device__ __forceinline__ cpx<T>* at(int e) const { return base + line_slot(e); }
 };
 
 template <typename T, int TL>

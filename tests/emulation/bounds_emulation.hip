@@ -208,7 +208,7 @@ bool emulate_z_pairs(int64_t nlines, int64_t pitch) {
         const int e = tl + TPL * q;
         if (has_a) CHECK(la * N + e < nlines * N, "z load a");
         if (has_b) CHECK((la + 1) * N + e < nlines * N, "z load b");
-        const int64_t slot = N + pr * G::LS + e + (e >> 4);
+        const int64_t slot = N + pr * G::LS + line_slot(e);
         CHECK(slot < lds_elems, "z LDS %lld outside %lld", (long long)slot, (long long)lds_elems);
       }
       for (int q = 0; q < (NH + TPL - 1) / TPL; ++q) {
