@@ -116,6 +116,45 @@ struct Ops<false> {
 };
 #undef EVX_OPS_BODY
 
+// The contracted flavour's coefficient products folded once (on the host, or
+// once per block for per-channel parameters) instead of at every point: the
+// same float products Ops<false>::mu and Ops<false>::div form, so the
+// results are the same.
+template <typename T>
+struct ChW {
+  T wx, wy, wz, wc;  // (2eps)*idx2, (2eps)*idy2, (2eps)*idz2, (2*(2eps))*sum2
+  T s18;             // 18/eps
+  T dx, dy, dz;      // (D/4)*idx2, (D/4)*idy2, (D/4)*idz2
+};
+
+template <typename T>
+__host__ __device__ inline ChW<T> ch_weights(const Geo<T>& g, const ChCoef<T>& k) {
+  ChW<T> o;
+  const T w = k.two_eps;
+  o.wx = w * g.idx2;
+  o.wy = w * g.idy2;
+  o.wz = w * g.idz2;
+  o.wc = (T(2) * w) * g.sum2;
+  o.s18 = k.s18;
+  const T q = T(0.25) * k.D;
+  o.dx = q * g.idx2;
+  o.dy = q * g.idy2;
+  o.dz = q * g.idz2;
+  return o;
+}
+
+// Ops<false>::mu and Ops<false>::div with folded weights.
+template <typename T>
+__device__ __forceinline__ T mu_w(T c, T xp, T xm, T yp, T ym, T zp, T zm, const ChW<T>& W) {
+  const T l = (((xp + xm) * W.wx + (yp + ym) * W.wy) + (zp + zm) * W.wz) - c * W.wc;
+  const T p = (T(2) * c - T(3)) * c + T(1);
+  return (W.s18 * c) * p - l;
+}
+template <typename T>
+__device__ __forceinline__ T div_w(T fxm, T fxp, T fym, T fyp, T fzm, T fzp, const ChW<T>& W) {
+  return ((fxp - fxm) * W.dx + (fyp - fym) * W.dy) + (fzp - fzm) * W.dz;
+}
+
 // Strict-order shorthands used by code that must match the reference rounding.
 template <typename T>
 __device__ __forceinline__ T lap7(T c, T xp, T xm, T yp, T ym, T zp, T zm, const Geo<T>& g) {

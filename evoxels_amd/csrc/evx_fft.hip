@@ -296,11 +296,11 @@ __global__ __launch_bounds__((ZGeo<T, N>::THREADS)) void z_c2r_fast(const cpx<T>
 // Fused CH right-hand side + z R2C, float32, nz = 512 (evx_rz_tile.hpp).
 __global__ __launch_bounds__(RzGeo::THREADS) void ch_rhs_z_r2c_512(const float* __restrict__ u,
                                                                    cpx<float>* __restrict__ spec, RzMap m,
-                                                                   Geo<float> g, ChCoef<float> k,
+                                                                   Geo<float> g, ChW<float> W,
                                                                    const double* __restrict__ chan,
                                                                    const cpx<float>* __restrict__ twg) {
   __shared__ __align__(16) float lds[RzGeo::LDS_FLOATS];
-  rz_tile_body(u, spec, m, g, k, chan, twg, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, lds);
+  rz_tile_body(u, spec, m, g, W, chan, twg, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, lds);
 }
 
 template <typename T, int N, int MODE>
@@ -753,7 +753,7 @@ int run_rz(evx_fft_plan* p, const void* u, void* spec, const double dd[3], const
   const ChCoef<float> k{static_cast<float>(18.0 / prm->eps), static_cast<float>(2.0 * prm->eps),
                         static_cast<float>(prm->D)};
   hipLaunchKernelGGL(ch_rhs_z_r2c_512, dim3(blocks), dim3(RzGeo::THREADS), 0, s, static_cast<const float*>(u),
-                     static_cast<cpx<float>*>(spec), m, g, k, prm->chan_params,
+                     static_cast<cpx<float>*>(spec), m, g, ch_weights(g, k), prm->chan_params,
                      static_cast<const cpx<float>*>(p->ax[2].tw));
   return check_launch("ch_rhs_z_r2c_512");
 }

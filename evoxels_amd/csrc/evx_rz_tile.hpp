@@ -178,8 +178,7 @@ struct Pair8 {
 // neighbours cA (x) and cC (x+2), y neighbours from the own other row and
 // the c(x+1) plane in LDS (rows a-1 and b+1).
 __device__ __forceinline__ void mu_pair(const Pair8& cA, const Pair8& cB, const Pair8& cC, const float* lds, int s,
-                                        int w, int l, const Geo<float>& g, const ChCoef<float>& k, Pair8& m) {
-  using Op = Ops<false>;
+                                        int w, int l, const ChW<float>& W, Pair8& m) {
   float zp[8], zm[8];
   {  // row a: y-1 is row a-1 of the plane, y+1 the own row b
     Row8 ym;
@@ -187,7 +186,7 @@ __device__ __forceinline__ void mu_pair(const Pair8& cA, const Pair8& cB, const 
     z_neighbours(cB.a, zp, zm, l);
 #pragma unroll
     for (int e = 0; e < 8; ++e)
-      m.a.v[e] = Op::mu(cB.a.v[e], cC.a.v[e], cA.a.v[e], cB.b.v[e], ym.v[e], zp[e], zm[e], g, k);
+      m.a.v[e] = mu_w(cB.a.v[e], cC.a.v[e], cA.a.v[e], cB.b.v[e], ym.v[e], zp[e], zm[e], W);
   }
   {  // row b: y-1 is the own row a, y+1 row b+1 of the plane
     Row8 yp;
@@ -195,7 +194,7 @@ __device__ __forceinline__ void mu_pair(const Pair8& cA, const Pair8& cB, const 
     z_neighbours(cB.b, zp, zm, l);
 #pragma unroll
     for (int e = 0; e < 8; ++e)
-      m.b.v[e] = Op::mu(cB.b.v[e], cC.b.v[e], cA.b.v[e], yp.v[e], cB.a.v[e], zp[e], zm[e], g, k);
+      m.b.v[e] = mu_w(cB.b.v[e], cC.b.v[e], cA.b.v[e], yp.v[e], cB.a.v[e], zp[e], zm[e], W);
   }
 }
 
@@ -203,7 +202,7 @@ __device__ __forceinline__ void mu_pair(const Pair8& cA, const Pair8& cB, const 
 // mu(x+1) (registers), the c(x) and mu(x) planes in LDS (rows a-1, b+1) and
 // the carried x-face fluxes x-1/2; updates the carry to x+1/2.
 __device__ __forceinline__ void div_pair(const Pair8& cA, const Pair8& cB, const Pair8& mA, const Pair8& mB,
-                                         const float* lds, int s, int w, int l, const Geo<float>& g, float D,
+                                         const float* lds, int s, int w, int l, const ChW<float>& W,
                                          Pair8& carry, Pair8& r) {
   using Op = Ops<false>;
 #pragma unroll
@@ -245,7 +244,7 @@ __device__ __forceinline__ void div_pair(const Pair8& cA, const Pair8& cB, const
         fym = Op::flux_q(1.0f - cA.a.v[e], c.v[e], mA.a.v[e], m.v[e]);
         fyp = Op::flux_q(q, cy.v[e], m.v[e], my.v[e]);
       }
-      out.v[e] = Op::div(cr.v[e], fxp, fym, fyp, fzm[e], fz[e], g, D);
+      out.v[e] = div_w(cr.v[e], fxp, fym, fyp, fzm[e], fz[e], W);
       cr.v[e] = fxp;
     }
   }
@@ -255,23 +254,22 @@ __device__ __forceinline__ void div_pair(const Pair8& cA, const Pair8& cB, const
 // centre hB, x neighbours hA, hC, outer-row y neighbour hO, the inner y
 // neighbour and the z neighbours from the c(x+1) plane in LDS.
 __device__ __forceinline__ void mu_halo(const float (&hA)[2], const float (&hB)[2], const float (&hC)[2],
-                                        const float (&hO)[2], const float* lds, int s, int z, const Geo<float>& g,
-                                        const ChCoef<float>& k, float (&mh)[2]) {
-  using Op = Ops<false>;
+                                        const float (&hO)[2], const float* lds, int s, int z, const ChW<float>& W,
+                                        float (&mh)[2]) {
   constexpr int N = RzGeo::N, BY = RzGeo::BY;
   const int zp = (z + 1) & (N - 1), zm = (z + N - 1) & (N - 1);
   // row -1: y-1 is the outer row -2, y+1 is row 0
-  mh[0] = Op::mu(hB[0], hC[0], hA[0], lds[RzMap::lds(s, 0, z)], hO[0], lds[RzMap::lds(s, -1, zp)],
-                 lds[RzMap::lds(s, -1, zm)], g, k);
+  mh[0] = mu_w(hB[0], hC[0], hA[0], lds[RzMap::lds(s, 0, z)], hO[0], lds[RzMap::lds(s, -1, zp)],
+               lds[RzMap::lds(s, -1, zm)], W);
   // row BY: y-1 is row BY-1, y+1 the outer row BY+1
-  mh[1] = Op::mu(hB[1], hC[1], hA[1], hO[1], lds[RzMap::lds(s, BY - 1, z)], lds[RzMap::lds(s, BY, zp)],
-                 lds[RzMap::lds(s, BY, zm)], g, k);
+  mh[1] = mu_w(hB[1], hC[1], hA[1], hO[1], lds[RzMap::lds(s, BY - 1, z)], lds[RzMap::lds(s, BY, zp)],
+               lds[RzMap::lds(s, BY, zm)], W);
 }
 
 // One tile (block) of the fused sweep.  u: state (C, nx, ny, 512), spec:
 // half spectrum with row pitch m.pitch, twg: the 512 twiddles.
 __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<float>* __restrict__ spec,
-                                             const RzMap& m, const Geo<float>& g, ChCoef<float> k,
+                                             const RzMap& m, const Geo<float>& g, ChW<float> W,
                                              const double* __restrict__ chan, const cpx<float>* __restrict__ twg,
                                              unsigned tile, int tid, float* lds) {
   constexpr int N = RzGeo::N, NH = RzGeo::NH, BY = RzGeo::BY;
@@ -281,9 +279,9 @@ __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<fl
   const RzTile t = m.tile(tile);
   if (chan) {  // per-channel (eps, D), rounded as the host rounds the scalars
     const double e = chan[2 * t.ch], dd = chan[2 * t.ch + 1];
-    k = ChCoef<float>{static_cast<float>(18.0 / e), static_cast<float>(2.0 * e), static_cast<float>(dd)};
+    W = ch_weights(g, ChCoef<float>{static_cast<float>(18.0 / e), static_cast<float>(2.0 * e),
+                                    static_cast<float>(dd)});
   }
-  const float D = k.D;
   cpx<float>* tw = reinterpret_cast<cpx<float>*>(lds + RzMap::twiddle(0));
   for (int i = tid; i < N; i += RzGeo::THREADS) tw[i] = twg[i];
   const int ra = 2 * w, rb = 2 * w + 1;
@@ -351,11 +349,11 @@ __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<fl
       Pair8 c2;
       load_pair(x0 - 2, c2);
       clip_pair(c2);
-      mu_pair(c2, cP, cA, lds, 2, w, l, g, k, mP);  // mu(x0-1), own points
+      mu_pair(c2, cP, cA, lds, 2, w, l, W, mP);  // mu(x0-1), own points
     }
-    mu_pair(cP, cA, cB, lds, 0, w, l, g, k, mA);    // mu(x0)
+    mu_pair(cP, cA, cB, lds, 0, w, l, W, mA);    // mu(x0)
     float mh[2];
-    mu_halo(hP, hA, hB, hOA, lds, 0, hz, g, k, mh);
+    mu_halo(hP, hA, hB, hOA, lds, 0, hz, W, mh);
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       carry.a.v[e] = Ops<false>::flux_q(1.0f - cP.a.v[e], cA.a.v[e], mP.a.v[e], mA.a.v[e]);
@@ -382,11 +380,11 @@ __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<fl
     load_halo(x + 3, preH);
     load_outer(x + 2, preO);
     // mu(x+1), own points and halo points
-    mu_pair(cA, cB, cC, lds, sB, w, l, g, k, mB);
+    mu_pair(cA, cB, cC, lds, sB, w, l, W, mB);
     float mh[2];
-    mu_halo(hA, hB, hC, hO, lds, sB, hz, g, k, mh);
+    mu_halo(hA, hB, hC, hO, lds, sB, hz, W, mh);
     // rhs at x
-    div_pair(cA, cB, mA, mB, lds, sA, w, l, g, D, carry, r);
+    div_pair(cA, cB, mA, mB, lds, sA, w, l, W, carry, r);
     __syncthreads();  // every read of c(x) and mu(x) is done
     put_mu(mB, mh);
     put_c(sF, cC, hC);

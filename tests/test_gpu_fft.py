@@ -88,9 +88,6 @@ def oracle_imex(kind, arr, spacing_abs, precision, dt, kw):
 def test_imex_step_matches_oracle(kind, shape, spacing, precision):
     """One semi-implicit step against the FP64 oracle; the long-x shapes run
     the 512- and 1024-point forward*multiplier*inverse x sweeps."""
-    if kind == "ch" and shape[2] == 1024:
-        pytest.skip("CH semi-implicit step at nz=1024 faulted the GPU once (with a since-removed fused "
-                    "RHS+FFT kernel); kept off the GPU until a cause is shown")
     kw = {"eps": 3.0, "D": 1.0} if kind == "ch" else {"D": 0.8}
     dt = 0.1
     vf, vg, prob, step, arr = imex_pair(kind, shape, precision, spacing, 3, dt, **kw)
