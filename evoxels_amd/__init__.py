@@ -7,6 +7,6 @@ libevoxels_hip.so (C ABI: include/evoxels_hip.h); build it with
 from .voxelfields import VoxelFields
 from .precompiled_solvers.cahn_hilliard import run_cahn_hilliard_solver
 from .precompiled_solvers.allen_cahn import run_allen_cahn_solver
-from .inversion import InversionModel
+from .inversion import BatchedInversion, InversionModel, SaveAt
 
 __all__ = ["VoxelFields", "run_cahn_hilliard_solver", "run_allen_cahn_solver", "InversionModel"]

@@ -213,3 +213,22 @@ def test_replaced_rhs_disables_fused_steps():
             return super().rhs(t, c)
 
     assert not MyCH(vg).fused
+
+
+def test_inversion_sequence_checks_and_step_counts():
+    """InversionModel.train's sequence validation (inversion.py:170-192;
+    test_inversion.py:12-22) and the fixed-step counts of output times."""
+    from evoxels_amd.inversion import SaveAt, steps_of, validate_inds
+    with pytest.raises(ValueError, match="different length"):
+        validate_inds([[0, 1, 2], [0, 1]])
+    with pytest.raises(ValueError, match="different spacing"):
+        validate_inds([[0, 1, 2], [0, 2, 4]])
+    with pytest.raises(ValueError, match="at least 2"):
+        validate_inds([[3]])
+    validate_inds([[1, 2, 3], [4, 5, 6]])
+    assert steps_of(np.array([0.0, 0.1, 0.2, 0.30000000000000004]), 0.1) == [0, 1, 2, 3]
+    assert steps_of(SaveAt.times(SaveAt(ts=[1.0, 2.0, 4.0])), 0.5) == [0, 2, 6]
+    with pytest.raises(ValueError):
+        steps_of(np.array([0.0, 0.15]), 0.1)
+    s = SaveAt(ts=np.linspace(0, 1, 5))
+    assert s.subs.ts is s.ts and len(SaveAt.times(s)) == 5
