@@ -35,6 +35,7 @@ OPT_SPECIALIZED = 1
 OPT_EXACT_SYMBOL = 2
 OPT_SLABS = 3
 OPT_FUSED_RHS = 4
+OPT_STEP_SUMS = 5
 
 
 class EvxBC(C.Structure):
@@ -106,6 +107,7 @@ SIGNATURES = {
     "evx_fft_plan_trace_next_step": [_P, C.POINTER(_P), C.c_int],
     "evx_fft_plan_trace_kinds": [_P, C.POINTER(_I32), C.POINTER(_I32)],
     "evx_fft_plan_set_option": [_P, C.c_int, C.c_int],
+    "evx_fft_plan_step_sums": [_P, _P, _P, _P, _P],
     "evx_reduce_workspace_bytes": [_I64, C.POINTER(C.c_size_t)],
     "evx_sum_f64": [_I32, _P, _I64, _I64, _P, _P, _P, _P],
     "evx_sum_f64_noalloc": [_I32, _P, _I64, _I64, _P, _P, _P],

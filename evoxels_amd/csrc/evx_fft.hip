@@ -288,9 +288,10 @@ __global__ __launch_bounds__((ZGeo<T, N>::THREADS)) void z_r2c_fast(const T* __r
 template <typename T, int N>
 __global__ __launch_bounds__((ZGeo<T, N>::THREADS)) void z_c2r_fast(const cpx<T>* __restrict__ spec, T* out,
                                                                   const T* base, T scale, int64_t nlines,
-                                                                  int64_t pitch, const cpx<T>* __restrict__ twg) {
+                                                                  int64_t pitch, const cpx<T>* __restrict__ twg,
+                                                                  SumRec* rec) {
   __shared__ __align__(16) unsigned char smem[ZTile<T, N>::LDS_BYTES];
-  z_c2r_tile<T, N>(spec, out, base, scale, nlines, pitch, twg, blockIdx.x, threadIdx.x, smem);
+  z_c2r_tile<T, N>(spec, out, base, scale, nlines, pitch, twg, blockIdx.x, threadIdx.x, smem, rec);
 }
 
 // Fused CH right-hand side + z R2C, float32, nz = 512 (evx_rz_tile.hpp).
@@ -332,7 +333,7 @@ bool fast_z_r2c(int n, const void* x, void* spec, int64_t nlines, int64_t pitch,
 
 template <typename T>
 bool fast_z_c2r(int n, const void* spec, void* out, const void* base, double scale, int64_t nlines,
-                int64_t pitch, const void* tw, hipStream_t s) {
+                int64_t pitch, const void* tw, hipStream_t s, SumRec* rec = nullptr) {
   switch (n) {
 #define X(NN)                                                                                               \
   case NN: {                                                                                                \
@@ -340,7 +341,7 @@ bool fast_z_c2r(int n, const void* spec, void* out, const void* base, double sca
     const int64_t blocks = G::blocks(nlines);                                                               \
     hipLaunchKernelGGL((z_c2r_fast<T, NN>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,                 \
                        static_cast<const cpx<T>*>(spec), static_cast<T*>(out), static_cast<const T*>(base), \
-                       static_cast<T>(scale), nlines, pitch, static_cast<const cpx<T>*>(tw));               \
+                       static_cast<T>(scale), nlines, pitch, static_cast<const cpx<T>*>(tw), rec);          \
     return true;                                                                                            \
   }
     EVX_FAST_SIZES(X)
@@ -435,6 +436,8 @@ struct evx_fft_plan {
   bool exact_symbol = false;  // float64 multiplier from exact wavenumbers
   int slabs = 0;               // > 1: semi-implicit CH steps run x-slab by x-slab (one-channel plans)
   bool fused_rhs = true;       // CH steps: fused right-hand side + z R2C where it applies
+  bool step_sums = true;       // one-channel steps: mass and NaN/Inf records from the z C2R epilogue
+  const void* sums_ws = nullptr;  // workspace holding the records of the last recording step
 };
 
 namespace evx {
@@ -531,7 +534,7 @@ int run_z_r2c(evx_fft_plan* p, const void* x, void* spec, int64_t pitch, hipStre
 
 template <typename T>
 int run_z_c2r(evx_fft_plan* p, const void* spec, void* out, const void* base, double scale, int64_t pitch,
-              hipStream_t s, int64_t xa = 0, int64_t xb = -1) {
+              hipStream_t s, int64_t xa = 0, int64_t xb = -1, SumRec* rec = nullptr) {
   const AxisHost& h = p->ax[2];
   int64_t nlines = p->d.c * p->d.nx * p->d.ny;
   if (xb >= 0) {
@@ -541,8 +544,9 @@ int run_z_c2r(evx_fft_plan* p, const void* spec, void* out, const void* base, do
     spec = static_cast<const cpx<T>*>(spec) + xa * p->d.ny * pitch;
     nlines = (xb - xa) * p->d.ny;
   }
-  if (p->fast && fast_z_c2r<T>(h.n, spec, out, base, scale, nlines, pitch, h.tw, s))
+  if (p->fast && fast_z_c2r<T>(h.n, spec, out, base, scale, nlines, pitch, h.tw, s, rec))
     return check_launch("z_c2r_fast");
+  if (rec) return fail(EVX_ERR_INVALID, "step sums need the size-specialised z C2R");
   const Geom g = z_geom<T>(h, (nlines + 1) / 2);
   const int64_t blocks = (nlines + 2 * g.lpw - 1) / (2 * g.lpw);
   hipLaunchKernelGGL(z_c2r_kernel<T>, dim3((unsigned)blocks), dim3(g.threads), g.lds, s,
@@ -617,6 +621,65 @@ size_t spec_bytes(const evx_fft_plan* p) {
   return align_up((size_t)p->d.c * p->d.nx * p->d.ny * p->pitch * (p->dtype == EVX_F32 ? 8 : 16));
 }
 
+// Records of the z C2R epilogue: one per block of the size-specialised z C2R
+// of a one-channel plan whose blocks are whole wavefronts.
+template <typename T>
+int64_t rec_blocks_t(const evx_fft_plan* p) {
+  const int64_t nlines = p->d.c * p->d.nx * p->d.ny;
+  switch (p->ax[2].n) {
+#define X(NN)                                                              \
+  case NN:                                                                 \
+    return ZGeo<T, NN>::THREADS % 64 == 0 ? ZGeo<T, NN>::blocks(nlines) : 0;
+    EVX_FAST_SIZES(X)
+#undef X
+    default:
+      return 0;
+  }
+}
+int64_t rec_blocks(const evx_fft_plan* p) {
+  if (p->d.c != 1) return 0;
+  return p->dtype == EVX_F32 ? rec_blocks_t<float>(p) : rec_blocks_t<double>(p);
+}
+size_t rec_bytes(const evx_fft_plan* p) { return align_up((size_t)rec_blocks(p) * sizeof(SumRec)); }
+// Bytes of the forward workspace: right-hand-side staging, spectrum, records.
+size_t fwd_ws_bytes(const evx_fft_plan* p) { return real_bytes(p) + spec_bytes(p) + rec_bytes(p); }
+// Records of a step on workspace ws, or nullptr when this step keeps none.
+SumRec* step_recs(evx_fft_plan* p, void* ws) {
+  if (!p->step_sums || !p->fast || p->slabs > 1 || rec_blocks(p) == 0) return nullptr;
+  return reinterpret_cast<SumRec*>(static_cast<char*>(ws) + real_bytes(p) + spec_bytes(p));
+}
+
+__global__ __launch_bounds__(256) void fold_recs(const SumRec* __restrict__ rec, int64_t n, double* out,
+                                                 int32_t* nonfinite) {
+  __shared__ SumPair wsum[4];
+  __shared__ int wflag[4];
+  SumPair acc{0.0, 0.0};
+  int f = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) {
+    acc = sum_merge(acc, SumPair{rec[i].s, rec[i].c});
+    f |= rec[i].nan | (rec[i].inf << 1);
+  }
+  acc = wave_sum(acc);
+  const int wf = (__any(f & 1) ? 1 : 0) | (__any(f & 2) ? 2 : 0);
+  if ((threadIdx.x & 63) == 0) {
+    wsum[threadIdx.x >> 6] = acc;
+    wflag[threadIdx.x >> 6] = wf;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    SumPair b = wsum[0];
+    int g = wflag[0];
+    for (int i = 1; i < 4; ++i) {
+      b = sum_merge(b, wsum[i]);
+      g |= wflag[i];
+    }
+    out[0] = b.s + b.c;
+    // set only, as evx_sum_f64 does: the caller clears the flags
+    if (nonfinite && (g & 1)) nonfinite[0] = 1;
+    if (nonfinite && (g & 2)) nonfinite[1] = 1;
+  }
+}
+
 // The x sweep: forward x transform, spectral multiplier, inverse x transform.
 template <typename T>
 int run_x_multiplier(evx_fft_plan* p, void* spec, const Prefac& pf, int64_t pt, hipStream_t s) {
@@ -658,7 +721,7 @@ void trace_end(evx_fft_plan* p) {
 // spec (the fused first sweep).
 template <typename T>
 int spectral_update(evx_fft_plan* p, const void* rhs, void* spec, const void* u, void* u_out, const Prefac& pf,
-                    hipStream_t s) {
+                    hipStream_t s, SumRec* rec = nullptr) {
   const int64_t pt = p->pitch;
   if (rhs) {
     EVX_TRY(run_z_r2c<T>(p, rhs, spec, pt, s));
@@ -670,7 +733,7 @@ int spectral_update(evx_fft_plan* p, const void* rhs, void* spec, const void* u,
   EVX_TRY(trace_tick(p, TK_X, s));
   EVX_TRY((run_strided<T, 1>(p, 1, spec, pf, pt, s)));
   EVX_TRY(trace_tick(p, TK_Y_INV, s));
-  EVX_TRY(run_z_c2r<T>(p, spec, u_out, u, 1.0, pt, s));
+  EVX_TRY(run_z_c2r<T>(p, spec, u_out, u, 1.0, pt, s, 0, -1, rec));
   EVX_TRY(trace_tick(p, TK_Z_C2R, s));
   trace_end(p);
   return EVX_OK;
@@ -847,7 +910,7 @@ extern "C" int evx_fft_plan_destroy(evx_fft_plan* p) {
 extern "C" int evx_fft_plan_workspace_bytes(const evx_fft_plan* p, size_t* bytes) {
   EVX_TRY(check_plan(p));
   if (!bytes) return fail(EVX_ERR_INVALID, "null out-pointer");
-  *bytes = real_bytes(p) + spec_bytes(p);
+  *bytes = fwd_ws_bytes(p);
   return EVX_OK;
 }
 
@@ -910,16 +973,21 @@ int ch_spectral_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, cons
   if (p->slabs > 1 && p->d.c == 1)
     return p->dtype == EVX_F32 ? ch_slab_step<float>(p, u, u_out, rhs, spec, pf, dd, dd2, prm, s)
                                : ch_slab_step<double>(p, u, u_out, rhs, spec, pf, dd, dd2, prm, s);
+  SumRec* rec = step_recs(p, ws);
+  p->sums_ws = nullptr;
   EVX_TRY(trace_start(p, s));
   if (rz_applies(p, u)) {
     EVX_TRY(run_rz(p, u, spec, dd, dd2, prm, s));
     EVX_TRY(trace_tick(p, TK_RHS_Z_R2C, s));
-    return spectral_update<float>(p, nullptr, spec, u, u_out, pf, s);
+    EVX_TRY(spectral_update<float>(p, nullptr, spec, u, u_out, pf, s, rec));
+  } else {
+    EVX_TRY(stencil_rhs_ch_periodic(p->dtype, u, rhs, p->d, dd, dd2, prm->eps, prm->D, prm->chan_params, s));
+    EVX_TRY(trace_tick(p, TK_RHS, s));
+    EVX_TRY(p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s, rec)
+                                : spectral_update<double>(p, rhs, spec, u, u_out, pf, s, rec));
   }
-  EVX_TRY(stencil_rhs_ch_periodic(p->dtype, u, rhs, p->d, dd, dd2, prm->eps, prm->D, prm->chan_params, s));
-  EVX_TRY(trace_tick(p, TK_RHS, s));
-  return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
-                             : spectral_update<double>(p, rhs, spec, u, u_out, pf, s);
+  if (rec) p->sums_ws = ws;
+  return EVX_OK;
 }
 
 int rd_spectral_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
@@ -937,8 +1005,12 @@ int rd_spectral_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, cons
   Prefac pf = make_prefac(p, 0, (-prm->D) * prm->A, prm->dt, prm->spacing);
   pf.etd = etd;
   if (etd) pf.exact = 0;
-  return p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s)
-                             : spectral_update<double>(p, rhs, spec, u, u_out, pf, s);
+  SumRec* rec = step_recs(p, ws);
+  p->sums_ws = nullptr;
+  EVX_TRY(p->dtype == EVX_F32 ? spectral_update<float>(p, rhs, spec, u, u_out, pf, s, rec)
+                              : spectral_update<double>(p, rhs, spec, u, u_out, pf, s, rec));
+  if (rec) p->sums_ws = ws;
+  return EVX_OK;
 }
 
 // Mirror extension of the right-hand side along x (boundary_conditions.py:
@@ -971,6 +1043,7 @@ __global__ void mirror_x_kernel(T* __restrict__ buf, int64_t n, int64_t plane, i
 int mirror_spectral_step(evx_fft_plan* p, int ch, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
                          const evx_bc* bc, int staggered, hipStream_t s, int etd) {
   EVX_TRY(imex_common(p, u, u_out, ws, prm));
+  p->sums_ws = nullptr;
   if (!bc) return fail(EVX_ERR_INVALID, "null bc");
   if (prm->chan_params) return fail(EVX_ERR_UNSUPPORTED, "per-channel parameters need a periodic grid");
   if (p->d.c != 1) return fail(EVX_ERR_UNSUPPORTED, "the mirrored spectral step handles one channel");
@@ -1079,7 +1152,7 @@ extern "C" int evx_rd_etd1_step(evx_fft_plan* p, const void* u, void* u_out, voi
 extern "C" int evx_ch_imex_vjp_workspace_bytes(const evx_fft_plan* p, size_t* bytes) {
   EVX_TRY(check_plan(p));
   if (!bytes) return fail(EVX_ERR_INVALID, "null argument");
-  *bytes = real_bytes(p) + spec_bytes(p) + ch_rhs_vjp_ws_bytes(p->dtype, p->d);
+  *bytes = fwd_ws_bytes(p) + ch_rhs_vjp_ws_bytes(p->dtype, p->d);
   return EVX_OK;
 }
 
@@ -1088,6 +1161,7 @@ extern "C" int evx_ch_imex_vjp_workspace_bytes(const evx_fft_plan* p, size_t* by
 extern "C" int evx_ch_imex_step_vjp(evx_fft_plan* p, const void* u0, const void* u1, const void* lam1, void* lam0,
                                     double* dparams, void* ws, const evx_imex_params* prm, void* stream) {
   EVX_TRY(imex_common(p, u0, lam0, ws, prm));
+  p->sums_ws = nullptr;
   if (!u1 || !lam1 || !dparams) return fail(EVX_ERR_INVALID, "null argument to imex step vjp");
   if (!prm->chan_params && !(prm->eps != 0.0 && prm->D != 0.0))
     return fail(EVX_ERR_INVALID, "eps and D must be nonzero");
@@ -1098,7 +1172,7 @@ extern "C" int evx_ch_imex_step_vjp(evx_fft_plan* p, const void* u0, const void*
   geometry_from_spacing(p->dtype, prm->spacing, dd, dd2);
   void* w = ws;
   void* spec = static_cast<char*>(ws) + real_bytes(p);
-  void* adj = static_cast<char*>(spec) + spec_bytes(p);
+  void* adj = static_cast<char*>(ws) + fwd_ws_bytes(p);
   Prefac pf = make_prefac(p, 1, ((-2.0 * prm->eps) * prm->D) * prm->A, prm->dt, prm->spacing);
   pf.chan = prm->chan_params;
   pf.A = prm->A;
@@ -1127,6 +1201,19 @@ extern "C" int evx_fft_plan_trace_kinds(const evx_fft_plan* p, int32_t* kinds, i
   return EVX_OK;
 }
 
+extern "C" int evx_fft_plan_step_sums(evx_fft_plan* p, const void* ws, double* out, int32_t* nonfinite,
+                                      void* stream) {
+  EVX_TRY(check_plan(p));
+  if (!ws || !out) return fail(EVX_ERR_INVALID, "null argument");
+  if (p->sums_ws == nullptr || p->sums_ws != ws)
+    return fail(EVX_ERR_INVALID, "no step records on this workspace: the last semi-implicit step of this plan "
+                                 "on it kept none (multi-channel, slab or mirrored steps, step sums off)");
+  const SumRec* rec = reinterpret_cast<const SumRec*>(static_cast<const char*>(ws) + real_bytes(p) + spec_bytes(p));
+  hipLaunchKernelGGL(fold_recs, dim3(1), dim3(256), 0, static_cast<hipStream_t>(stream), rec, rec_blocks(p), out,
+                     nonfinite);
+  return check_launch("fold_recs");
+}
+
 extern "C" int evx_fft_plan_set_option(evx_fft_plan* p, int option, int value) {
   EVX_TRY(check_plan(p));
   switch (option) {
@@ -1137,6 +1224,10 @@ extern "C" int evx_fft_plan_set_option(evx_fft_plan* p, int option, int value) {
       p->slabs = value;
       return EVX_OK;
     case EVX_OPT_FUSED_RHS: p->fused_rhs = value != 0; return EVX_OK;
+    case EVX_OPT_STEP_SUMS:
+      p->step_sums = value != 0;
+      p->sums_ws = nullptr;
+      return EVX_OK;
     default: return fail(EVX_ERR_INVALID, "unknown plan option %d", option);
   }
 }

@@ -5,6 +5,7 @@
 #pragma once
 #include "evx_common.hpp"
 #include "evx_fft_engine.hpp"
+#include "evx_sum.hpp"
 
 #include <type_traits>
 
@@ -283,10 +284,12 @@ struct BrickLaySplit {
   __device__ __forceinline__ T* at(int e) const { return base + e * TL; }
 };
 
-// LDS of one z tile: twiddles plus LP padded lines.
+// LDS of one z tile: twiddles plus LP padded lines, and one pair per
+// wavefront for the block sum of the C2R epilogue.
 template <typename T, int N>
 struct ZTile {
-  static constexpr size_t LDS_BYTES = sizeof(cpx<T>) * (N + ZGeo<T, N>::LP * ZGeo<T, N>::LS);
+  static constexpr size_t LINES_BYTES = sizeof(cpx<T>) * (N + ZGeo<T, N>::LP * ZGeo<T, N>::LS);
+  static constexpr size_t LDS_BYTES = LINES_BYTES + sizeof(SumPair) * 16 + 16;
 };
 
 // z R2C of tile `tile` (LP line pairs): two real lines per complex FFT.
@@ -339,11 +342,14 @@ __device__ __forceinline__ cpx<T> herm_at(const cpx<T>* __restrict__ line, int k
 }
 
 // z C2R of tile `tile`: out = base + scale * irfft(spec) (base may be NULL,
-// out may alias base).
+// out may alias base).  With rec, the block also writes rec[tile]: the
+// compensated float64 sum of its outputs and their NaN / Inf flags (every
+// thread sums its own outputs in float64, the wavefronts merge with TwoSum,
+// then the block in wavefront order).
 template <typename T, int N>
 __device__ __forceinline__ void z_c2r_tile(const cpx<T>* __restrict__ spec, T* out, const T* base, T scale,
                                            int64_t nlines, int64_t pitch, const cpx<T>* __restrict__ twg,
-                                           int64_t tile, int tid, unsigned char* smem) {
+                                           int64_t tile, int tid, unsigned char* smem, SumRec* rec = nullptr) {
   using G = ZGeo<T, N>;
   constexpr int TPL = G::TPL, E = N / TPL, NH = N / 2 + 1;
   cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem);
@@ -374,11 +380,42 @@ __device__ __forceinline__ void z_c2r_tile(const cpx<T>* __restrict__ spec, T* o
   const LineLay<T, G::LS, G::WAVE> lay{lds + pr * G::LS};
   fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
   T* oa = out + la * N;
+  double acc = 0.0;
+  bool nan = false, inf = false;
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int e = tl + TPL * q;
-    if (has_a) oa[e] = bv[q].x + scale * v[q].x;
-    if (has_b) oa[N + e] = bv[q].y + scale * v[q].y;
+    const T oa_v = bv[q].x + scale * v[q].x, ob_v = bv[q].y + scale * v[q].y;
+    if (has_a) oa[e] = oa_v;
+    if (has_b) oa[N + e] = ob_v;
+    if (rec) {
+      const double da = has_a ? static_cast<double>(oa_v) : 0.0, db = has_b ? static_cast<double>(ob_v) : 0.0;
+      acc += da;
+      acc += db;
+      nan = nan || da != da || db != db;
+      inf = inf || fabs(da) == __builtin_inf() || fabs(db) == __builtin_inf();
+    }
+  }
+  if (rec) {  // uniform branch: the whole block writes its record
+    SumPair* ws = reinterpret_cast<SumPair*>(smem + ZTile<T, N>::LINES_BYTES);
+    int* wf = reinterpret_cast<int*>(ws + 16);
+    const SumPair wp = wave_sum(SumPair{acc, 0.0});
+    const int wflags = (__any(nan) ? 1 : 0) | (__any(inf) ? 2 : 0);
+    const int w = tid >> 6, nw = (G::THREADS + 63) / 64;
+    if ((tid & 63) == 0) {
+      ws[w] = wp;
+      wf[w] = wflags;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      SumPair b = ws[0];
+      int f = wf[0];
+      for (int i = 1; i < nw; ++i) {
+        b = sum_merge(b, ws[i]);
+        f |= wf[i];
+      }
+      rec[tile] = SumRec{b.s, b.c, f & 1, (f >> 1) & 1, {0, 0}};
+    }
   }
 }
 

@@ -80,7 +80,7 @@ class _FrameExporter:
             self.free.remove(slot)
         cur = torch.cuda.current_stream(self.device)
         slot.flags.zero_()
-        slot.mass = self.solver.vg.channel_sums(u, slot.flags)
+        slot.mass = self.solver._frame_sums(u, slot.flags)
         slot.snap.copy_(u_out)
         self.stream.wait_stream(cur)
         with torch.cuda.stream(self.stream):
@@ -186,6 +186,7 @@ class BaseSolver(ABC):
         """solvers.py:80-114."""
         u = self._init_fields()
         step = self._init_stepper(time_increment, problem_kwargs, jit)
+        self._u_from_step = False
         self._exporter = _FrameExporter(self, self.vg.device) if self.async_output else None
         torch.cuda.synchronize(self.vg.device)
         start = timer()
@@ -202,6 +203,17 @@ class BaseSolver(ABC):
         self.computation_time = end - start
         if verbose:
             self.profiler.print_memory_stats(start, end, max_iters)
+
+    def _frame_sums(self, u, flags):
+        """Mass (compensated float64 per channel) and NaN/Inf flags of the
+        frame state u: from the records the last step's final sweep left when
+        u is that step's output, otherwise by a reduction over u."""
+        ts = self.timestepper
+        if getattr(self, "_u_from_step", False) and hasattr(ts, "last_step_sums"):
+            sums = ts.last_step_sums(u, flags)
+            if sums is not None:
+                return sums
+        return self.vg.channel_sums(u, flags)
 
     def _export_fields(self, u_out):
         host = u_out.cpu().numpy()
@@ -233,7 +245,7 @@ class BaseSolver(ABC):
                 warnings.warn("live plotting is not part of the 'hip' backend; frames are exported to vf.fields")
             return
         flags = torch.zeros(2, dtype=torch.int32, device=u.device)
-        mass = self.vg.channel_sums(u, flags)
+        mass = self._frame_sums(u, flags)
         self._export_fields(self._output_field(u))
         self.mass_history.append((frame, time, mass.cpu().numpy()))
         if verbose:
@@ -293,6 +305,7 @@ class TimeDependentSolver(BaseSolver):
             else:
                 for k in range(i, seg_end):
                     u = step(k * time_increment, u)
+            self._u_from_step = seg_end > i
             i = seg_end
         time = max_iters * time_increment
         self._handle_outputs(u, frame, time, vtk_out, verbose, plot_bounds, colormap)

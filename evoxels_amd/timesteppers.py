@@ -171,6 +171,20 @@ class PseudoSpectralIMEX(TimeStepper):
             self._ws = torch.empty(plan.workspace_bytes, dtype=torch.uint8, device=u.device)
         return plan, self._ws
 
+    def last_step_sums(self, u, nonfinite=None):
+        """Compensated float64 sum, as a (1,) device tensor, of the state the
+        last fused step wrote (``u``), folded from the per-block records of
+        that step's final sweep; NaN / Inf set ``nonfinite`` like
+        ``VoxelGridHip.channel_sums``.  None when that step kept no records
+        (several channels, or a generic, mirrored or slab step)."""
+        if self._kind is None or self._ws is None or u.shape[0] != 1:
+            return None
+        plan = self.problem.vg.plan(u.shape, u.dtype)
+        out = torch.empty(1, dtype=torch.float64, device=u.device)
+        st = _lib.load().evx_fft_plan_step_sums(plan.handle, _lib.ptr(self._ws), _lib.ptr(out), _lib.ptr(nonfinite),
+                                                _lib.stream_handle(u.device))
+        return out if st == _lib.EVX_OK else None
+
     def step_into(self, u, out):
         """Fused step writing u + update into ``out`` (``out`` may be ``u``)."""
         plan, ws = self.workspace(u)

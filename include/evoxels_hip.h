@@ -367,10 +367,25 @@ int evx_fft_plan_trace_kinds(const evx_fft_plan* plan, int32_t* kinds,
  *                         nz = 512 and ny a multiple of 16 compute the
  *                         right-hand side inside the z R2C sweep (five
  *                         sweeps per step); 0: separate right-hand-side
- *                         kernel and z R2C (six sweeps). */
+ *                         kernel and z R2C (six sweeps).
+ *   EVX_OPT_STEP_SUMS     1 (default): one-channel semi-implicit steps whose
+ *                         z C2R is size-specialised leave per-block records
+ *                         of the state they write (compensated float64 sum,
+ *                         NaN / Inf flags) in the workspace, at no extra HBM
+ *                         traffic; 0: no records. */
 enum evx_plan_option { EVX_OPT_SPECIALIZED = 1, EVX_OPT_EXACT_SYMBOL = 2, EVX_OPT_SLABS = 3,
-                       EVX_OPT_FUSED_RHS = 4 };
+                       EVX_OPT_FUSED_RHS = 4, EVX_OPT_STEP_SUMS = 5 };
 int evx_fft_plan_set_option(evx_fft_plan* plan, int option, int value);
+
+/* Mass and non-finite flags of the state the last semi-implicit step of this
+ * plan wrote with workspace `workspace` (evx_ch_imex_step, evx_rd_imex_step,
+ * evx_ch_etd1_step, evx_rd_etd1_step): out[0] = its compensated float64 sum
+ * (the numerator of VoxelGrid.average, voxelgrid.py:137-141) folded from the
+ * step's per-block records; nonfinite, if not NULL, as for evx_sum_f64
+ * (set to 1 on NaN / Inf, left untouched otherwise; the per-frame NaN check
+ * of solvers.py:163-165).  EVX_ERR_INVALID when that step kept no records. */
+int evx_fft_plan_step_sums(evx_fft_plan* plan, const void* workspace, double* out,
+                           int32_t* nonfinite, void* stream);
 
 /* ---- reductions ----------------------------------------------------------- */
 

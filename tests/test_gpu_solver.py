@@ -295,3 +295,73 @@ def test_mms_slopes_of_hip_kernels():
                                  _hip_rhs(TwoPhaseAllenCahn, N3, "float64", eps=3.0, curvature=0.5))
     np.testing.assert_allclose(errs, [0.19028211, 0.05396869, 0.01346211, 0.00334846, 0.00102839], rtol=5e-6)
     assert abs(s - 1.907376693930398) < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_step_records_give_the_mass_and_nan_flag(precision):
+    """The semi-implicit step's final sweep leaves per-block compensated sums
+    of the state it writes: folded, they equal the standalone reduction of
+    that state (to float64 rounding), also after graph replay, and flag NaN."""
+    shape = (24, 32, 512) if precision == "float32" else (20, 16, 64)
+    vf = evx.VoxelFields(shape, shape)
+    vf.precision = precision
+    vg = VoxelGridHip(vf.grid_info(), precision=precision, device="cuda")
+    st = PseudoSpectralIMEX(CahnHilliard(vg, eps=3.0, D=1.0), 0.05)
+    u = vg.init_scalar_field(0.5 + 0.05 * np.random.default_rng(21).standard_normal(shape)).contiguous()
+    v = st.step(0.0, u)
+    flags = torch.zeros(2, dtype=torch.int32, device="cuda")
+    rec = st.last_step_sums(v, flags)
+    assert rec is not None and flags.tolist() == [0, 0]
+    ref = vg.channel_sums(v)
+    exact = math.fsum(v.cpu().double().numpy().ravel())
+    print(f"{precision}: records {float(rec[0])!r}, reduction {float(ref[0])!r}, fsum {exact!r}", flush=True)
+    assert abs(float(rec[0]) - exact) <= 1e-14 * abs(exact)
+    assert abs(float(ref[0]) - exact) <= 1e-14 * abs(exact)
+    g = torch.cuda.CUDAGraph()
+    w = u.clone()
+    st.step_into(w, w)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.graph(g, stream=side):
+        st.step_into(w, w)
+    for _ in range(3):
+        g.replay()
+        r = st.last_step_sums(w)
+        assert abs(float(r[0]) - float(vg.channel_sums(w)[0])) <= 1e-14 * abs(float(r[0]))
+    bad = u.clone()
+    bad.view(-1)[12345] = float("nan")
+    out = st.step(0.0, bad)
+    flags.zero_()
+    st.last_step_sums(out, flags)
+    assert flags.tolist()[0] == 1
+    # several channels keep no records
+    u2 = torch.cat([u, u], 0)
+    st2 = PseudoSpectralIMEX(CahnHilliard(vg, eps=3.0, D=1.0), 0.05)
+    assert st2.last_step_sums(st2.step(0.0, u2)) is None
+
+
+def test_solver_mass_history_from_step_records():
+    """The solver's per-frame mass comes from the step records after the
+    first frame: equal to the reduction of the exported frames."""
+    n = 32
+    vf = evx.VoxelFields((16, n, 512), (16, n, 512))
+    c0 = 0.5 + 0.05 * np.random.default_rng(22).standard_normal((16, n, 512))
+    vf.add_field("c", c0.astype(np.float32))
+    frames = []
+    orig = TimeDependentSolver._export_fields
+
+    def spy(self, u_out):
+        orig(self, u_out)
+        frames.append(self.vf.fields["c"].astype(np.float64).copy())
+
+    TimeDependentSolver._export_fields = spy
+    try:
+        solver = TimeDependentSolver(vf, "c", backend="hip", problem_cls=CahnHilliard,
+                                     timestepper_cls=PseudoSpectralIMEX, device="cuda")
+        solver.solve(time_increment=0.05, frames=4, max_iters=8, jit=True, verbose=False)
+    finally:
+        TimeDependentSolver._export_fields = orig
+    masses = [float(m[0]) for _, _, m in solver.mass_history]
+    assert len(masses) == len(frames) == 5
+    for m, f in zip(masses, frames):
+        assert abs(m - math.fsum(f.ravel())) <= 1e-14 * abs(m)
