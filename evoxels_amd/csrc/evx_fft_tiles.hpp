@@ -224,6 +224,36 @@ __device__ __forceinline__ float prefac_f32(int jx, int jy, int jz, const Prefac
   return (1.0f / den) * p.dt;
 }
 
+// The float32 multiplier of one x line (jy, jz fixed): the per-line part of
+// the k^2 sum is formed once, in the same float roundings as prefac_f32
+// (k^2 = (kx*kx + ky*ky) + kz*kz).  FAST takes the hardware reciprocal (1 ulp)
+// for 1/(1 - dt*sigma): used by float32 plans, whose FFT rounding is far
+// larger; float64 plans keep the IEEE quotient, so their multiplier is the
+// reference's float32 table value exactly.
+struct PrefacLine {
+  float ky2, kz2, ssc;
+};
+
+__device__ __forceinline__ PrefacLine prefac_line(int jy, int jz, const Prefac& p, float ssc) {
+#pragma clang fp contract(off)
+  const int sy = jy < (p.ny + 1) / 2 ? jy : jy - p.ny;
+  const float ky = p.twopi * (static_cast<float>(sy) * p.inv_nd[1]);
+  const float kz = p.twopi * (static_cast<float>(jz) * p.inv_nd[2]);
+  return PrefacLine{ky * ky, kz * kz, ssc};
+}
+
+template <bool FAST>
+__device__ __forceinline__ float prefac_f32_line(int jx, const PrefacLine& L, const Prefac& p) {
+#pragma clang fp contract(off)
+  const int sx = jx < (p.nx + 1) / 2 ? jx : jx - p.nx;
+  const float kx = p.twopi * (static_cast<float>(sx) * p.inv_nd[0]);
+  const float k2 = (kx * kx + L.ky2) + L.kz2;
+  const float sym = p.ch ? L.ssc * (k2 * k2) : L.ssc * k2;
+  const float den = 1.0f - p.dt * sym;
+  if constexpr (FAST) return __builtin_amdgcn_rcpf(den) * p.dt;
+  return (1.0f / den) * p.dt;
+}
+
 // --------------------------------------------------------- specialised sizes --
 // Compile-time radix plans: L = lcm of the radices (minimum values per thread).
 template <int N> struct FftSpec;
@@ -381,19 +411,28 @@ __device__ __forceinline__ void z_c2r_tile(const cpx<T>* __restrict__ spec, T* o
   fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
   T* oa = out + la * N;
   double acc = 0.0;
-  bool nan = false, inf = false;
+  bool bad = false;  // some output NaN or +-Inf
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int e = tl + TPL * q;
-    const T oa_v = bv[q].x + scale * v[q].x, ob_v = bv[q].y + scale * v[q].y;
+    T oa_v = bv[q].x + scale * v[q].x, ob_v = bv[q].y + scale * v[q].y;
     if (has_a) oa[e] = oa_v;
     if (has_b) oa[N + e] = ob_v;
     if (rec) {
-      const double da = has_a ? static_cast<double>(oa_v) : 0.0, db = has_b ? static_cast<double>(ob_v) : 0.0;
-      acc += da;
-      acc += db;
-      nan = nan || da != da || db != db;
-      inf = inf || fabs(da) == __builtin_inf() || fabs(db) == __builtin_inf();
+      oa_v = has_a ? oa_v : T(0);
+      ob_v = has_b ? ob_v : T(0);
+      acc += static_cast<double>(oa_v);
+      acc += static_cast<double>(ob_v);
+      bad = bad || !__builtin_isfinite(oa_v) || !__builtin_isfinite(ob_v);
+    }
+  }
+  bool nan = false, inf = false;
+  if (rec && __any(bad)) {  // rare: tell NaN from Inf over this thread's outputs again
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const T a = has_a ? bv[q].x + scale * v[q].x : T(0), b = has_b ? bv[q].y + scale * v[q].y : T(0);
+      nan = nan || a != a || b != b;
+      inf = inf || __builtin_isinf(a) || __builtin_isinf(b);
     }
   }
   if (rec) {  // uniform branch: the whole block writes its record
@@ -530,9 +569,10 @@ __device__ __forceinline__ void brick_transform(cpx<T> (&v)[E], const Lay& lay, 
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = cscale(v[q], static_cast<T>(etd1_f32(tl + TPL * q, jy, jz, pf, ssc)) * sc);
     } else {
-      const float ssc = symbol_scalar(pf, ch);
+      const PrefacLine L = prefac_line(jy, jz, pf, symbol_scalar(pf, ch));
 #pragma unroll
-      for (int q = 0; q < E; ++q) v[q] = cscale(v[q], static_cast<T>(prefac_f32(tl + TPL * q, jy, jz, pf, ssc)) * sc);
+      for (int q = 0; q < E; ++q)
+        v[q] = cscale(v[q], static_cast<T>(prefac_f32_line<sizeof(T) == 4>(tl + TPL * q, L, pf)) * sc);
     }
     fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
   }
