@@ -112,8 +112,11 @@ __device__ __forceinline__ T apply_rules(T v, const int8_t o[3], const Bc& bc) {
 
 template <typename T>
 __device__ __forceinline__ T clip01(T v) {
-  // torch.clip(c, 0, 1): min(max(v, 0), 1), NaN propagates.
-  return v != v ? v : (v < T(0) ? T(0) : (v > T(1) ? T(1) : v));
+  // torch.clip(c, 0, 1): min(max(v, 0), 1), NaN propagates (both
+  // comparisons are false for NaN, so v passes through).  Two compares and
+  // two selects; the form with an explicit v != v test compiled to branches.
+  const T lo = v < T(0) ? T(0) : v;
+  return lo > T(1) ? T(1) : lo;
 }
 
 // ------------------------------------------------------------- complex ----

@@ -69,11 +69,14 @@ struct RzMap {
   }
   // z of element e (0..7) of lane l: two 4-wide chunks, N/2 apart.
   __host__ __device__ static int z_of(int l, int e) { return (e < 4 ? 4 * l : RzGeo::N / 2 + 4 * l) + (e & 3); }
-  // Element offset of (x, y) line start of channel ch in the field (x, y
-  // taken modulo the grid: periodic wrap of halo and warm-up planes).
-  __host__ __device__ int64_t line(int64_t ch, int64_t x, int64_t y) const {
-    return ((ch * d.nx + wrap_near(x, d.nx)) * d.ny + wrap_near(y, d.ny)) * (int64_t)RzGeo::N;
+  // Element offset of plane x of channel ch and of row y inside a plane (x,
+  // y taken modulo the grid: periodic wrap of halo and warm-up planes); a
+  // line starts at plane + row.
+  __host__ __device__ int64_t plane(int64_t ch, int64_t x) const {
+    return (ch * d.nx + wrap_near(x, d.nx)) * d.ny * (int64_t)RzGeo::N;
   }
+  __host__ __device__ int64_t row(int64_t y) const { return wrap_near(y, d.ny) * (int64_t)RzGeo::N; }
+  __host__ __device__ int64_t line(int64_t ch, int64_t x, int64_t y) const { return plane(ch, x) + row(y); }
   // Spectrum element k of the line of row y (inside the grid) at plane x.
   __host__ __device__ int64_t spec(int64_t ch, int64_t x, int64_t y, int k) const {
     return ((ch * d.nx + x) * d.ny + y) * pitch + k;
@@ -88,6 +91,34 @@ struct RzMap {
   __host__ __device__ static int twiddle(int i) { return RzGeo::NSLOT * RzGeo::SLOT + 2 * i; }
   // Halo points of thread (w, l): z = 64w + l on rows -1 and BY.
   __host__ __device__ static int halo_z(int w, int l) { return 64 * w + l; }
+};
+
+// Row offsets of a wavefront's loads, fixed for the whole tile: its own rows
+// 2w, 2w+1, the halo rows -1, BY and the outer rows -2, BY+1.
+struct RzRows {
+  int64_t a, b, h0, h1, o0, o1;
+};
+__host__ __device__ inline RzRows rz_rows(const RzMap& m, int64_t y0, int w) {
+  return RzRows{m.row(y0 + 2 * w), m.row(y0 + 2 * w + 1), m.row(y0 - 1),
+                m.row(y0 + RzGeo::BY),  m.row(y0 - 2),         m.row(y0 + RzGeo::BY + 1)};
+}
+
+// Planes the main loop loads at iteration x: `next` = plane x+3 (own and
+// halo rows), `outer` = plane x+2 (outer rows).  Wrapped once at set-up,
+// then advanced by one plane per iteration.
+struct RzPlanes {
+  int64_t chb, stride, nx, xn, next, outer;
+  __host__ __device__ RzPlanes(const RzMap& m, int64_t ch, int64_t x)
+      : chb(ch * m.d.nx * m.d.ny * (int64_t)RzGeo::N), stride(m.d.ny * (int64_t)RzGeo::N), nx(m.d.nx) {
+    xn = wrap_near(x + 3, nx);
+    next = chb + xn * stride;
+    outer = m.plane(ch, x + 2);
+  }
+  __host__ __device__ void advance() {
+    outer = next;
+    xn = xn + 1 == nx ? 0 : xn + 1;
+    next = chb + xn * stride;
+  }
 };
 
 // Launch geometry: about `resident` blocks (one resident round), tiles of
@@ -205,6 +236,7 @@ __device__ __forceinline__ void div_pair(const Pair8& cA, const Pair8& cB, const
                                          const float* lds, int s, int w, int l, const ChW<float>& W,
                                          Pair8& carry, Pair8& r) {
   using Op = Ops<false>;
+  float fab[8];  // face a|b: row a's y+1/2 face is row b's y-1/2 face
 #pragma unroll
   for (int row = 0; row < 2; ++row) {
     const Row8& c = row ? cA.b : cA.a;
@@ -240,8 +272,9 @@ __device__ __forceinline__ void div_pair(const Pair8& cA, const Pair8& cB, const
       if (row == 0) {  // faces a-1|a and a|b
         fym = Op::flux_q(1.0f - cy.v[e], c.v[e], my.v[e], m.v[e]);
         fyp = Op::flux_q(q, cA.b.v[e], m.v[e], mA.b.v[e]);
+        fab[e] = fyp;
       } else {  // faces a|b and b|b+1
-        fym = Op::flux_q(1.0f - cA.a.v[e], c.v[e], mA.a.v[e], m.v[e]);
+        fym = fab[e];
         fyp = Op::flux_q(q, cy.v[e], m.v[e], my.v[e]);
       }
       out.v[e] = div_w(cr.v[e], fxp, fym, fyp, fzm[e], fz[e], W);
@@ -288,18 +321,21 @@ __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<fl
   const int hz = RzMap::halo_z(w, l);
   const int64_t y0 = t.y0;
 
-  auto load_pair = [&](int64_t x, Pair8& p) {
-    load_row(u + m.line(t.ch, x, y0 + ra), p.a, l);
-    load_row(u + m.line(t.ch, x, y0 + rb), p.b, l);
+  // loads of plane offset p (RzMap::plane)
+  const RzRows rw = rz_rows(m, y0, w);
+  auto load_pair = [&](int64_t p, Pair8& q) {
+    load_row(u + p + rw.a, q.a, l);
+    load_row(u + p + rw.b, q.b, l);
   };
-  auto load_halo = [&](int64_t x, float (&h)[2]) {  // rows -1, BY
-    h[0] = u[m.line(t.ch, x, y0 - 1) + hz];
-    h[1] = u[m.line(t.ch, x, y0 + BY) + hz];
+  auto load_halo = [&](int64_t p, float (&h)[2]) {  // rows -1, BY
+    h[0] = u[p + rw.h0 + hz];
+    h[1] = u[p + rw.h1 + hz];
   };
-  auto load_outer = [&](int64_t x, float (&h)[2]) {  // rows -2, BY+1
-    h[0] = u[m.line(t.ch, x, y0 - 2) + hz];
-    h[1] = u[m.line(t.ch, x, y0 + BY + 1) + hz];
+  auto load_outer = [&](int64_t p, float (&h)[2]) {  // rows -2, BY+1
+    h[0] = u[p + rw.o0 + hz];
+    h[1] = u[p + rw.o1 + hz];
   };
+  auto plane = [&](int64_t x) { return m.plane(t.ch, x); };
   auto clip_pair = [&](Pair8& p) {
     clip_row(p.a);
     clip_row(p.b);
@@ -330,13 +366,13 @@ __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<fl
   {
     Pair8 cP;
     float hP[2], hOA[2];
-    load_pair(x0 - 1, cP);
-    load_pair(x0, cA);
-    load_pair(x0 + 1, cB);
-    load_halo(x0 - 1, hP);
-    load_halo(x0, hA);
-    load_halo(x0 + 1, hB);
-    load_outer(x0, hOA);
+    load_pair(plane(x0 - 1), cP);
+    load_pair(plane(x0), cA);
+    load_pair(plane(x0 + 1), cB);
+    load_halo(plane(x0 - 1), hP);
+    load_halo(plane(x0), hA);
+    load_halo(plane(x0 + 1), hB);
+    load_outer(plane(x0), hOA);
     clip_pair(cP); clip_pair(cA); clip_pair(cB);
     clip2(hP); clip2(hA); clip2(hB); clip2(hOA);
     // slot 2 <- c(x0-1), slot 0 <- c(x0), slot 1 <- c(x0+1)
@@ -347,7 +383,7 @@ __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<fl
     Pair8 mP;
     {
       Pair8 c2;
-      load_pair(x0 - 2, c2);
+      load_pair(plane(x0 - 2), c2);
       clip_pair(c2);
       mu_pair(c2, cP, cA, lds, 2, w, l, W, mP);  // mu(x0-1), own points
     }
@@ -359,9 +395,9 @@ __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<fl
       carry.a.v[e] = Ops<false>::flux_q(1.0f - cP.a.v[e], cA.a.v[e], mP.a.v[e], mA.a.v[e]);
       carry.b.v[e] = Ops<false>::flux_q(1.0f - cP.b.v[e], cA.b.v[e], mP.b.v[e], mA.b.v[e]);
     }
-    load_pair(x0 + 2, cC);
-    load_halo(x0 + 2, hC);
-    load_outer(x0 + 1, hO);
+    load_pair(plane(x0 + 2), cC);
+    load_halo(plane(x0 + 2), hC);
+    load_outer(plane(x0 + 1), hO);
     clip_pair(cC);
     clip2(hC);
     clip2(hO);
@@ -374,11 +410,14 @@ __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<fl
   int sA = 0, sB = 1, sF = 2;
   Pair8 pre;
   float preH[2], preO[2];
+  RzPlanes pl(m, t.ch, x0);
+  cpx<float>* sa = spec + m.spec(t.ch, x0, y0 + ra, 0);
+  const int64_t sstep = m.d.ny * m.pitch;
   for (int64_t x = x0; x < t.x1; ++x) {
     // next plane's loads: c(x+3) (own and halo rows), outer rows of c(x+2)
-    load_pair(x + 3, pre);
-    load_halo(x + 3, preH);
-    load_outer(x + 2, preO);
+    load_pair(pl.next, pre);
+    load_halo(pl.next, preH);
+    load_outer(pl.outer, preO);
     // mu(x+1), own points and halo points
     mu_pair(cA, cB, cC, lds, sB, w, l, W, mB);
     float mh[2];
@@ -402,7 +441,6 @@ __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<fl
 #pragma unroll
     for (int q = 0; q < 8; ++q) *xlay.at(l + 64 * q) = v[q];
     xsync<true>();
-    cpx<float>* sa = spec + m.spec(t.ch, x, y0 + ra, 0);
 #pragma unroll
     for (int q = 0; q < (NH + 63) / 64; ++q) {
       const int kk = l + 64 * q;
@@ -427,6 +465,8 @@ __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<fl
     sA = sB;
     sB = sF;
     sF = s0;
+    pl.advance();
+    sa += sstep;
     __syncthreads();  // the new c and mu planes are visible; the exchange is done
   }
 }

@@ -68,12 +68,19 @@ bool emulate_rz(int64_t C, int64_t nx, int64_t ny, int64_t pitch, int64_t reside
       const int w = tid >> 6, l = tid & 63;
       const int ra = 2 * w, rb = 2 * w + 1, hz = RzMap::halo_z(w, l);
       CHECK(hz >= 0 && hz < N, "halo z %d", hz);
-      auto pair = [&](int64_t x) -> bool {
-        for (int r : {ra, rb}) {
-          const int64_t base = m.line(t.ch, x, t.y0 + r);
-          if (!gload(base + 4 * l, 4) || !gload(base + N / 2 + 4 * l, 4)) return false;
-        }
+      // the kernel's loads: plane offset p plus the tile's fixed row offsets
+      const RzRows rw = rz_rows(m, t.y0, w);
+      CHECK(rw.a == m.row(t.y0 + ra) && rw.b == m.row(t.y0 + rb) && rw.h0 == m.row(t.y0 - 1) &&
+                rw.h1 == m.row(t.y0 + BY) && rw.o0 == m.row(t.y0 - 2) && rw.o1 == m.row(t.y0 + BY + 1),
+            "row offsets of tile %u wavefront %d", tile, w);
+      auto pair_at = [&](int64_t p) -> bool {
+        for (int64_t r : {rw.a, rw.b})
+          if (!gload(p + r + 4 * l, 4) || !gload(p + r + N / 2 + 4 * l, 4)) return false;
         return true;
+      };
+      auto pair = [&](int64_t x) -> bool { return pair_at(m.plane(t.ch, x)); };
+      auto halo_at = [&](int64_t p, int64_t r0, int64_t r1) -> bool {
+        return gload(p + r0 + hz, 1) && gload(p + r1 + hz, 1);
       };
       auto halo = [&](int64_t x, int r0, int r1) -> bool {
         return gload(m.line(t.ch, x, t.y0 + r0) + hz, 1) && gload(m.line(t.ch, x, t.y0 + r1) + hz, 1);
@@ -104,8 +111,14 @@ bool emulate_rz(int64_t C, int64_t nx, int64_t ny, int64_t pitch, int64_t reside
       if (!put_c(2) || !put_c(0) || !put_c(1) || !mu_pair(2) || !mu_pair(0) || !mu_halo(0) || !put_c(3))
         return false;
       int sA = 0, sB = 1, sF = 2;
+      RzPlanes pl(m, t.ch, t.x0);
+      int64_t sa = m.spec(t.ch, t.x0, t.y0 + ra, 0);
+      const int64_t sstep = m.d.ny * m.pitch;
       for (int64_t x = t.x0; x < t.x1; ++x) {
-        if (!pair(x + 3) || !halo(x + 3, -1, BY) || !halo(x + 2, -2, BY + 1)) return false;
+        CHECK(pl.next == m.plane(t.ch, x + 3) && pl.outer == m.plane(t.ch, x + 2), "plane walk at x %lld",
+              (long long)x);
+        CHECK(sa == m.spec(t.ch, x, t.y0 + ra, 0), "spectrum pointer at x %lld", (long long)x);
+        if (!pair_at(pl.next) || !halo_at(pl.next, rw.h0, rw.h1) || !halo_at(pl.outer, rw.o0, rw.o1)) return false;
         if (!mu_pair(sB) || !mu_halo(sB)) return false;
         if (!mu_pair(sA) || !mu_pair(3)) return false;  // div_pair: c(x) and mu(x) planes
         if (!put_c(3) || !put_c(sF)) return false;
@@ -122,7 +135,7 @@ bool emulate_rz(int64_t C, int64_t nx, int64_t ny, int64_t pitch, int64_t reside
           const int kk = l + 64 * q;
           if (kk >= NH) continue;
           for (int r : {ra, rb}) {
-            const int64_t off = m.spec(t.ch, x, t.y0 + r, kk);
+            const int64_t off = sa + (r - ra) * m.pitch + kk;
             CHECK(off >= 0 && off < spec_elems, "spectrum store %lld outside %lld", (long long)off,
                   (long long)spec_elems);
             const int64_t o = ((t.ch * nx + x) * ny + t.y0 + r) * NH + kk;
@@ -137,6 +150,8 @@ bool emulate_rz(int64_t C, int64_t nx, int64_t ny, int64_t pitch, int64_t reside
         sA = sB;
         sB = sF;
         sF = s0;
+        pl.advance();
+        sa += sstep;
       }
     }
   }
@@ -233,7 +248,8 @@ int main() {
     int64_t C, nx, ny, resident;
   };
   const RzCase rz[] = {{1, 512, 512, 256}, {1, 12, 16, 256}, {3, 5, 32, 256},  {1, 4, 16, 256},
-                       {2, 1, 48, 7},      {1, 9, 16, 1024}, {1, 512, 512, 8}};
+                       {2, 1, 48, 7},      {1, 9, 16, 1024}, {1, 512, 512, 8}, {1, 2, 16, 256},
+                       {2, 3, 32, 4}};
   for (const RzCase& c : rz) {
     const bool ok = emulate_rz(c.C, c.nx, c.ny, RzGeo::NH, c.resident);
     std::printf("%s fused rhs+z-r2c (%lld,%lld,%lld,512) resident %lld\n", ok ? "OK" : "FAILED", (long long)c.C,

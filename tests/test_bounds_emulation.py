@@ -34,4 +34,4 @@ def test_first_sweep_index_math_stays_in_bounds(tmp_path):
     print(run.stdout)
     assert run.returncode == 0, run.stdout + run.stderr
     lines = run.stdout.strip().splitlines()
-    assert len(lines) == 13 and all(line.startswith("OK ") for line in lines)
+    assert len(lines) == 15 and all(line.startswith("OK ") for line in lines)
