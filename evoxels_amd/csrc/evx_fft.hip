@@ -304,8 +304,10 @@ __global__ __launch_bounds__(RzGeo::THREADS) void ch_rhs_z_r2c_512(const float* 
   rz_tile_body(u, spec, m, g, W, chan, twg, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, lds);
 }
 
+// The wavefront-per-line x sweep keeps two bricks per CU: eight waves per
+// SIMD, 64 VGPRs.
 template <typename T, int N, int MODE>
-__global__ __launch_bounds__((SGeo<T, N, MODE>::THREADS)) void strided_fast(cpx<T>* __restrict__ spec, int64_t nlines,
+__global__ __launch_bounds__((SGeo<T, N, MODE>::THREADS), (SWave<T, N, MODE>::ON ? 8 : 1)) void strided_fast(cpx<T>* __restrict__ spec, int64_t nlines,
                                                                     LineMap lm, const cpx<T>* __restrict__ twg,
                                                                     Prefac pf) {
   __shared__ __align__(16) unsigned char smem[STile<T, N, MODE>::LDS_BYTES];

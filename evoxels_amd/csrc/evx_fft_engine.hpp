@@ -159,7 +159,7 @@ __device__ __forceinline__ void stockham(cpx<T> (&v)[N / TPL], const Lay& lay, i
       }
       xsync<Lay::WAVE>();
 #pragma unroll
-      for (int q = 0; q < E; ++q) v[q].x = *lay.at(tl + TPL * q);
+      for (int q = 0; q < E; ++q) v[q].x = *lay.template at_q<TPL>(tl, q);
       xsync<Lay::WAVE>();
 #pragma unroll
       for (int s = 0; s < S; ++s) {
@@ -171,7 +171,7 @@ __device__ __forceinline__ void stockham(cpx<T> (&v)[N / TPL], const Lay& lay, i
       }
       xsync<Lay::WAVE>();
 #pragma unroll
-      for (int q = 0; q < E; ++q) v[q].y = *lay.at(tl + TPL * q);
+      for (int q = 0; q < E; ++q) v[q].y = *lay.template at_q<TPL>(tl, q);
     } else {
       xsync<Lay::WAVE>();  // previous readers of the exchange buffer are done
 #pragma unroll
@@ -184,7 +184,7 @@ __device__ __forceinline__ void stockham(cpx<T> (&v)[N / TPL], const Lay& lay, i
       }
       xsync<Lay::WAVE>();
 #pragma unroll
-      for (int q = 0; q < E; ++q) v[q] = *lay.at(tl + TPL * q);
+      for (int q = 0; q < E; ++q) v[q] = *lay.template at_q<TPL>(tl, q);
     }
     stockham<T, N, TPL, INV, P * R, Lay, Rest...>(v, lay, tl, tw);
   }

@@ -289,12 +289,20 @@ struct ZGeo {
 // N + N/16 slots.
 __host__ __device__ __forceinline__ int line_slot(int e) { return e + (e >> 4); }
 
+// Layouts give element e's slot (at) and element tl + TPL*q's slot
+// (at_q): for TPL a multiple of 16, line_slot(tl + TPL*q) = line_slot(tl) +
+// q*(TPL + TPL/16), one lane base plus constant offsets.
 template <typename T, int LS, bool W = false>
 struct LineLay {
   static constexpr bool SPLIT = false;
   static constexpr bool WAVE = W;
   cpx<T>* base;
   __device__ __forceinline__ cpx<T>* at(int e) const { return base + line_slot(e); }
+  template <int TPL>
+  __device__ __forceinline__ cpx<T>* at_q(int tl, int q) const {
+    if constexpr (TPL % 16 == 0) return base + line_slot(tl) + q * (TPL + TPL / 16);
+    else return at(tl + TPL * q);
+  }
 };
 
 template <typename T, int TL>
@@ -303,6 +311,8 @@ struct BrickLay {
   static constexpr bool WAVE = false;
   cpx<T>* base;
   __device__ __forceinline__ cpx<T>* at(int e) const { return base + e * TL; }
+  template <int TPL>
+  __device__ __forceinline__ cpx<T>* at_q(int tl, int q) const { return at(tl + TPL * q); }
 };
 
 // Brick of real numbers (one component at a time; see the engine).
@@ -312,6 +322,8 @@ struct BrickLaySplit {
   static constexpr bool WAVE = false;
   T* base;
   __device__ __forceinline__ T* at(int e) const { return base + e * TL; }
+  template <int TPL>
+  __device__ __forceinline__ T* at_q(int tl, int q) const { return at(tl + TPL * q); }
 };
 
 // LDS of one z tile: twiddles plus LP padded lines, and one pair per
@@ -486,13 +498,31 @@ struct SGeo {
   static constexpr bool SPLIT = (size_t)TL * N * sizeof(cpx<T>) > 32768;
 };
 
+// Float32 x sweeps (MODE >= 2) whose lines take one wavefront each (TPL =
+// 64): after the coalesced brick load the block transposes the brick once
+// through LDS, so that wavefront w owns line L0 + w and runs both transforms
+// with wavefront-private exchanges, and transposes back for the coalesced
+// store: two workgroup barriers per brick instead of one per exchange.
+// Lines sit LSX complex values apart, LSX = 2 mod 32, so that the 16 lines
+// a half-wavefront writes in the transposes start 4 banks apart.
+template <typename T, int N, int MODE>
+struct SWave {
+  using G = SGeo<T, N, MODE>;
+  static constexpr bool ON = MODE >= 2 && sizeof(T) == 4 && G::TPL == 64 && G::TL * 64 == G::THREADS;
+  static constexpr int LS0 = N + N / 16;  // line_slot(N - 1) + 1
+  static constexpr int LSX = LS0 + ((2 - LS0 % 32) + 32) % 32;
+  static_assert(!ON || G::TL == 16, "64 threads per line, 16 lines per brick: tl < 64");
+};
+
 // The forward*prefac*inverse sweeps (MODE 2, 3) are register-bound, not
 // LDS-bound, so they keep the full-size exchange.
 template <typename T, int N, int MODE>
 struct STile {
   static constexpr bool SPLIT = SGeo<T, N, MODE>::SPLIT && MODE < 2;
   static constexpr size_t LDS_BYTES =
-      sizeof(cpx<T>) * N + (SPLIT ? sizeof(T) : sizeof(cpx<T>)) * SGeo<T, N, MODE>::TL * N;
+      SWave<T, N, MODE>::ON
+          ? sizeof(cpx<T>) * (N + (size_t)SGeo<T, N, MODE>::TL * SWave<T, N, MODE>::LSX)
+          : sizeof(cpx<T>) * N + (SPLIT ? sizeof(T) : sizeof(cpx<T>)) * SGeo<T, N, MODE>::TL * N;
 };
 
 // Global side of one brick (TL adjacent lines): lane t serves line L0 + t,
@@ -596,8 +626,35 @@ __device__ __forceinline__ void strided_tile(cpx<T>* __restrict__ spec, int64_t 
   io.setup(spec, nlines, lm, (uint32_t)tile, t, tl);
   cpx<T> v[E];
   io.load(v);
-  __syncthreads();
-  brick_transform<T, N, MODE>(v, brick_layout<T, N, MODE>(lds, t), tl, tw, io.bl, pf);
+  if constexpr (SWave<T, N, MODE>::ON) {
+    constexpr int LSX = SWave<T, N, MODE>::LSX;
+    // brick -> lines: a thread holds elements tl + 64q of line t, at slot
+    // line_slot(tl + 64q) = line_slot(tl) + 68q
+    cpx<T>* const bt = lds + t * LSX + line_slot(tl);
+#pragma unroll
+    for (int q = 0; q < E; ++q) bt[68 * q] = v[q];
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
+    const LineLay<T, LSX, true> lay{lds + w * LSX};
+    const cpx<T>* const wl = lds + w * LSX + line_slot(l);
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = wl[68 * q];
+    // wavefront w transforms line L0 + w (line L0 past the last line: its
+    // results are not stored)
+    const uint32_t L0 = (uint32_t)tile * TL;
+    const BrickLine bl = brick_line(lm, L0, (int64_t)L0 + w < nlines ? (uint32_t)w : 0u);
+    brick_transform<T, N, MODE>(v, lay, l, tw, bl, pf);
+    xsync<true>();  // the wavefront's last exchange reads are done
+    cpx<T>* const wo = lds + w * LSX + line_slot(l);
+#pragma unroll
+    for (int q = 0; q < E; ++q) wo[68 * q] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = bt[68 * q];
+  } else {
+    __syncthreads();
+    brick_transform<T, N, MODE>(v, brick_layout<T, N, MODE>(lds, t), tl, tw, io.bl, pf);
+  }
   io.store(v);
 }
 
