@@ -67,6 +67,28 @@ def traffic_from_profiles(kernel):
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
+
+def energy_decay_leg(out, vg, stepper, c0):
+    """Free energy and mass over 500 steps of the headline workload, frames
+    every 50 steps: F must not increase and the mass must stay put."""
+    import torch
+    u = vg.init_scalar_field(c0).contiguous()
+    energies, masses = [float(vg.ch_energy(u, 3.0)[0])], [float(vg.channel_sums(u)[0])]
+    for k in range(1, 501):
+        stepper.step_into(u, u)
+        if k % 50 == 0:
+            energies.append(float(vg.ch_energy(u, 3.0)[0]))
+            masses.append(float(vg.channel_sums(u)[0]))
+    de = np.diff(energies)
+    out["energy_decay"] = {"steps": 500, "frame_every": 50, "free_energy": [round(e, 3) for e in energies],
+                           "non_increasing": bool(np.all(de <= 0)), "largest_frame_change": float(de.max()),
+                           "mass_rel_drift": float(np.abs(np.array(masses) - masses[0]).max() / abs(masses[0])),
+                           "functional": "int eps|grad c|^2 + 9/eps c^2(1-c)^2 dV, central gradients, float64 sums"}
+    log(f"[bench] energy decay: {out['energy_decay']}")
+    del u
+    torch.cuda.empty_cache()
+
+
 def cpu_baseline(c0, steps, gpu_step1=None):
     """Times the CPU oracle on `steps` steps (after one warm-up step).  With
     ``gpu_step1`` (the HIP step of c0, on the host) one float64 oracle step
@@ -671,6 +693,7 @@ def main():
     ap.add_argument("--no-inversion", action="store_true", help="skip the forward+adjoint inversion leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-grid leg")
     ap.add_argument("--no-frames", action="store_true", help="skip the frame-output leg")
+    ap.add_argument("--no-energy", action="store_true", help="skip the 500-step free-energy leg")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip configs 1-2, the next-row paths and the copy ceiling")
     ap.add_argument("--batch-steps", type=int, default=200)
@@ -802,21 +825,8 @@ def main():
         "mass_rel_drift": abs(mass1 - mass0) / abs(mass0),
     }
     # free-energy decay over the workload's 500 steps (frames every 50)
-    u = vg.init_scalar_field(c0).contiguous()
-    energies, masses = [float(vg.ch_energy(u, 3.0)[0])], [float(vg.channel_sums(u)[0])]
-    for k in range(1, 501):
-        stepper.step_into(u, u)
-        if k % 50 == 0:
-            energies.append(float(vg.ch_energy(u, 3.0)[0]))
-            masses.append(float(vg.channel_sums(u)[0]))
-    de = np.diff(energies)
-    out["energy_decay"] = {"steps": 500, "frame_every": 50, "free_energy": [round(e, 3) for e in energies],
-                           "non_increasing": bool(np.all(de <= 0)), "largest_frame_change": float(de.max()),
-                           "mass_rel_drift": float(np.abs(np.array(masses) - masses[0]).max() / abs(masses[0])),
-                           "functional": "int eps|grad c|^2 + 9/eps c^2(1-c)^2 dV, central gradients, float64 sums"}
-    log(f"[bench] energy decay: {out['energy_decay']}")
-    del u
-    torch.cuda.empty_cache()
+    if not args.no_energy:
+        energy_decay_leg(out, vg, stepper, c0)
     if not args.no_batched:
         try:
             out["batched"] = batched_leg(args.batch_steps, 10, 0 if args.no_cpu_baseline else 2)
