@@ -375,14 +375,6 @@ __device__ __forceinline__ void z_r2c_tile(const T* __restrict__ x, cpx<T>* __re
   }
 }
 
-template <typename T, int N>
-__device__ __forceinline__ cpx<T> herm_at(const cpx<T>* __restrict__ line, int k) {
-  constexpr int NH = N / 2 + 1;
-  cpx<T> v = k < NH ? line[k] : cpx<T>{line[N - k].x, -line[N - k].y};
-  if (k == 0 || 2 * k == N) v.y = T(0);
-  return v;
-}
-
 // z C2R of tile `tile`: out = base + scale * irfft(spec) (base may be NULL,
 // out may alias base).  With rec, the block also writes rec[tile]: the
 // compensated float64 sum of its outputs and their NaN / Inf flags (every
@@ -401,13 +393,49 @@ __device__ __forceinline__ void z_c2r_tile(const cpx<T>* __restrict__ spec, T* o
   const int64_t la = G::first_line(tile, pr);
   const bool has_a = la < nlines, has_b = la + 1 < nlines;
   const cpx<T>* sa = spec + la * pitch;
+  const LineLay<T, G::LS, G::WAVE> lay{lds + pr * G::LS};
   cpx<T> v[E];
+  // Each stored element of the two half spectra is read from HBM once and
+  // parked in the pair's exchange line: slot k holds A[k] and slot N/2 + k
+  // holds B[k] (0 < k < N/2); slot 0 holds (A[0], A[N/2]) and slot N/2 holds
+  // (B[0], B[N/2]), real parts only (a real line's DC and Nyquist terms are
+  // real: C2R drops their imaginary parts, as numpy's irfft does).
+  {
+    T* const d0 = &lay.at(0)->x;
+    T* const dh = &lay.at(N / 2)->x;
+#pragma unroll
+    for (int q = 0; q < (NH + TPL - 1) / TPL; ++q) {
+      const int k = tl + TPL * q;
+      if (k < NH) {
+        const cpx<T> A = has_a ? sa[k] : cpx<T>{T(0), T(0)};
+        const cpx<T> B = has_b ? sa[pitch + k] : cpx<T>{T(0), T(0)};
+        if (k == 0) {
+          d0[0] = A.x;
+          dh[0] = B.x;
+        } else if (k == N / 2) {
+          d0[1] = A.x;
+          dh[1] = B.x;
+        } else {
+          *lay.at(k) = A;
+          *lay.at(N / 2 + k) = B;
+        }
+      }
+    }
+  }
+  xsync<G::WAVE>();
+  // Z[k] = A[k] + i B[k] of the Hermitian extensions, A[N-k] = conj(A[k]).
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int k = tl + TPL * q;
-    const cpx<T> A = has_a ? herm_at<T, N>(sa, k) : cpx<T>{T(0), T(0)};
-    const cpx<T> B = has_b ? herm_at<T, N>(sa + pitch, k) : cpx<T>{T(0), T(0)};
-    v[q] = cpx<T>{A.x - B.y, A.y + B.x};
+    const int j = k <= N / 2 ? k : N - k;
+    if (j == 0 || j == N / 2) {
+      const cpx<T> a0 = *lay.at(0), ah = *lay.at(N / 2);
+      const T ax = j == 0 ? a0.x : a0.y, bx = j == 0 ? ah.x : ah.y;
+      v[q] = cpx<T>{ax - T(0), T(0) + bx};
+    } else {
+      const cpx<T> a = *lay.at(j), b = *lay.at(N / 2 + j);
+      v[q] = k < N / 2 ? cpx<T>{a.x - b.y, a.y + b.x} : cpx<T>{a.x + b.y, b.x - a.y};
+    }
   }
   // The state added to the update is loaded now so its latency overlaps the FFT.
   const T* ba = base ? base + la * N : nullptr;
@@ -419,7 +447,6 @@ __device__ __forceinline__ void z_c2r_tile(const cpx<T>* __restrict__ spec, T* o
     bv[q].y = (ba && has_b) ? ba[N + e] : T(0);
   }
   __syncthreads();  // twiddles
-  const LineLay<T, G::LS, G::WAVE> lay{lds + pr * G::LS};
   fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
   T* oa = out + la * N;
   double acc = 0.0;
