@@ -530,15 +530,16 @@ struct SGeo {
 // through LDS, so that wavefront w owns line L0 + w and runs both transforms
 // with wavefront-private exchanges, and transposes back for the coalesced
 // store: two workgroup barriers per brick instead of one per exchange.
-// Lines sit LSX complex values apart, LSX = 2 mod 32, so that the 16 lines
-// a half-wavefront writes in the transposes start 4 banks apart.
+// Lines sit LSX complex values apart, LSX = 32/TL mod 32, so that the TL
+// lines a half-wavefront writes in the transposes (64/TL consecutive
+// elements each) start 64/TL banks apart.
 template <typename T, int N, int MODE>
 struct SWave {
   using G = SGeo<T, N, MODE>;
   static constexpr bool ON = MODE >= 2 && sizeof(T) == 4 && G::TPL == 64 && G::TL * 64 == G::THREADS;
   static constexpr int LS0 = N + N / 16;  // line_slot(N - 1) + 1
-  static constexpr int LSX = LS0 + ((2 - LS0 % 32) + 32) % 32;
-  static_assert(!ON || G::TL == 16, "64 threads per line, 16 lines per brick: tl < 64");
+  static constexpr int LSX = LS0 + ((32 / G::TL - LS0 % 32) + 32) % 32;
+  static_assert(!ON || G::THREADS / G::TL == 64, "64 threads per line: tl < 64");
 };
 
 // The forward*prefac*inverse sweeps (MODE 2, 3) are register-bound, not
