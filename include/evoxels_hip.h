@@ -342,7 +342,8 @@ int evx_batch_ch_imex_step(evx_batch_plan* plan, void* state, void* workspace,
  * its i-th kernel, for as many kernels as there are events (at most 64).
  * The arming is consumed by that one step.  evx_fft_plan_trace_kinds then
  * gives the kernel of each recorded interval: 0 right-hand side, 1 z R2C,
- * 2 forward y, 3 x forward*prefac*inverse, 4 inverse y, 5 z C2R + add. */
+ * 2 forward y, 3 x forward*prefac*inverse, 4 inverse y, 5 z C2R + add,
+ * 6 right-hand side fused with the z R2C, 7 other. */
 int evx_fft_plan_trace_next_step(evx_fft_plan* plan, void* const* events,
                                  int n_events);
 int evx_fft_plan_trace_kinds(const evx_fft_plan* plan, int32_t* kinds,
