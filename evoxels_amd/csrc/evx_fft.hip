@@ -304,14 +304,37 @@ __global__ __launch_bounds__(RzGeo::THREADS) void ch_rhs_z_r2c_512(const float* 
   rz_tile_body(u, spec, m, g, W, chan, twg, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, lds);
 }
 
-// The wavefront-per-line x sweep keeps two bricks per CU: eight waves per
-// SIMD, 64 VGPRs.
 template <typename T, int N, int MODE>
-__global__ __launch_bounds__((SGeo<T, N, MODE>::THREADS), (SWave<T, N, MODE>::ON ? 8 : 1)) void strided_fast(cpx<T>* __restrict__ spec, int64_t nlines,
+__global__ __launch_bounds__((SGeo<T, N, MODE>::THREADS)) void strided_fast(cpx<T>* __restrict__ spec, int64_t nlines,
                                                                     LineMap lm, const cpx<T>* __restrict__ twg,
                                                                     Prefac pf) {
   __shared__ __align__(16) unsigned char smem[STile<T, N, MODE>::LDS_BYTES];
   strided_tile<T, N, MODE>(spec, nlines, lm, twg, pf, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, smem);
+}
+
+// The persistent x sweep: one resident round of blocks (98 VGPRs, one
+// 1024-thread block per CU) walking the bricks.
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__((SGeo<T, N, MODE>::THREADS), 4) void strided_x_pipe_kernel(
+    cpx<T>* __restrict__ spec, int64_t nlines, LineMap lm, const cpx<T>* __restrict__ twg, Prefac pf,
+    uint32_t nbricks) {
+  __shared__ __align__(16) unsigned char smem[STile<T, N, MODE>::LDS_BYTES];
+  strided_x_pipe<T, N, MODE>(spec, nlines, lm, twg, pf, nbricks, blockIdx.x, gridDim.x, threadIdx.x, smem);
+}
+
+template <typename T, int N, int MODE>
+int64_t x_pipe_resident() {
+  static const int64_t resident = [] {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, strided_x_pipe_kernel<T, N, MODE>,
+                                                     SGeo<T, N, MODE>::THREADS, 0) != hipSuccess ||
+        per < 1 || cus < 1)
+      return (int64_t)256;
+    return (int64_t)per * cus;
+  }();
+  return resident;
 }
 
 template <typename T>
@@ -362,8 +385,16 @@ bool fast_strided(int n, void* spec, int64_t nlines, const LineMap& lm, const vo
     using G = SGeo<T, NN, MODE>;                                                                            \
     if (!brick_offsets_fit<T>(nlines, lm, NN, G::TPL, G::TL)) return false;                                 \
     const int64_t blocks = (nlines + G::TL - 1) / G::TL;                                                    \
-    hipLaunchKernelGGL((strided_fast<T, NN, MODE>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,         \
-                       static_cast<cpx<T>*>(spec), nlines, lm, static_cast<const cpx<T>*>(tw), pf);          \
+    if constexpr (SWave<T, NN, MODE>::ON) {                                                                 \
+      const int64_t res = x_pipe_resident<T, NN, MODE>();                                                   \
+      const int64_t grid = blocks < res ? blocks : res;                                                     \
+      hipLaunchKernelGGL((strided_x_pipe_kernel<T, NN, MODE>), dim3((unsigned)grid), dim3(G::THREADS), 0, s, \
+                         static_cast<cpx<T>*>(spec), nlines, lm, static_cast<const cpx<T>*>(tw), pf,          \
+                         (uint32_t)blocks);                                                                 \
+    } else {                                                                                                \
+      hipLaunchKernelGGL((strided_fast<T, NN, MODE>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,       \
+                         static_cast<cpx<T>*>(spec), nlines, lm, static_cast<const cpx<T>*>(tw), pf);        \
+    }                                                                                                       \
     return true;                                                                                            \
   }
     EVX_FAST_SIZES(X)

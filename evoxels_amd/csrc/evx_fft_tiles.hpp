@@ -526,10 +526,11 @@ struct SGeo {
 };
 
 // Float32 x sweeps (MODE >= 2) whose lines take one wavefront each (TPL =
-// 64): after the coalesced brick load the block transposes the brick once
-// through LDS, so that wavefront w owns line L0 + w and runs both transforms
-// with wavefront-private exchanges, and transposes back for the coalesced
-// store: two workgroup barriers per brick instead of one per exchange.
+// 64), strided_x_pipe: after the coalesced brick load the block transposes
+// the brick once through LDS, so that wavefront w owns line L0 + w and runs
+// both transforms with wavefront-private exchanges, and transposes back for
+// the coalesced store: three workgroup barriers per brick instead of one per
+// exchange.
 // Lines sit LSX complex values apart, LSX = 32/TL mod 32, so that the TL
 // lines a half-wavefront writes in the transposes (64/TL consecutive
 // elements each) start 64/TL banks apart.
@@ -650,40 +651,73 @@ __device__ __forceinline__ void strided_tile(cpx<T>* __restrict__ spec, int64_t 
   cpx<T>* lds = tw + N;
   const int t = tid % TL, tl = tid / TL;
   for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
+  static_assert(!SWave<T, N, MODE>::ON, "the wavefront-per-line x sweeps run strided_x_pipe");
   BrickIo<T, N, MODE> io;
   io.setup(spec, nlines, lm, (uint32_t)tile, t, tl);
   cpx<T> v[E];
   io.load(v);
-  if constexpr (SWave<T, N, MODE>::ON) {
-    constexpr int LSX = SWave<T, N, MODE>::LSX;
-    // brick -> lines: a thread holds elements tl + 64q of line t, at slot
-    // line_slot(tl + 64q) = line_slot(tl) + 68q
-    cpx<T>* const bt = lds + t * LSX + line_slot(tl);
+  __syncthreads();
+  brick_transform<T, N, MODE>(v, brick_layout<T, N, MODE>(lds, t), tl, tw, io.bl, pf);
+  io.store(v);
+}
+
+// The wavefront-per-line x sweep, persistent: a block walks bricks first,
+// first + step, ... and loads the next brick into registers while it
+// transforms the current one from LDS, so that its loads and its transforms
+// overlap instead of every CU alternating between the two (a thread holds
+// elements tl + 64q of line t of a brick, at exchange slot line_slot(tl +
+// 64q) = line_slot(tl) + 68q).  Wavefront w transforms line L0 + w; line
+// L0 stands in past the last line, and its results are not stored.
+template <typename T, int N, int MODE>
+__device__ __forceinline__ void strided_x_pipe(cpx<T>* __restrict__ spec, int64_t nlines, const LineMap& lm,
+                                               const cpx<T>* __restrict__ twg, const Prefac& pf, uint32_t nbricks,
+                                               uint32_t first, uint32_t step, int tid, unsigned char* smem) {
+  using G = SGeo<T, N, MODE>;
+  static_assert(SWave<T, N, MODE>::ON, "wavefront-per-line x sweeps only");
+  constexpr int TL = G::TL, E = N / G::TPL, LSX = SWave<T, N, MODE>::LSX;
+  cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* lds = tw + N;
+  const int t = tid % TL, tl = tid / TL;
+  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
+  if (first >= nbricks) return;  // uniform in the block
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
+  cpx<T>* const bt = lds + t * LSX + line_slot(tl);
+  cpx<T>* const wl = lds + w * LSX + line_slot(l);
+  const LineLay<T, LSX, true> lay{lds + w * LSX};
+  uint32_t tile = first;
+  BrickIo<T, N, MODE> io;
+  io.setup(spec, nlines, lm, tile, t, tl);
+  cpx<T> v[E];
+  io.load(v);
+  while (true) {
 #pragma unroll
     for (int q = 0; q < E; ++q) bt[68 * q] = v[q];
-    __syncthreads();
-    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
-    const LineLay<T, LSX, true> lay{lds + w * LSX};
-    const cpx<T>* const wl = lds + w * LSX + line_slot(l);
+    __syncthreads();  // the brick (and the twiddles) are in LDS
+    const uint32_t next = tile + step;
+    const bool more = next < nbricks;  // uniform in the block
+    BrickIo<T, N, MODE> io2;
+    if (more) {
+      io2.setup(spec, nlines, lm, next, t, tl);
+      io2.load(v);  // in flight during the transform below
+    }
+    cpx<T> x[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = wl[68 * q];
-    // wavefront w transforms line L0 + w (line L0 past the last line: its
-    // results are not stored)
-    const uint32_t L0 = (uint32_t)tile * TL;
+    for (int q = 0; q < E; ++q) x[q] = wl[68 * q];
+    const uint32_t L0 = tile * TL;
     const BrickLine bl = brick_line(lm, L0, (int64_t)L0 + w < nlines ? (uint32_t)w : 0u);
-    brick_transform<T, N, MODE>(v, lay, l, tw, bl, pf);
+    brick_transform<T, N, MODE>(x, lay, l, tw, bl, pf);
     xsync<true>();  // the wavefront's last exchange reads are done
-    cpx<T>* const wo = lds + w * LSX + line_slot(l);
 #pragma unroll
-    for (int q = 0; q < E; ++q) wo[68 * q] = v[q];
+    for (int q = 0; q < E; ++q) wl[68 * q] = x[q];
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = bt[68 * q];
-  } else {
-    __syncthreads();
-    brick_transform<T, N, MODE>(v, brick_layout<T, N, MODE>(lds, t), tl, tw, io.bl, pf);
+    for (int q = 0; q < E; ++q) x[q] = bt[68 * q];
+    io.store(x);
+    if (!more) break;
+    __syncthreads();  // every read of this brick's LDS is done before the next one is written
+    io = io2;
+    tile = next;
   }
-  io.store(v);
 }
 
 }  // namespace evx

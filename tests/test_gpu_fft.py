@@ -83,12 +83,15 @@ def oracle_imex(kind, arr, spacing_abs, precision, dt, kw):
                                            ((8, 16, 128), (1.0, 0.9, 1.1)), ((4, 8, 256), (1.0, 1.0, 1.0)),
                                            ((12, 8, 512), (1.0, 1.0, 1.0)), ((4, 4, 1024), (2.0, 2.0, 2.0)),
                                            ((512, 4, 6), (1.0, 1.0, 1.0)), ((6, 512, 4), (1.0, 1.0, 1.0)),
-                                           ((1024, 2, 4), (1.0, 1.0, 1.0)), ((512, 5, 6), (1.0, 1.1, 0.9))])
+                                           ((1024, 2, 4), (1.0, 1.0, 1.0)), ((512, 5, 6), (1.0, 1.1, 0.9)),
+                                           ((512, 24, 340), (1.0, 1.0, 1.0))])
 @pytest.mark.parametrize("kind", ["ch", "rd"])
 def test_imex_step_matches_oracle(kind, shape, spacing, precision):
     """One semi-implicit step against the FP64 oracle; the long-x shapes run
     the 512- and 1024-point forward*multiplier*inverse x sweeps ((512, 5, 6):
-    20 x lines, a second brick with 4 of its 16 lines present)."""
+    20 x lines, a second brick with 4 of its 16 lines present; (512, 24,
+    340): 257 bricks, more than one resident round of the persistent 512-point
+    x sweep, the last brick partial)."""
     kw = {"eps": 3.0, "D": 1.0} if kind == "ch" else {"D": 0.8}
     dt = 0.1
     vf, vg, prob, step, arr = imex_pair(kind, shape, precision, spacing, 3, dt, **kw)
