@@ -767,6 +767,12 @@ def main():
     ms_step = t_start.elapsed_time(t_end) / K
     mass1 = float(vg.channel_sums(u)[0])
     log(f"[bench] {ms_step:.3f} ms/step (events), wall {1e3 * wall / K:.3f} ms/step")
+    # the standalone compensated float64 mass + NaN/Inf pass (evx_sum_f64),
+    # which frames use when no step records are available
+    sum_ms = _time_steps(lambda: vg.channel_sums(u), 20, reps=2)
+    sum_f64 = {"ms": round(sum_ms, 4), "gbps": round(4 * n ** 3 / (sum_ms * 1e-3) / 1e9, 1),
+               "what": f"evx_sum_f64 of the {n}^3 float32 field (compensated float64 sum and NaN/Inf flags: "
+                       "partial and fold kernels), best mean of 2 x 20 calls"}
     nint = ctypes.c_int32()
     _lib.check(lib.evx_fft_plan_trace_kinds(plan.handle, None, ctypes.byref(nint)), "trace kinds")
     kinds = (ctypes.c_int32 * max(1, nint.value))()
@@ -823,6 +829,7 @@ def main():
         "sum_kernel_ms": round(float(mean_ms.sum()), 4),
         "wall_ms_per_step": round(1e3 * wall / K, 4),
         "mass_rel_drift": abs(mass1 - mass0) / abs(mass0),
+        "sum_f64": sum_f64,
     }
     # free-energy decay over the workload's 500 steps (frames every 50)
     if not args.no_energy:
