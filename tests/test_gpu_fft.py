@@ -261,12 +261,14 @@ def test_slab_ordering_matches_whole_grid_sweeps(shape, slabs, precision):
 @pytest.mark.parametrize("precision", ["float32", "float64"])
 @pytest.mark.parametrize("shape,spacing", [((16, 16, 16), (1.0, 1.0, 1.0)), ((12, 10, 14), (0.8, 1.2, 1.0)),
                                            ((20, 9, 33), (1.0, 1.0, 1.0)), ((8, 16, 128), (1.0, 0.9, 1.1)),
-                                           ((12, 8, 512), (1.0, 1.0, 1.0))])
+                                           ((12, 8, 512), (1.0, 1.0, 1.0)), ((512, 5, 6), (1.0, 1.1, 0.9)),
+                                           ((512, 24, 340), (1.0, 1.0, 1.0))])
 @pytest.mark.parametrize("kind", ["ch", "rd"])
 def test_etd1_step_matches_oracle(kind, shape, spacing, precision):
     """ExponentialEuler (timesteppers.py:136-199) through the fused ETD1
     pipeline against the float64 oracle with the reference's float32
-    multiplier."""
+    multiplier (the long-x shapes: the 512-point exponential x sweep over
+    one partial brick and over 257 bricks)."""
     kw = {"eps": 3.0, "D": 1.0} if kind == "ch" else {"D": 0.8}
     dt = 0.4
     vf, vg = grid_for(shape, precision, spacing)
