@@ -114,17 +114,25 @@ class BatchedCahnHilliard:
     # ------------------------------------------------------------ tiles
     @property
     def schedule(self):
-        x, ni = C.c_int(), C.c_int()
-        _lib.check(_lib.load().evx_batch_get_tiles(self.handle, C.byref(x), C.byref(ni)), "evx_batch_get_tiles")
-        return {"launches_per_step": 6, "xchunk": x.value, "items": ni.value}
+        x, ni, g = C.c_int(), C.c_int(), C.c_int()
+        lib = _lib.load()
+        _lib.check(lib.evx_batch_get_tiles(self.handle, C.byref(x), C.byref(ni)), "evx_batch_get_tiles")
+        _lib.check(lib.evx_batch_get_fft_group(self.handle, C.byref(g)), "evx_batch_get_fft_group")
+        return {"launches_per_step": 6, "xchunk": x.value, "fft_group": g.value, "items": ni.value}
 
     _generation = 0
 
-    def set_tiles(self, xchunk=8):
+    def set_tiles(self, xchunk=8, fft_group=None):
+        """Tile shape: x planes per CH tile, and (if given) the number of
+        consecutive FFT tiles each workgroup of the FFT sweeps covers."""
         torch.cuda.synchronize(self.device)
-        if int(xchunk) != self.schedule["xchunk"]:
+        now = self.schedule
+        if int(xchunk) != now["xchunk"] or (fft_group is not None and int(fft_group) != now["fft_group"]):
             self._generation += 1
-        _lib.check(_lib.load().evx_batch_set_tiles(self.handle, int(xchunk)), "evx_batch_set_tiles")
+        lib = _lib.load()
+        _lib.check(lib.evx_batch_set_tiles(self.handle, int(xchunk)), "evx_batch_set_tiles")
+        if fft_group is not None:
+            _lib.check(lib.evx_batch_set_fft_group(self.handle, int(fft_group)), "evx_batch_set_fft_group")
 
     def autotune(self, state, steps: int = 5):
         """Times the candidate tile shapes on a copy of ``state`` and keeps the

@@ -6,8 +6,10 @@
 // into tiles; an item is (grid, sweep, tile).  A step is six launches, one
 // per sweep, each covering the items of every grid (one workgroup per item,
 // whatever the grid's shape), so small grids share the machine instead of
-// each paying its own launches.  The host keeps the item list sweep-major;
-// the x-chunk of the CH tiles is chosen by the autotuner.
+// each paying its own launches.  The host keeps the item list sweep-major.
+// Two tile-shape parameters are chosen by the autotuner: the x-chunk of the
+// CH tiles, and the FFT group - how many consecutive z tiles or y/x bricks
+// of one grid an FFT item covers (a workgroup runs them back to back).
 //
 // The tile bodies are the ones of the single-grid kernels.
 #include <cmath>
@@ -72,9 +74,21 @@ struct BArgs {
   const C* tw;
 };
 
+// Tiles first .. first+count-1 of one sweep run back to back, a workgroup
+// barrier between them (each tile body reuses the LDS).  The loop sits
+// inside each size case, so only that case's set-up stays live across it.
+template <typename F>
+__device__ __forceinline__ void tile_run(int first, int count, F&& body) {
+#pragma unroll 1
+  for (int j = 0; j < count; ++j) {
+    if (j > 0) __syncthreads();
+    body(first + j);
+  }
+}
+
 template <int MODE>
-__device__ __forceinline__ void strided_dispatch(const BInst& b, C* spec, const C* tw, int axis, int tile, int tid,
-                                                 unsigned char* smem) {
+__device__ __forceinline__ void strided_dispatch(const BInst& b, C* spec, const C* tw, int axis, int tile, int count,
+                                                 int tid, unsigned char* smem) {
   LineMap lm;
   lm.nh = b.nh;
   int64_t nlines;
@@ -101,7 +115,7 @@ __device__ __forceinline__ void strided_dispatch(const BInst& b, C* spec, const 
   switch (n) {
 #define X(N)                                                            \
   case N:                                                               \
-    strided_tile<T, N, MODE>(s, nlines, lm, t, b.pf, tile, tid, smem); \
+    tile_run(tile, count, [&](int k) { strided_tile<T, N, MODE>(s, nlines, lm, t, b.pf, k, tid, smem); }); \
     break;
     EVX_BATCH_SIZES(X)
 #undef X
@@ -110,10 +124,14 @@ __device__ __forceinline__ void strided_dispatch(const BInst& b, C* spec, const 
   }
 }
 
-// One item of the batched step.
+// One item: it = {grid, sweep, first tile, tile count}.  GROUP = false:
+// items of one tile (the count is not read, and the tile loops vanish at
+// compile time); the grouped kernel is a separate instantiation, so its
+// register needs never touch the one-tile schedule.
+template <bool GROUP>
 __device__ __forceinline__ void run_item(const BArgs& a, int4 it, int tid, unsigned char* smem) {
   const BInst& b = a.inst[it.x];
-  const int tile = it.z;
+  const int tile = it.z, count = GROUP ? it.w : 1;
   switch (it.y) {
     case 0: {
       const Dims d{1, b.nx, b.ny, b.nz};
@@ -127,7 +145,7 @@ __device__ __forceinline__ void run_item(const BArgs& a, int4 it, int tid, unsig
       switch (b.nz) {
 #define X(N)                                                                                                   \
   case N:                                                                                                      \
-    z_r2c_tile<T, N>(a.rhs + b.r_off, a.spec + b.s_off, nlines, b.nh, a.tw + b.tw[2], tile, tid, smem); \
+    tile_run(tile, count, [&](int k) { z_r2c_tile<T, N>(a.rhs + b.r_off, a.spec + b.s_off, nlines, b.nh, a.tw + b.tw[2], k, tid, smem); }); \
     break;
         EVX_BATCH_SIZES(X)
 #undef X
@@ -137,13 +155,13 @@ __device__ __forceinline__ void run_item(const BArgs& a, int4 it, int tid, unsig
       break;
     }
     case 2:
-      strided_dispatch<0>(b, a.spec, a.tw, 1, tile, tid, smem);
+      strided_dispatch<0>(b, a.spec, a.tw, 1, tile, count, tid, smem);
       break;
     case 3:
-      strided_dispatch<2>(b, a.spec, a.tw, 0, tile, tid, smem);
+      strided_dispatch<2>(b, a.spec, a.tw, 0, tile, count, tid, smem);
       break;
     case 4:
-      strided_dispatch<1>(b, a.spec, a.tw, 1, tile, tid, smem);
+      strided_dispatch<1>(b, a.spec, a.tw, 1, tile, count, tid, smem);
       break;
     default: {
       const int64_t nlines = (int64_t)b.nx * b.ny;
@@ -151,7 +169,7 @@ __device__ __forceinline__ void run_item(const BArgs& a, int4 it, int tid, unsig
       switch (b.nz) {
 #define X(N)                                                                                              \
   case N:                                                                                                 \
-    z_c2r_tile<T, N>(a.spec + b.s_off, u, u, T(1), nlines, b.nh, a.tw + b.tw[2], tile, tid, smem); \
+    tile_run(tile, count, [&](int k) { z_c2r_tile<T, N>(a.spec + b.s_off, u, u, T(1), nlines, b.nh, a.tw + b.tw[2], k, tid, smem); }); \
     break;
         EVX_BATCH_SIZES(X)
 #undef X
@@ -163,9 +181,10 @@ __device__ __forceinline__ void run_item(const BArgs& a, int4 it, int tid, unsig
   }
 }
 
+template <bool GROUP>
 __global__ __launch_bounds__(BT) void batch_stage(BArgs a, int first) {
   __shared__ __align__(16) unsigned char smem[BATCH_LDS];
-  run_item(a, a.items[first + (int)blockIdx.x], threadIdx.x, smem);
+  run_item<GROUP>(a, a.items[first + (int)blockIdx.x], threadIdx.x, smem);
 }
 
 size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
@@ -222,6 +241,7 @@ struct evx_batch_plan {
   size_t ws_bytes = 0;
   // schedule
   int xchunk = 8;   // x planes per CH tile
+  int fft_group = 1;  // consecutive tiles per FFT item
   int n_items = 0;
   int items_cap = 0;  // capacity of d_items: the item count of the finest tiling (one x plane per CH tile)
   int stage_first[evx::NSTAGE + 1] = {0};
@@ -243,7 +263,8 @@ void free_plan(evx_batch_plan* p) {
   delete p;
 }
 
-// Tile counts of every sweep for the current x-chunk, and the item list.
+// Tile counts of every sweep for the current x-chunk, and the item list
+// (FFT sweeps: groups of fft_group consecutive tiles per item).
 int build_items(evx_batch_plan* p) {
   std::vector<int4> items;
   for (int s = 0; s < NSTAGE; ++s) {
@@ -275,7 +296,8 @@ int build_items(evx_batch_plan* p) {
       }
       if (nt > (1 << 30)) return fail(EVX_ERR_UNSUPPORTED, "too many tiles in one sweep");
       b.nt[s] = (int)nt;
-      for (int t = 0; t < (int)nt; ++t) items.push_back(int4{i, s, t, 0});
+      const int g = s == 0 ? 1 : p->fft_group;
+      for (int t = 0; t < (int)nt; t += g) items.push_back(int4{i, s, t, (int)nt - t < g ? (int)nt - t : g});
     }
   }
   p->stage_first[NSTAGE] = (int)items.size();
@@ -321,7 +343,10 @@ int launch_step(evx_batch_plan* p, void* state, void* ws, hipStream_t s) {
   for (int st = 0; st < NSTAGE; ++st) {
     const int n = p->stage_first[st + 1] - p->stage_first[st];
     if (n == 0) continue;
-    hipLaunchKernelGGL(batch_stage, dim3(n), dim3(BT), 0, s, a, p->stage_first[st]);
+    if (st > 0 && p->fft_group > 1)
+      hipLaunchKernelGGL(batch_stage<true>, dim3(n), dim3(BT), 0, s, a, p->stage_first[st]);
+    else
+      hipLaunchKernelGGL(batch_stage<false>, dim3(n), dim3(BT), 0, s, a, p->stage_first[st]);
     EVX_TRY(check_launch("batch_stage"));
   }
   return EVX_OK;
@@ -469,6 +494,23 @@ extern "C" int evx_batch_set_tiles(evx_batch_plan* p, int xchunk) {
   return EVX_OK;
 }
 
+extern "C" int evx_batch_set_fft_group(evx_batch_plan* p, int tiles_per_item) {
+  if (!p) return fail(EVX_ERR_INVALID, "null plan");
+  if (tiles_per_item < 1 || tiles_per_item > 64) return fail(EVX_ERR_INVALID, "tiles_per_item must be in [1, 64]");
+  // As for evx_batch_set_tiles: the previous launch must be complete.
+  if (tiles_per_item != p->fft_group) {
+    p->fft_group = tiles_per_item;
+    EVX_TRY(build_items(p));
+  }
+  return EVX_OK;
+}
+
+extern "C" int evx_batch_get_fft_group(const evx_batch_plan* p, int* tiles_per_item) {
+  if (!p || !tiles_per_item) return fail(EVX_ERR_INVALID, "null argument");
+  *tiles_per_item = p->fft_group;
+  return EVX_OK;
+}
+
 extern "C" int evx_batch_get_tiles(const evx_batch_plan* p, int* xchunk, int* n_items) {
   if (!p || !xchunk || !n_items) return fail(EVX_ERR_INVALID, "null argument");
   *xchunk = p->xchunk;
@@ -482,9 +524,27 @@ extern "C" int evx_batch_ch_imex_step(evx_batch_plan* p, void* state, void* work
   return launch_step(p, state, workspace, static_cast<hipStream_t>(stream));
 }
 
-// Autotuner: times each candidate tile shape (x planes per CH tile, which
-// sets the number of CH items and the halo overhead of each) over `steps`
-// steps with HIP events after one warm-up step, and keeps the fastest.  The candidates run on the caller's scratch state, which is
+// Mean time per step of `steps` steps on the current schedule, after one
+// warm-up step (HIP events; synchronises the stream).
+static int time_schedule(evx_batch_plan* p, void* state, void* ws, int steps, hipStream_t s, hipEvent_t e0,
+                         hipEvent_t e1, double* ms_per_step) {
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(EVX_ERR_HIP, "stream synchronisation failed");
+  EVX_TRY(launch_step(p, state, ws, s));
+  if (hipEventRecord(e0, s) != hipSuccess) return fail(EVX_ERR_HIP, "hipEventRecord failed");
+  for (int i = 0; i < steps; ++i) EVX_TRY(launch_step(p, state, ws, s));
+  if (hipEventRecord(e1, s) != hipSuccess) return fail(EVX_ERR_HIP, "hipEventRecord failed");
+  if (hipEventSynchronize(e1) != hipSuccess) return fail(EVX_ERR_HIP, "event synchronisation failed");
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return fail(EVX_ERR_HIP, "hipEventElapsedTime failed");
+  *ms_per_step = ms / steps;
+  return EVX_OK;
+}
+
+// Autotuner: times the candidate tile shapes over `steps` steps each and
+// keeps the fastest: first the x-chunk of the CH tiles (x planes per tile,
+// which sets the number of CH items and the halo overhead of each) at the
+// current FFT group, then the FFT group (tiles per FFT item) at that
+// x-chunk.  The candidates run on the caller's scratch state, which is
 // overwritten.  Synchronises the stream (set-up time only).
 extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* workspace, int steps, void* stream,
                                   double* best_ms) {
@@ -492,7 +552,10 @@ extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* 
   if (!p->params_set) return fail(EVX_ERR_INVALID, "evx_batch_set_params has not been called");
   if (steps < 1 || steps > 1000) return fail(EVX_ERR_INVALID, "steps must be in [1, 1000]");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const int cands[] = {4, 8, 16, 32, 64};
+  // the item list is rewritten between candidates: nothing may be in flight
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(EVX_ERR_HIP, "stream synchronisation failed");
+  const int xcands[] = {4, 8, 16, 32, 64};
+  const int gcands[] = {1, 2, 4, 8};
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return fail(EVX_ERR_HIP, "hipEventCreate failed");
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -500,28 +563,34 @@ extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* 
     return fail(EVX_ERR_HIP, "hipEventCreate failed");
   }
   double best = 1e300;
-  int pick = p->xchunk;
+  int xpick = p->xchunk, gpick = p->fft_group;
   int st = EVX_OK;
-  for (const int c : cands) {
-    if (hipStreamSynchronize(s) != hipSuccess) { st = fail(EVX_ERR_HIP, "stream synchronisation failed"); break; }
+  for (const int c : xcands) {
+    double ms = 0.0;
     if ((st = evx_batch_set_tiles(p, c)) != EVX_OK) break;
-    if ((st = launch_step(p, scratch_state, workspace, s)) != EVX_OK) break;
-    if (hipEventRecord(e0, s) != hipSuccess) { st = fail(EVX_ERR_HIP, "hipEventRecord failed"); break; }
-    for (int i = 0; i < steps && st == EVX_OK; ++i) st = launch_step(p, scratch_state, workspace, s);
+    if ((st = time_schedule(p, scratch_state, workspace, steps, s, e0, e1, &ms)) != EVX_OK) break;
+    if (ms < best) {
+      best = ms;
+      xpick = c;
+    }
+  }
+  if (st == EVX_OK) st = evx_batch_set_tiles(p, xpick);
+  for (const int g : gcands) {
     if (st != EVX_OK) break;
-    if (hipEventRecord(e1, s) != hipSuccess) { st = fail(EVX_ERR_HIP, "hipEventRecord failed"); break; }
-    if (hipEventSynchronize(e1) != hipSuccess) { st = fail(EVX_ERR_HIP, "event synchronisation failed"); break; }
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { st = fail(EVX_ERR_HIP, "hipEventElapsedTime failed"); break; }
-    if (ms / steps < best) {
-      best = ms / steps;
-      pick = c;
+    double ms = 0.0;
+    if ((st = evx_batch_set_fft_group(p, g)) != EVX_OK) break;
+    if ((st = time_schedule(p, scratch_state, workspace, steps, s, e0, e1, &ms)) != EVX_OK) break;
+    if (ms < best) {
+      best = ms;
+      gpick = g;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (st != EVX_OK) return st;
-  EVX_TRY(evx_batch_set_tiles(p, pick));
+  if (hipStreamSynchronize(s) != hipSuccess) return fail(EVX_ERR_HIP, "stream synchronisation failed");
+  EVX_TRY(evx_batch_set_tiles(p, xpick));
+  EVX_TRY(evx_batch_set_fft_group(p, gpick));
   if (best_ms) *best_ms = best;
   return EVX_OK;
 }

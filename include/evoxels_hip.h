@@ -324,9 +324,15 @@ int evx_batch_set_params(evx_batch_plan* plan, const double* eps_D, double dt,
  * flight. */
 int evx_batch_set_tiles(evx_batch_plan* plan, int xchunk);
 int evx_batch_get_tiles(const evx_batch_plan* plan, int* xchunk, int* n_items);
-/* Times every candidate tile shape on scratch_state (overwritten) and keeps
- * the fastest; best_ms (may be NULL) receives its time per step.
- * Synchronises the stream. */
+/* Consecutive z tiles or y/x bricks of one grid per workgroup of the five
+ * FFT sweeps (1..64, default 1; a workgroup runs them back to back).  The
+ * results do not depend on it.  Set-up call, as evx_batch_set_tiles. */
+int evx_batch_set_fft_group(evx_batch_plan* plan, int tiles_per_item);
+int evx_batch_get_fft_group(const evx_batch_plan* plan, int* tiles_per_item);
+/* Times the candidate tile shapes on scratch_state (overwritten) and keeps
+ * the fastest: the Cahn-Hilliard x-chunk (4, 8, 16, 32, 64), then the FFT
+ * group (1, 2, 4, 8) at that x-chunk; best_ms (may be NULL) receives the
+ * chosen schedule's time per step.  Synchronises the stream. */
 int evx_batch_autotune(evx_batch_plan* plan, void* scratch_state,
                        void* workspace, int steps, void* stream,
                        double* best_ms);

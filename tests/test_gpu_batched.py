@@ -78,6 +78,27 @@ def test_batched_grids_equal_single_grid_steps(reference, mode, xchunk):
     compare(b.views(state), want, f"{mode}/{xchunk}")
 
 
+def test_batched_fft_group_is_bitwise_neutral(reference):
+    """FFT items covering 3 (partial last groups) or 8 consecutive tiles give
+    the same bits as one tile per item: a workgroup runs the same tile bodies
+    back to back."""
+    arrs, eps, D, want = reference
+    b = BatchedCahnHilliard(SHAPES, DT, eps=eps, D=D)
+    outs = {}
+    for g in (1, 3, 8):
+        b.set_tiles(8, fft_group=g)
+        sched = b.schedule
+        assert sched["fft_group"] == g
+        state = b.pack(arrs)
+        b.step(state, 3)
+        torch.cuda.synchronize()
+        print(f"fft_group {g}: {sched['items']} items", flush=True)
+        outs[g] = [v.clone() for v in b.views(state)]
+    assert all(torch.equal(a, c) for a, c in zip(outs[1], outs[3]))
+    assert all(torch.equal(a, c) for a, c in zip(outs[1], outs[8]))
+    compare(outs[3], want, "fft_group 3")
+
+
 def test_batched_graph_replay(reference):
     arrs, eps, D, want = reference
     b = BatchedCahnHilliard(SHAPES, DT, eps=eps, D=D)
@@ -103,7 +124,8 @@ def test_batched_autotune(reference):
     t0 = time.perf_counter()
     sched = b.autotune(state, steps=1)
     print(f"autotune {time.perf_counter() - t0:.3f} s -> {sched}", flush=True)
-    assert sched["xchunk"] in (4, 8, 16, 32, 64) and sched["ms_per_step"] > 0
+    assert sched["xchunk"] in (4, 8, 16, 32, 64) and sched["fft_group"] in (1, 2, 4, 8)
+    assert sched["ms_per_step"] > 0
     # the autotuner ran on a copy: state is untouched
     assert all(torch.equal(v, torch.tensor(a, dtype=torch.float32, device="cuda"))
                for v, a in zip(b.views(state), arrs))
