@@ -115,7 +115,12 @@ __device__ __forceinline__ void xsync() {
   }
 }
 
-template <typename T, int N, int TPL, bool INV, int P, typename Lay, int R, int... Rest>
+// Twiddles of a pass come from the plain table (PT < 0: tw[j] = w_N^j, read
+// at r*k*N/(P*R)) or from per-pass tables (PT >= 0: pass tables laid out
+// pass after pass from offset 0, entry (r-1)*P + k of a pass = w_N^(r*k*N/(P*R)),
+// filled by PassTw), where lanes with consecutive k read consecutive entries
+// instead of entries r*k*N/(P*R) apart (bank conflicts).
+template <typename T, int N, int TPL, bool INV, int P, int PT, typename Lay, int R, int... Rest>
 __device__ __forceinline__ void stockham(cpx<T> (&v)[N / TPL], const Lay& lay, int tl, const cpx<T>* __restrict__ tw) {
   constexpr int E = N / TPL;
   constexpr int S = E / R;
@@ -131,7 +136,7 @@ __device__ __forceinline__ void stockham(cpx<T> (&v)[N / TPL], const Lay& lay, i
       const int k = (tl + s * TPL) % P;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        const cpx<T> w = tw[r * k * STR];
+        const cpx<T> w = PT < 0 ? tw[r * k * STR] : tw[(PT < 0 ? 0 : PT) + (r - 1) * P + k];
         x[r] = INV ? cmulc(x[r], w) : cmul(x[r], w);
       }
     }
@@ -186,13 +191,51 @@ __device__ __forceinline__ void stockham(cpx<T> (&v)[N / TPL], const Lay& lay, i
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = *lay.template at_q<TPL>(tl, q);
     }
-    stockham<T, N, TPL, INV, P * R, Lay, Rest...>(v, lay, tl, tw);
+    stockham<T, N, TPL, INV, P * R, (PT < 0 ? -1 : PT + (P > 1 ? (R - 1) * P : 0)), Lay, Rest...>(v, lay, tl, tw);
   }
 }
 
 template <typename T, int N, int TPL, bool INV, typename Lay, int... R>
 __device__ __forceinline__ void fft_regs(cpx<T> (&v)[N / TPL], const Lay& lay, int tl, const cpx<T>* tw, Rad<R...>) {
-  stockham<T, N, TPL, INV, 1, Lay, R...>(v, lay, tl, tw);
+  stockham<T, N, TPL, INV, 1, -1, Lay, R...>(v, lay, tl, tw);
+}
+// The same with the per-pass twiddle tables of PassTw<N, 1, R...> at tw.
+template <typename T, int N, int TPL, bool INV, typename Lay, int... R>
+__device__ __forceinline__ void fft_regs_pt(cpx<T> (&v)[N / TPL], const Lay& lay, int tl, const cpx<T>* tw,
+                                            Rad<R...>) {
+  stockham<T, N, TPL, INV, 1, 0, Lay, R...>(v, lay, tl, tw);
+}
+
+// Per-pass twiddle tables of an N-point transform with radices R...: the
+// pass of stride P and radix R holds (R-1)*P entries (none for P = 1).
+template <int N, int P, int... R>
+struct PassTw;
+template <int N, int P>
+struct PassTw<N, P> {
+  static constexpr int SIZE = 0;
+  template <typename T>
+  __device__ static void fill(cpx<T>*, const cpx<T>*, int, int) {}
+};
+template <int N, int P, int R, int... Rest>
+struct PassTw<N, P, R, Rest...> {
+  static constexpr int HERE = P > 1 ? (R - 1) * P : 0;
+  static constexpr int SIZE = HERE + PassTw<N, P * R, Rest...>::SIZE;
+  // dst[i] from the plain table twg (twg[j] = w_N^j), threads tid of nt
+  template <typename T>
+  __device__ static void fill(cpx<T>* dst, const cpx<T>* twg, int tid, int nt) {
+    constexpr int STR = N / (P * R);
+    for (int i = tid; i < HERE; i += nt) {
+      const int r = 1 + i / P, k = i % P;
+      dst[i] = twg[r * k * STR];
+    }
+    PassTw<N, P * R, Rest...>::template fill<T>(dst + HERE, twg, tid, nt);
+  }
+};
+template <int N, int... R>
+constexpr int pass_tw_size(Rad<R...>) { return PassTw<N, 1, R...>::SIZE; }
+template <typename T, int N, int... R>
+__device__ __forceinline__ void pass_tw_fill(cpx<T>* dst, const cpx<T>* twg, int tid, int nt, Rad<R...>) {
+  PassTw<N, 1, R...>::template fill<T>(dst, twg, tid, nt);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; remap

@@ -607,12 +607,14 @@ __device__ __forceinline__ BrickLayout<T, N, MODE> brick_layout(cpx<T>* lds, int
 // In-register transform of a brick: forward (MODE 0, 2-4) or inverse (1);
 // MODE >= 2 multiplies by the step's multiplier and transforms back.  The
 // caller has synchronised the block since the exchange buffer's last use.
-template <typename T, int N, int MODE, int E, typename Lay>
+template <typename T, int N, int MODE, bool PTW = false, int E, typename Lay>
 __device__ __forceinline__ void brick_transform(cpx<T> (&v)[E], const Lay& lay, int tl, const cpx<T>* tw,
                                                 const BrickLine& bl, const Prefac& pf) {
   constexpr int TPL = SGeo<T, N, MODE>::TPL;
   static_assert(E == N / TPL, "one brick's values per thread");
-  fft_regs<T, N, TPL, MODE == 1>(v, lay, tl, tw, typename FftSpec<N>::R{});
+  // PTW: tw holds the per-pass twiddle tables (pass_tw_fill), else the plain one
+  if constexpr (PTW) fft_regs_pt<T, N, TPL, MODE == 1>(v, lay, tl, tw, typename FftSpec<N>::R{});
+  else fft_regs<T, N, TPL, MODE == 1>(v, lay, tl, tw, typename FftSpec<N>::R{});
   if constexpr (MODE >= 2) {
     // x lines: G = ny, so (k, o, g) = (jz, jy, channel)
     const int jz = (int)bl.k;
@@ -633,7 +635,8 @@ __device__ __forceinline__ void brick_transform(cpx<T> (&v)[E], const Lay& lay, 
       for (int q = 0; q < E; ++q)
         v[q] = cscale(v[q], static_cast<T>(prefac_f32_line<sizeof(T) == 4>(tl + TPL * q, L, pf)) * sc);
     }
-    fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
+    if constexpr (PTW) fft_regs_pt<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
+    else fft_regs<T, N, TPL, true>(v, lay, tl, tw, typename FftSpec<N>::R{});
   }
 }
 
@@ -675,10 +678,12 @@ __device__ __forceinline__ void strided_x_pipe(cpx<T>* __restrict__ spec, int64_
   using G = SGeo<T, N, MODE>;
   static_assert(SWave<T, N, MODE>::ON, "wavefront-per-line x sweeps only");
   constexpr int TL = G::TL, E = N / G::TPL, LSX = SWave<T, N, MODE>::LSX;
+  // per-pass twiddle tables (conflict-free reads) in the twiddle region
+  static_assert(pass_tw_size<N>(typename FftSpec<N>::R{}) <= N, "pass tables fit the twiddle region");
   cpx<T>* tw = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* lds = tw + N;
   const int t = tid % TL, tl = tid / TL;
-  for (int i = tid; i < N; i += G::THREADS) tw[i] = twg[i];
+  pass_tw_fill<T, N>(tw, twg, tid, G::THREADS, typename FftSpec<N>::R{});
   if (first >= nbricks) return;  // uniform in the block
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63;
   cpx<T>* const bt = lds + t * LSX + line_slot(tl);
@@ -705,7 +710,7 @@ __device__ __forceinline__ void strided_x_pipe(cpx<T>* __restrict__ spec, int64_
     for (int q = 0; q < E; ++q) x[q] = wl[68 * q];
     const uint32_t L0 = tile * TL;
     const BrickLine bl = brick_line(lm, L0, (int64_t)L0 + w < nlines ? (uint32_t)w : 0u);
-    brick_transform<T, N, MODE>(x, lay, l, tw, bl, pf);
+    brick_transform<T, N, MODE, true>(x, lay, l, tw, bl, pf);
     xsync<true>();  // the wavefront's last exchange reads are done
 #pragma unroll
     for (int q = 0; q < E; ++q) wl[68 * q] = x[q];

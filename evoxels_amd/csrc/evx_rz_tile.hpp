@@ -315,8 +315,10 @@ __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<fl
     W = ch_weights(g, ChCoef<float>{static_cast<float>(18.0 / e), static_cast<float>(2.0 * e),
                                     static_cast<float>(dd)});
   }
+  // per-pass twiddle tables of the 512-point transform (conflict-free reads)
+  static_assert(pass_tw_size<N>(FftSpec<N>::R{}) <= N, "pass tables fit the twiddle region");
   cpx<float>* tw = reinterpret_cast<cpx<float>*>(lds + RzMap::twiddle(0));
-  for (int i = tid; i < N; i += RzGeo::THREADS) tw[i] = twg[i];
+  pass_tw_fill<float, N>(tw, twg, tid, RzGeo::THREADS, FftSpec<N>::R{});
   const int ra = 2 * w, rb = 2 * w + 1;
   const int hz = RzMap::halo_z(w, l);
   const int64_t y0 = t.y0;
@@ -436,7 +438,7 @@ __device__ __forceinline__ void rz_tile_body(const float* __restrict__ u, cpx<fl
     cpx<float> v[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) v[q] = *xlay.at(l + 64 * q);
-    fft_regs<float, N, 64, false>(v, xlay, l, tw, typename FftSpec<N>::R{});
+    fft_regs_pt<float, N, 64, false>(v, xlay, l, tw, typename FftSpec<N>::R{});
     xsync<true>();
 #pragma unroll
     for (int q = 0; q < 8; ++q) *xlay.at(l + 64 * q) = v[q];
