@@ -51,7 +51,7 @@ def stage_bytes(n, s=4):
         ("ch_rhs_kernel", 2 * s),                    # read c, write rhs
         ("z_r2c_fast", s + spec),                    # read rhs, write spectrum
         ("strided_fast<y,fwd>", 2 * spec),           # read + write spectrum
-        ("strided_fast<x,fwd*prefac*inv>", 2 * spec),
+        ("x_sweep<fwd*prefac*inv>", 2 * spec),
         ("strided_fast<y,inv>", 2 * spec),
         ("z_c2r_fast", spec + 2 * s),                # read spectrum + u, write u
         ("ch_rhs_z_r2c_512", s + spec),              # read c, write spectrum
