@@ -120,6 +120,28 @@ def test_batched_forward_per_channel_params_matches_oracle():
         assert np.linalg.norm(upd - upd_ref) / np.linalg.norm(upd_ref) < 1e-10
 
 
+def test_batched_forward_per_channel_params_512_x_lines():
+    """Three float32 channels of a (512, 5, 6) grid: 60 x lines of the
+    512-point persistent x sweep in four 16-line bricks, two of which hold
+    lines of two channels; each channel against its float64 oracle step."""
+    shape, dt = (512, 5, 6), 0.1
+    vf, vg = grid_for(shape, "float32", (1.0, 1.1, 0.9))
+    rng = np.random.default_rng(12)
+    arr = (0.5 + 0.05 * rng.standard_normal((3,) + shape)).astype(np.float32)
+    prm = np.array([[3.0, 1.0], [2.2, 0.6], [4.1, 1.7]])
+    eng = CHImexEngine(vg, 3, dt)
+    u = torch.tensor(arr, device="cuda")
+    got = eng.step(u, eng.params(prm)).cpu().numpy().astype(np.float64)
+    for c in range(3):
+        a64 = arr[c:c + 1].astype(np.float64)
+        ref = oracle_step_autograd(torch.tensor(a64), shape, vf.spacing, dt,
+                                   torch.tensor(prm[c, 0]), torch.tensor(prm[c, 1])).numpy()
+        upd, upd_ref = got[c] - a64[0], ref[0] - a64[0]
+        err = np.linalg.norm(upd - upd_ref) / np.linalg.norm(upd_ref)
+        print(f"channel {c}: update rel-L2 {err:.3e}", flush=True)
+        assert err < 5e-4
+
+
 @pytest.mark.parametrize("shape,spacing", [((16, 16, 16), (1.0, 1.0, 1.0)), ((12, 10, 18), (0.9, 1.1, 1.0))])
 def test_imex_step_vjp_matches_oracle_autograd(shape, spacing):
     dt = 0.1
