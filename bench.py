@@ -380,15 +380,25 @@ def batched_leg(steps, seq_steps, cpu_steps):
     log(f"[bench] batched: {len(shapes)} grids, {vox} voxels; autotuning")
     sched = b.autotune(state, steps=5)
     log(f"[bench] batched: schedule {sched}")
-    g = b.capture(state, steps)
-    g.replay()  # warm-up replay
-    torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    g.replay()
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
+
+    def graph_ms():
+        g = b.capture(state, steps)
+        g.replay()  # warm-up replay
+        torch.cuda.synchronize()
+        e0.record()
+        g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    ms = graph_ms()
+    # the other launch schedule at the chosen tile shapes, for comparison
+    b.set_tiles(sched["xchunk"], fft_group=sched["fft_group"], one_launch=not sched["one_launch"])
+    ms_other = graph_ms()
+    b.set_tiles(sched["xchunk"], fft_group=sched["fft_group"], one_launch=sched["one_launch"])
+    by_launches = {("one_launch" if sched["one_launch"] else "six_launches"): round(ms, 4),
+                   ("six_launches" if sched["one_launch"] else "one_launch"): round(ms_other, 4)}
     # grid-by-grid single-grid fused steps on the same GPU
     singles = []
     for s, a in zip(shapes, arrs):
@@ -407,7 +417,8 @@ def batched_leg(steps, seq_steps, cpu_steps):
     torch.cuda.synchronize()
     ms_seq = e0.elapsed_time(e1) / seq_steps
     out = {"workload": "ch_batched_48_uneven_fp32", "grids": len(shapes), "voxels": vox, "steps": steps,
-           "schedule": sched, "ms_per_step": round(ms, 4), "voxel_steps_per_s": vox / (ms * 1e-3),
+           "schedule": sched, "ms_per_step": round(ms, 4), "ms_per_step_by_launch_schedule": by_launches,
+           "voxel_steps_per_s": vox / (ms * 1e-3),
            "alg_gbps": round(b_alg / (ms * 1e-3) / 1e9, 1),
            "note": "every grid fits the 256 MB last-level cache, so the GB/s figure is not an HBM fraction",
            "single_grid_loop_ms_per_step": round(ms_seq, 4), "speedup_vs_single_grid_loop": round(ms_seq / ms, 2)}

@@ -104,6 +104,8 @@ SIGNATURES = {
     "evx_batch_get_tiles": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "evx_batch_set_fft_group": [_P, C.c_int],
     "evx_batch_get_fft_group": [_P, C.POINTER(C.c_int)],
+    "evx_batch_set_one_launch": [_P, C.c_int],
+    "evx_batch_get_one_launch": [_P, C.POINTER(C.c_int)],
     "evx_batch_autotune": [_P, _P, _P, C.c_int, _P, C.POINTER(_D)],
     "evx_batch_ch_imex_step": [_P, _P, _P, _P],
     "evx_fft_plan_trace_next_step": [_P, C.POINTER(_P), C.c_int],

@@ -114,25 +114,31 @@ class BatchedCahnHilliard:
     # ------------------------------------------------------------ tiles
     @property
     def schedule(self):
-        x, ni, g = C.c_int(), C.c_int(), C.c_int()
+        x, ni, g, one = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         lib = _lib.load()
         _lib.check(lib.evx_batch_get_tiles(self.handle, C.byref(x), C.byref(ni)), "evx_batch_get_tiles")
         _lib.check(lib.evx_batch_get_fft_group(self.handle, C.byref(g)), "evx_batch_get_fft_group")
-        return {"launches_per_step": 6, "xchunk": x.value, "fft_group": g.value, "items": ni.value}
+        _lib.check(lib.evx_batch_get_one_launch(self.handle, C.byref(one)), "evx_batch_get_one_launch")
+        return {"launches_per_step": 1 if one.value else 6, "one_launch": bool(one.value), "xchunk": x.value,
+                "fft_group": g.value, "items": ni.value}
 
     _generation = 0
 
-    def set_tiles(self, xchunk=8, fft_group=None):
-        """Tile shape: x planes per CH tile, and (if given) the number of
-        consecutive FFT tiles each workgroup of the FFT sweeps covers."""
+    def set_tiles(self, xchunk=8, fft_group=None, one_launch=None):
+        """Schedule: x planes per CH tile; (if given) the number of
+        consecutive FFT tiles each workgroup of the FFT sweeps covers; (if
+        given) one cooperative launch per step instead of one per sweep."""
         torch.cuda.synchronize(self.device)
         now = self.schedule
-        if int(xchunk) != now["xchunk"] or (fft_group is not None and int(fft_group) != now["fft_group"]):
+        if int(xchunk) != now["xchunk"] or (fft_group is not None and int(fft_group) != now["fft_group"]) \
+                or (one_launch is not None and bool(one_launch) != now["one_launch"]):
             self._generation += 1
         lib = _lib.load()
         _lib.check(lib.evx_batch_set_tiles(self.handle, int(xchunk)), "evx_batch_set_tiles")
         if fft_group is not None:
             _lib.check(lib.evx_batch_set_fft_group(self.handle, int(fft_group)), "evx_batch_set_fft_group")
+        if one_launch is not None:
+            _lib.check(lib.evx_batch_set_one_launch(self.handle, int(bool(one_launch))), "evx_batch_set_one_launch")
 
     def autotune(self, state, steps: int = 5):
         """Times the candidate tile shapes on a copy of ``state`` and keeps the

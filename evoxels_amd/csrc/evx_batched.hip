@@ -7,6 +7,9 @@
 // per sweep, each covering the items of every grid (one workgroup per item,
 // whatever the grid's shape), so small grids share the machine instead of
 // each paying its own launches.  The host keeps the item list sweep-major.
+// The same item list also runs as one cooperative launch per step: every
+// block resident, the blocks walking each sweep's items and meeting at the
+// runtime's grid barrier between sweeps (one_launch schedule).
 // Two tile-shape parameters are chosen by the autotuner: the x-chunk of the
 // CH tiles, and the FFT group - how many consecutive z tiles or y/x bricks
 // of one grid an FFT item covers (a workgroup runs them back to back).
@@ -15,6 +18,8 @@
 #include <cmath>
 #include <cstring>
 #include <vector>
+
+#include <hip/hip_cooperative_groups.h>
 
 #include "evx_fft_tiles.hpp"
 #include "evx_stencil_tiles.hpp"
@@ -65,13 +70,14 @@ struct BInst {
 
 struct BArgs {
   const BInst* inst;
-  const int4* items;  // {grid, sweep, tile, 0}
+  const int4* items;  // {grid, sweep, first tile, tile count}
   int n_items;
   int n_inst;
   T* u;
   T* rhs;
   C* spec;
   const C* tw;
+  int first[NSTAGE + 1];  // items of sweep s: first[s] .. first[s+1]-1
 };
 
 // Tiles first .. first+count-1 of one sweep run back to back, a workgroup
@@ -128,56 +134,63 @@ __device__ __forceinline__ void strided_dispatch(const BInst& b, C* spec, const 
 // items of one tile (the count is not read, and the tile loops vanish at
 // compile time); the grouped kernel is a separate instantiation, so its
 // register needs never touch the one-tile schedule.
-template <bool GROUP>
-__device__ __forceinline__ void run_item(const BArgs& a, int4 it, int tid, unsigned char* smem) {
+// Sweep ST of one item: it = {grid, sweep ST, first tile, tile count}.
+// GROUP = false: items of one tile (the count is not read, and the tile
+// loops vanish at compile time); the grouped kernels are separate
+// instantiations, so their register needs never touch the one-tile schedule.
+template <int ST, bool GROUP>
+__device__ __forceinline__ void run_sweep(const BArgs& a, int4 it, int tid, unsigned char* smem) {
   const BInst& b = a.inst[it.x];
   const int tile = it.z, count = GROUP ? it.w : 1;
-  switch (it.y) {
-    case 0: {
-      const Dims d{1, b.nx, b.ny, b.nz};
-      Bc bc{};
-      const Epi ep{EVX_OUT_RHS, 0.0, nullptr, nullptr, nullptr};
-      ch_rhs_tile<T, false>(a.u + b.u_off, a.rhs + b.r_off, d, bc, b.g, b.k, ep, b.tg, nullptr, tile, tid, smem);
-      break;
-    }
-    case 1: {
-      const int64_t nlines = (int64_t)b.nx * b.ny;
-      switch (b.nz) {
+  if constexpr (ST == 0) {
+    const Dims d{1, b.nx, b.ny, b.nz};
+    Bc bc{};
+    const Epi ep{EVX_OUT_RHS, 0.0, nullptr, nullptr, nullptr};
+    ch_rhs_tile<T, false>(a.u + b.u_off, a.rhs + b.r_off, d, bc, b.g, b.k, ep, b.tg, nullptr, tile, tid, smem);
+  } else if constexpr (ST == 1) {
+    const int64_t nlines = (int64_t)b.nx * b.ny;
+    switch (b.nz) {
 #define X(N)                                                                                                   \
   case N:                                                                                                      \
     tile_run(tile, count, [&](int k) { z_r2c_tile<T, N>(a.rhs + b.r_off, a.spec + b.s_off, nlines, b.nh, a.tw + b.tw[2], k, tid, smem); }); \
     break;
-        EVX_BATCH_SIZES(X)
+      EVX_BATCH_SIZES(X)
 #undef X
-        default:
-          break;
-      }
-      break;
+      default:
+        break;
     }
-    case 2:
-      strided_dispatch<0>(b, a.spec, a.tw, 1, tile, count, tid, smem);
-      break;
-    case 3:
-      strided_dispatch<2>(b, a.spec, a.tw, 0, tile, count, tid, smem);
-      break;
-    case 4:
-      strided_dispatch<1>(b, a.spec, a.tw, 1, tile, count, tid, smem);
-      break;
-    default: {
-      const int64_t nlines = (int64_t)b.nx * b.ny;
-      T* u = a.u + b.u_off;
-      switch (b.nz) {
+  } else if constexpr (ST == 2) {
+    strided_dispatch<0>(b, a.spec, a.tw, 1, tile, count, tid, smem);
+  } else if constexpr (ST == 3) {
+    strided_dispatch<2>(b, a.spec, a.tw, 0, tile, count, tid, smem);
+  } else if constexpr (ST == 4) {
+    strided_dispatch<1>(b, a.spec, a.tw, 1, tile, count, tid, smem);
+  } else {
+    const int64_t nlines = (int64_t)b.nx * b.ny;
+    T* u = a.u + b.u_off;
+    switch (b.nz) {
 #define X(N)                                                                                              \
   case N:                                                                                                 \
     tile_run(tile, count, [&](int k) { z_c2r_tile<T, N>(a.spec + b.s_off, u, u, T(1), nlines, b.nh, a.tw + b.tw[2], k, tid, smem); }); \
     break;
-        EVX_BATCH_SIZES(X)
+      EVX_BATCH_SIZES(X)
 #undef X
-        default:
-          break;
-      }
-      break;
+      default:
+        break;
     }
+  }
+}
+
+// One item of any sweep.
+template <bool GROUP>
+__device__ __forceinline__ void run_item(const BArgs& a, int4 it, int tid, unsigned char* smem) {
+  switch (it.y) {
+    case 0: run_sweep<0, GROUP>(a, it, tid, smem); break;
+    case 1: run_sweep<1, GROUP>(a, it, tid, smem); break;
+    case 2: run_sweep<2, GROUP>(a, it, tid, smem); break;
+    case 3: run_sweep<3, GROUP>(a, it, tid, smem); break;
+    case 4: run_sweep<4, GROUP>(a, it, tid, smem); break;
+    default: run_sweep<5, GROUP>(a, it, tid, smem); break;
   }
 }
 
@@ -185,6 +198,43 @@ template <bool GROUP>
 __global__ __launch_bounds__(BT) void batch_stage(BArgs a, int first) {
   __shared__ __align__(16) unsigned char smem[BATCH_LDS];
   run_item<GROUP>(a, a.items[first + (int)blockIdx.x], threadIdx.x, smem);
+}
+
+// The whole step in one cooperative launch (all blocks resident): block b
+// takes items b, b + grid, ... of each sweep, and every block passes the
+// grid barrier between two sweeps the same number of times (no early exit),
+// so the barrier always completes.
+// Items b, b + grid, ... of sweep ST for block b of the cooperative step.
+template <int ST, bool GROUP>
+__device__ __forceinline__ void coop_sweep(const BArgs& a, unsigned char* smem) {
+  for (int i = a.first[ST] + (int)blockIdx.x; i < a.first[ST + 1]; i += (int)gridDim.x) {
+    // the thread index through an opaque copy: the tile bodies' lane offsets
+    // of every size are then recomputed per item instead of being hoisted
+    // out of the loop and held in registers all at once
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    run_sweep<ST, GROUP>(a, a.items[i], tid, smem);
+    __syncthreads();  // the next item's tile body rewrites the LDS
+  }
+}
+
+template <bool GROUP>
+__global__ __launch_bounds__(BT) void batch_step_coop(BArgs a) {
+  __shared__ __align__(16) unsigned char smem[BATCH_LDS];
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  // one loop per sweep (only that sweep's code live in it); every block
+  // reaches each of the five grid barriers exactly once
+  coop_sweep<0, GROUP>(a, smem);
+  grid.sync();  // the sweep's outputs are visible to every block
+  coop_sweep<1, GROUP>(a, smem);
+  grid.sync();
+  coop_sweep<2, GROUP>(a, smem);
+  grid.sync();
+  coop_sweep<3, GROUP>(a, smem);
+  grid.sync();
+  coop_sweep<4, GROUP>(a, smem);
+  grid.sync();
+  coop_sweep<5, GROUP>(a, smem);
 }
 
 size_t align_up(size_t v) { return (v + 255) & ~size_t(255); }
@@ -242,6 +292,7 @@ struct evx_batch_plan {
   // schedule
   int xchunk = 8;   // x planes per CH tile
   int fft_group = 1;  // consecutive tiles per FFT item
+  bool one_launch = false;  // the step as one cooperative launch
   int n_items = 0;
   int items_cap = 0;  // capacity of d_items: the item count of the finest tiling (one x plane per CH tile)
   int stage_first[evx::NSTAGE + 1] = {0};
@@ -335,11 +386,40 @@ BArgs make_args(evx_batch_plan* p, void* state, void* ws) {
   a.rhs = static_cast<T*>(ws);
   a.spec = reinterpret_cast<C*>(static_cast<char*>(ws) + align_up(sizeof(T) * p->state_elems));
   a.tw = p->d_tw;
+  for (int st = 0; st <= NSTAGE; ++st) a.first[st] = p->stage_first[st];
   return a;
+}
+
+// Blocks of the cooperative step: as many as the device holds at once.
+template <bool GROUP>
+int coop_resident() {
+  static const int resident = [] {
+    int dev = 0, cus = 0, per = 0, coop = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_step_coop<GROUP>, BT, 0) != hipSuccess)
+      return 0;
+    return per * cus;
+  }();
+  return resident;
+}
+
+template <bool GROUP>
+int launch_coop(const BArgs& a, hipStream_t s) {
+  const int res = coop_resident<GROUP>();
+  if (res < 1) return fail(EVX_ERR_UNSUPPORTED, "cooperative launches are not available on this device");
+  BArgs arg = a;
+  void* params[] = {&arg};
+  if (hipLaunchCooperativeKernel(reinterpret_cast<const void*>(batch_step_coop<GROUP>), dim3((unsigned)res),
+                                 dim3(BT), params, 0, s) != hipSuccess)
+    return fail(EVX_ERR_HIP, "cooperative launch of the batched step failed: %s", hipGetErrorString(hipGetLastError()));
+  return EVX_OK;
 }
 
 int launch_step(evx_batch_plan* p, void* state, void* ws, hipStream_t s) {
   const BArgs a = make_args(p, state, ws);
+  if (p->one_launch) return p->fft_group > 1 ? launch_coop<true>(a, s) : launch_coop<false>(a, s);
   for (int st = 0; st < NSTAGE; ++st) {
     const int n = p->stage_first[st + 1] - p->stage_first[st];
     if (n == 0) continue;
@@ -511,6 +591,20 @@ extern "C" int evx_batch_get_fft_group(const evx_batch_plan* p, int* tiles_per_i
   return EVX_OK;
 }
 
+extern "C" int evx_batch_set_one_launch(evx_batch_plan* p, int one_launch) {
+  if (!p) return fail(EVX_ERR_INVALID, "null plan");
+  if (one_launch && (p->fft_group > 1 ? coop_resident<true>() : coop_resident<false>()) < 1)
+    return fail(EVX_ERR_UNSUPPORTED, "cooperative launches are not available on this device");
+  p->one_launch = one_launch != 0;
+  return EVX_OK;
+}
+
+extern "C" int evx_batch_get_one_launch(const evx_batch_plan* p, int* one_launch) {
+  if (!p || !one_launch) return fail(EVX_ERR_INVALID, "null argument");
+  *one_launch = p->one_launch ? 1 : 0;
+  return EVX_OK;
+}
+
 extern "C" int evx_batch_get_tiles(const evx_batch_plan* p, int* xchunk, int* n_items) {
   if (!p || !xchunk || !n_items) return fail(EVX_ERR_INVALID, "null argument");
   *xchunk = p->xchunk;
@@ -540,11 +634,12 @@ static int time_schedule(evx_batch_plan* p, void* state, void* ws, int steps, hi
   return EVX_OK;
 }
 
-// Autotuner: times the candidate tile shapes over `steps` steps each and
+// Autotuner: times the candidate schedules over `steps` steps each and
 // keeps the fastest: first the x-chunk of the CH tiles (x planes per tile,
 // which sets the number of CH items and the halo overhead of each) at the
 // current FFT group, then the FFT group (tiles per FFT item) at that
-// x-chunk.  The candidates run on the caller's scratch state, which is
+// x-chunk, then one cooperative launch per step against one launch per
+// sweep.  The candidates run on the caller's scratch state, which is
 // overwritten.  Synchronises the stream (set-up time only).
 extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* workspace, int steps, void* stream,
                                   double* best_ms) {
@@ -585,12 +680,27 @@ extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* 
       gpick = g;
     }
   }
+  // then one cooperative launch per step against one launch per sweep
+  bool opick = p->one_launch;
+  if (st == EVX_OK) st = evx_batch_set_fft_group(p, gpick);
+  const bool coop = (gpick > 1 ? coop_resident<true>() : coop_resident<false>()) > 0;
+  for (const bool one : {false, true}) {
+    if (st != EVX_OK || (one && !coop)) break;
+    double ms = 0.0;
+    p->one_launch = one;
+    if ((st = time_schedule(p, scratch_state, workspace, steps, s, e0, e1, &ms)) != EVX_OK) break;
+    if (ms < best) {
+      best = ms;
+      opick = one;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (st != EVX_OK) return st;
   if (hipStreamSynchronize(s) != hipSuccess) return fail(EVX_ERR_HIP, "stream synchronisation failed");
   EVX_TRY(evx_batch_set_tiles(p, xpick));
   EVX_TRY(evx_batch_set_fft_group(p, gpick));
+  p->one_launch = opick;
   if (best_ms) *best_ms = best;
   return EVX_OK;
 }

@@ -329,10 +329,18 @@ int evx_batch_get_tiles(const evx_batch_plan* plan, int* xchunk, int* n_items);
  * results do not depend on it.  Set-up call, as evx_batch_set_tiles. */
 int evx_batch_set_fft_group(evx_batch_plan* plan, int tiles_per_item);
 int evx_batch_get_fft_group(const evx_batch_plan* plan, int* tiles_per_item);
-/* Times the candidate tile shapes on scratch_state (overwritten) and keeps
+/* The step as one cooperative launch (1) instead of one launch per sweep (0,
+ * default): every block resident, the blocks walking each sweep's items and
+ * meeting at the runtime's grid barrier between sweeps.  Same results as the
+ * six launches.  EVX_ERR_UNSUPPORTED where the device has no cooperative
+ * launches.  Set-up call, as evx_batch_set_tiles. */
+int evx_batch_set_one_launch(evx_batch_plan* plan, int one_launch);
+int evx_batch_get_one_launch(const evx_batch_plan* plan, int* one_launch);
+/* Times the candidate schedules on scratch_state (overwritten) and keeps
  * the fastest: the Cahn-Hilliard x-chunk (4, 8, 16, 32, 64), then the FFT
- * group (1, 2, 4, 8) at that x-chunk; best_ms (may be NULL) receives the
- * chosen schedule's time per step.  Synchronises the stream. */
+ * group (1, 2, 4, 8) at that x-chunk, then one cooperative launch per step
+ * against one launch per sweep; best_ms (may be NULL) receives the chosen
+ * schedule's time per step.  Synchronises the stream. */
 int evx_batch_autotune(evx_batch_plan* plan, void* scratch_state,
                        void* workspace, int steps, void* stream,
                        double* best_ms);

@@ -99,6 +99,49 @@ def test_batched_fft_group_is_bitwise_neutral(reference):
     compare(outs[3], want, "fft_group 3")
 
 
+@pytest.mark.parametrize("fft_group", [1, 3])
+def test_batched_one_launch_equals_six_launches(reference, fft_group):
+    """The step as one cooperative launch (blocks walking each sweep's items,
+    grid barriers between sweeps) gives the same bits as one launch per
+    sweep: the same tile bodies on the same items in the same sweep order."""
+    arrs, eps, D, want = reference
+    b = BatchedCahnHilliard(SHAPES, DT, eps=eps, D=D)
+    outs = {}
+    for one in (False, True):
+        b.set_tiles(8, fft_group=fft_group, one_launch=one)
+        assert b.schedule["one_launch"] is one and b.schedule["launches_per_step"] == (1 if one else 6)
+        state = b.pack(arrs)
+        t0 = time.perf_counter()
+        b.step(state, 3)
+        torch.cuda.synchronize()
+        print(f"one_launch {one} fft_group {fft_group}: 3 steps in {time.perf_counter() - t0:.3f} s", flush=True)
+        outs[one] = [v.clone() for v in b.views(state)]
+    assert all(torch.equal(x, y) for x, y in zip(outs[False], outs[True]))
+    compare(outs[True], want, f"one launch, fft_group {fft_group}")
+
+
+def test_batched_one_launch_graph_replay(reference):
+    """A captured graph of one-launch steps, replayed three times with eager
+    work in between, each replay checked."""
+    arrs, eps, D, want = reference
+    b = BatchedCahnHilliard(SHAPES, DT, eps=eps, D=D)
+    b.set_tiles(8, one_launch=True)
+    state = b.pack(arrs)
+    g = b.capture(state, 1)
+    torch.cuda.synchronize()
+    assert all(torch.equal(v, torch.tensor(a, dtype=torch.float32, device="cuda"))
+               for v, a in zip(b.views(state), arrs))
+    ref = BatchedCahnHilliard(SHAPES, DT, eps=eps, D=D)
+    rstate = ref.pack(arrs)
+    for r in range(3):
+        g.replay()
+        ref.step(rstate, 1)
+        torch.cuda.synchronize()
+        _ = (state * 2).sum().item()
+        assert all(torch.equal(x, y) for x, y in zip(b.views(state), ref.views(rstate))), f"replay {r}"
+    compare(b.views(state), want, "one-launch graph")
+
+
 def test_batched_graph_replay(reference):
     arrs, eps, D, want = reference
     b = BatchedCahnHilliard(SHAPES, DT, eps=eps, D=D)
@@ -125,6 +168,7 @@ def test_batched_autotune(reference):
     sched = b.autotune(state, steps=1)
     print(f"autotune {time.perf_counter() - t0:.3f} s -> {sched}", flush=True)
     assert sched["xchunk"] in (4, 8, 16, 32, 64) and sched["fft_group"] in (1, 2, 4, 8)
+    assert sched["launches_per_step"] == (1 if sched["one_launch"] else 6)
     assert sched["ms_per_step"] > 0
     # the autotuner ran on a copy: state is untouched
     assert all(torch.equal(v, torch.tensor(a, dtype=torch.float32, device="cuda"))
