@@ -204,7 +204,6 @@ __global__ __launch_bounds__(BT) void batch_stage(BArgs a, int first) {
 // takes items b, b + grid, ... of each sweep, and every block passes the
 // grid barrier between two sweeps the same number of times (no early exit),
 // so the barrier always completes.
-// Items b, b + grid, ... of sweep ST for block b of the cooperative step.
 template <int ST, bool GROUP>
 __device__ __forceinline__ void coop_sweep(const BArgs& a, unsigned char* smem) {
   for (int i = a.first[ST] + (int)blockIdx.x; i < a.first[ST + 1]; i += (int)gridDim.x) {
