@@ -658,6 +658,8 @@ extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* 
   }
   double best = 1e300;
   int xpick = p->xchunk, gpick = p->fft_group;
+  const bool one_before = p->one_launch;
+  p->one_launch = false;  // the tile shapes are timed with one launch per sweep
   int st = EVX_OK;
   for (const int c : xcands) {
     double ms = 0.0;
@@ -680,7 +682,7 @@ extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* 
     }
   }
   // then one cooperative launch per step against one launch per sweep
-  bool opick = p->one_launch;
+  bool opick = false;
   if (st == EVX_OK) st = evx_batch_set_fft_group(p, gpick);
   const bool coop = (gpick > 1 ? coop_resident<true>() : coop_resident<false>()) > 0;
   for (const bool one : {false, true}) {
@@ -695,7 +697,10 @@ extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* 
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (st != EVX_OK) return st;
+  if (st != EVX_OK) {
+    p->one_launch = one_before;
+    return st;
+  }
   if (hipStreamSynchronize(s) != hipSuccess) return fail(EVX_ERR_HIP, "stream synchronisation failed");
   EVX_TRY(evx_batch_set_tiles(p, xpick));
   EVX_TRY(evx_batch_set_fft_group(p, gpick));

@@ -312,7 +312,7 @@ __global__ __launch_bounds__((SGeo<T, N, MODE>::THREADS)) void strided_fast(cpx<
   strided_tile<T, N, MODE>(spec, nlines, lm, twg, pf, xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, smem);
 }
 
-// The persistent x sweep: one resident round of blocks (98 VGPRs, one
+// The persistent x sweep: one resident round of blocks (86 VGPRs, one
 // 1024-thread block per CU) walking the bricks.
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__((SGeo<T, N, MODE>::THREADS), 4) void strided_x_pipe_kernel(

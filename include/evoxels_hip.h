@@ -338,9 +338,10 @@ int evx_batch_set_one_launch(evx_batch_plan* plan, int one_launch);
 int evx_batch_get_one_launch(const evx_batch_plan* plan, int* one_launch);
 /* Times the candidate schedules on scratch_state (overwritten) and keeps
  * the fastest: the Cahn-Hilliard x-chunk (4, 8, 16, 32, 64), then the FFT
- * group (1, 2, 4, 8) at that x-chunk, then one cooperative launch per step
- * against one launch per sweep; best_ms (may be NULL) receives the chosen
- * schedule's time per step.  Synchronises the stream. */
+ * group (1, 2, 4, 8) at that x-chunk, both with one launch per sweep, then
+ * one cooperative launch per step against one launch per sweep; best_ms
+ * (may be NULL) receives the chosen schedule's time per step.  On an error
+ * the launch schedule is left as it was.  Synchronises the stream. */
 int evx_batch_autotune(evx_batch_plan* plan, void* scratch_state,
                        void* workspace, int steps, void* stream,
                        double* best_ms);

@@ -178,6 +178,21 @@ def test_batched_autotune(reference):
     compare(b.views(state), want, "autotuned")
 
 
+def test_batched_autotune_starting_from_one_launch(reference):
+    """The tile shapes are timed with one launch per sweep whatever the
+    schedule before the call; the last pass picks the launch schedule."""
+    arrs, eps, D, want = reference
+    b = BatchedCahnHilliard(SHAPES, DT, eps=eps, D=D)
+    b.set_tiles(8, fft_group=1, one_launch=True)
+    state = b.pack(arrs)
+    sched = b.autotune(state, steps=1)
+    print(f"autotune from one launch -> {sched}", flush=True)
+    assert sched["launches_per_step"] == (1 if sched["one_launch"] else 6)
+    b.step(state, 3)
+    torch.cuda.synchronize()
+    compare(b.views(state), want, "autotuned from one launch")
+
+
 def test_batched_grid_against_float64_oracle():
     shapes = [(32, 48, 16), (16, 16, 64)]
     arrs = initial(shapes, seed=8)
