@@ -54,6 +54,27 @@ def test_version_and_error_paths_without_gpu():
     assert "prime factor 37" in _lib.last_error()
 
 
+def test_batch_argument_checks_without_gpu():
+    """The batched entry points reject bad arguments before touching the
+    device: null plans and outputs, float64, grid counts out of range."""
+    lib = _lib.load()
+    p = C.c_void_p()
+    shapes = (C.c_int64 * 3)(32, 32, 32)
+    spacing = (C.c_double * 3)(1.0, 1.0, 1.0)
+    assert lib.evx_batch_plan_create(None, 0, 1, shapes, spacing) == _lib.EVX_ERR_INVALID
+    assert lib.evx_batch_plan_create(C.byref(p), 1, 1, shapes, spacing) == _lib.EVX_ERR_UNSUPPORTED
+    assert "float32" in _lib.last_error()
+    assert lib.evx_batch_plan_create(C.byref(p), 0, 0, shapes, spacing) == _lib.EVX_ERR_INVALID
+    assert "n_inst" in _lib.last_error()
+    i = C.c_int()
+    ms = C.c_double()
+    for st in (lib.evx_batch_set_tiles(None, 8), lib.evx_batch_set_fft_group(None, 1),
+               lib.evx_batch_get_fft_group(None, C.byref(i)), lib.evx_batch_set_one_launch(None, 1),
+               lib.evx_batch_get_one_launch(None, C.byref(i)),
+               lib.evx_batch_autotune(None, None, None, 1, None, C.byref(ms))):
+        assert st == _lib.EVX_ERR_INVALID
+
+
 def test_check_maps_status_to_reference_exceptions():
     with pytest.raises(ValueError):
         _lib.check(_lib.EVX_ERR_INVALID)
