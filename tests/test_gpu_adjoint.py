@@ -364,8 +364,8 @@ def test_config5_two_trajectories_128_adjoint_matches_oracle_autograd():
         want = np.array([float(ge), float(gd)])
         err = np.abs(grad[c] - want).max() / np.abs(want).max()
         print(f"trajectory {c}: adjoint {grad[c]}, oracle autograd {want}, rel {err:.3e}; "
-              f"loss {float(loss[c]):.6e} vs {float(J):.6e}", flush=True)
-        assert abs(float(loss[c]) - float(J)) <= 1e-10 * float(J)
+              f"loss {float(loss[c]):.6e} vs {float(J.detach()):.6e}", flush=True)
+        assert abs(float(loss[c]) - float(J.detach())) <= 1e-10 * float(J.detach())
         assert err <= 1e-8
 
 
