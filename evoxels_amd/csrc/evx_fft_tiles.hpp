@@ -263,7 +263,10 @@ template <> struct FftSpec<32>   { using R = Rad<4, 8>;       static constexpr i
 template <> struct FftSpec<48>   { using R = Rad<3, 4, 4>;    static constexpr int L = 12; };
 template <> struct FftSpec<64>   { using R = Rad<8, 8>;       static constexpr int L = 8; };
 template <> struct FftSpec<96>   { using R = Rad<3, 4, 4, 2>; static constexpr int L = 12; };
-template <> struct FftSpec<100>  { using R = Rad<4, 5, 5>;    static constexpr int L = 20; };
+// 100 = 2*5*2*5: ten values per thread (Rad<4, 5, 5> needs twenty), so a
+// 100^3 grid's sweeps run twice the threads and blocks; the 100^3 float64
+// CH step took 0.0581 instead of 0.0709 ms (MI355X).
+template <> struct FftSpec<100>  { using R = Rad<2, 5, 2, 5>; static constexpr int L = 10; };
 template <> struct FftSpec<128>  { using R = Rad<2, 8, 8>;    static constexpr int L = 8; };
 template <> struct FftSpec<256>  { using R = Rad<4, 8, 8>;    static constexpr int L = 8; };
 template <> struct FftSpec<512>  { using R = Rad<8, 8, 8>;    static constexpr int L = 8; };
