@@ -1051,24 +1051,28 @@ int rd_spectral_step(evx_fft_plan* p, const void* u, void* u_out, void* ws, cons
   return EVX_OK;
 }
 
-// Mirror extension of the right-hand side along x (boundary_conditions.py:
-// 61-71,133-144): planes [0, n) hold the field; the kernel writes the
-// extension planes after them.  mode 0: [f, flip f] (zero flux), 1:
-// [f, -flip f] (Dirichlet), 2: [f, 0, -flip f, 0] (staggered_x Dirichlet).
+// Mirror extension along x (boundary_conditions.py:61-71,133-144), applied
+// to the z/y half spectrum of the right-hand side: planes [0, n) hold the
+// transformed field; the kernel writes the extension planes after them.
+// mode 0: [f, flip f] (zero flux), 1: [f, -flip f] (Dirichlet), 2:
+// [f, 0, -flip f, 0] (staggered_x Dirichlet).  The z and y transforms act
+// plane by plane and commute exactly with a sign flip, so this equals
+// transforming the mirror-extended field.  Block (i, j) copies a chunk of
+// extension plane j; `plane` counts complex elements.
 template <typename T>
-__global__ void mirror_x_kernel(T* __restrict__ buf, int64_t n, int64_t plane, int mode) {
-  const int64_t planes = mode == 2 ? n + 2 : n;
-  const int64_t total = planes * plane;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t j = i / plane, r = i - j * plane;
-    T v;
-    if (mode < 2) {
-      const T f = buf[(n - 1 - j) * plane + r];
-      v = mode == 0 ? f : -f;
-    } else {
-      v = (j == 0 || j == n + 1) ? T(0) : -buf[(n - j) * plane + r];
+__global__ void mirror_spec_x_kernel(cpx<T>* __restrict__ spec, int64_t n, int64_t plane, int mode) {
+  const int64_t j = blockIdx.y;
+  cpx<T>* const dst = spec + (n + j) * plane;
+  const bool zero = mode == 2 && (j == 0 || j == n + 1);
+  const cpx<T>* const src = spec + (zero ? 0 : mode == 2 ? n - j : n - 1 - j) * plane;
+  const T sg = mode == 0 ? T(1) : T(-1);
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < plane; r += (int64_t)gridDim.x * blockDim.x) {
+    cpx<T> v = {T(0), T(0)};
+    if (!zero) {
+      const cpx<T> f = src[r];
+      v = {sg * f.x, sg * f.y};
     }
-    buf[(n + j) * plane + r] = v;
+    dst[r] = v;
   }
 }
 
@@ -1076,8 +1080,10 @@ __global__ void mirror_x_kernel(T* __restrict__ buf, int64_t n, int64_t plane, i
 // (timesteppers.py:85-89,195-199 with pad_fft_bc): the plan covers the
 // mirror-extended x length (2n, or 2n+2 on staggered_x grids); the
 // right-hand side of the n physical planes is evaluated with the ghost rules
-// of `bc`, extended in the staging buffer, transformed over the extended
-// grid, and only the n physical planes are transformed back and added to u.
+// of `bc` and transformed along z and y on those planes only, the half
+// spectrum is mirror-extended along x, the x sweep runs over the extended
+// length, and only the n physical planes are transformed back and added
+// to u.
 int mirror_spectral_step(evx_fft_plan* p, int ch, const void* u, void* u_out, void* ws, const evx_imex_params* prm,
                          const evx_bc* bc, int staggered, hipStream_t s, int etd) {
   EVX_TRY(imex_common(p, u, u_out, ws, prm));
@@ -1106,17 +1112,12 @@ int mirror_spectral_step(evx_fft_plan* p, int ch, const void* u, void* u_out, vo
   EVX_TRY(trace_start(p, s));
   EVX_TRY(ch ? evx_ch_rhs(p->dtype, u, rhs, &g, bc, prm->eps, prm->D, &e, s)
              : evx_rd_rhs(p->dtype, u, rhs, &g, bc, prm->D, &e, s));
-  const int64_t plane = p->d.ny * p->d.nz;
-  const int mode = staggered ? 2 : (dir ? 1 : 0);
-  const int64_t total = (staggered ? n + 2 : n) * plane;
-  const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 65535LL * 16);
-  if (p->dtype == EVX_F32)
-    hipLaunchKernelGGL(mirror_x_kernel<float>, dim3(blocks), dim3(256), 0, s, static_cast<float*>(rhs), n, plane, mode);
-  else
-    hipLaunchKernelGGL(mirror_x_kernel<double>, dim3(blocks), dim3(256), 0, s, static_cast<double*>(rhs), n, plane,
-                       mode);
-  EVX_TRY(check_launch("mirror_x_kernel"));
   EVX_TRY(trace_tick(p, TK_RHS, s));
+  const int mode = staggered ? 2 : (dir ? 1 : 0);
+  const int64_t ext = staggered ? n + 2 : n;  // extension planes
+  const int64_t cplane = p->d.ny * p->pitch;   // complex elements per spectrum plane
+  if (ext > 65535) return fail(EVX_ERR_UNSUPPORTED, "the mirrored spectral step takes at most 65533 x planes");
+  const dim3 mgrid((unsigned)std::min<int64_t>((cplane + 255) / 256, 1024), (unsigned)ext);
   // problem_definition.py:206,303 -> (-D)*A or ((-2*eps)*D)*A as Python doubles
   Prefac pf = make_prefac(p, ch, ch ? ((-2.0 * prm->eps) * prm->D) * prm->A : (-prm->D) * prm->A, prm->dt,
                           prm->spacing);
@@ -1125,20 +1126,28 @@ int mirror_spectral_step(evx_fft_plan* p, int ch, const void* u, void* u_out, vo
   if (etd) pf.exact = 0;
   const int64_t pt = p->pitch;
   if (p->dtype == EVX_F32) {
-    EVX_TRY(run_z_r2c<float>(p, rhs, spec, pt, s));
+    EVX_TRY(run_z_r2c<float>(p, rhs, spec, pt, s, 0, n));
     EVX_TRY(trace_tick(p, TK_Z_R2C, s));
-    EVX_TRY((run_strided<float, 0>(p, 1, spec, pf, pt, s)));
+    EVX_TRY((run_strided<float, 0>(p, 1, spec, pf, pt, s, 0, n)));
     EVX_TRY(trace_tick(p, TK_Y_FWD, s));
+    hipLaunchKernelGGL(mirror_spec_x_kernel<float>, mgrid, dim3(256), 0, s, static_cast<cpx<float>*>(spec), n, cplane,
+                       mode);
+    EVX_TRY(check_launch("mirror_spec_x_kernel"));
+    EVX_TRY(trace_tick(p, TK_OTHER, s));
     EVX_TRY(run_x_multiplier<float>(p, spec, pf, pt, s));
     EVX_TRY(trace_tick(p, TK_X, s));
     EVX_TRY((run_strided<float, 1>(p, 1, spec, pf, pt, s, 0, n)));
     EVX_TRY(trace_tick(p, TK_Y_INV, s));
     EVX_TRY(run_z_c2r<float>(p, spec, u_out, u, 1.0, pt, s, 0, n));
   } else {
-    EVX_TRY(run_z_r2c<double>(p, rhs, spec, pt, s));
+    EVX_TRY(run_z_r2c<double>(p, rhs, spec, pt, s, 0, n));
     EVX_TRY(trace_tick(p, TK_Z_R2C, s));
-    EVX_TRY((run_strided<double, 0>(p, 1, spec, pf, pt, s)));
+    EVX_TRY((run_strided<double, 0>(p, 1, spec, pf, pt, s, 0, n)));
     EVX_TRY(trace_tick(p, TK_Y_FWD, s));
+    hipLaunchKernelGGL(mirror_spec_x_kernel<double>, mgrid, dim3(256), 0, s, static_cast<cpx<double>*>(spec), n,
+                       cplane, mode);
+    EVX_TRY(check_launch("mirror_spec_x_kernel"));
+    EVX_TRY(trace_tick(p, TK_OTHER, s));
     EVX_TRY(run_x_multiplier<double>(p, spec, pf, pt, s));
     EVX_TRY(trace_tick(p, TK_X, s));
     EVX_TRY((run_strided<double, 1>(p, 1, spec, pf, pt, s, 0, n)));
