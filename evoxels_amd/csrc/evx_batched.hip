@@ -651,7 +651,7 @@ extern "C" int evx_batch_autotune(evx_batch_plan* p, void* scratch_state, void* 
   hipStream_t s = static_cast<hipStream_t>(stream);
   // the item list is rewritten between candidates: nothing may be in flight
   if (hipStreamSynchronize(s) != hipSuccess) return fail(EVX_ERR_HIP, "stream synchronisation failed");
-  const int xcands[] = {4, 8, 16, 32, 64};
+  const int xcands[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
   const int gcands[] = {1, 2, 4, 8};
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return fail(EVX_ERR_HIP, "hipEventCreate failed");

@@ -337,7 +337,7 @@ int evx_batch_get_fft_group(const evx_batch_plan* plan, int* tiles_per_item);
 int evx_batch_set_one_launch(evx_batch_plan* plan, int one_launch);
 int evx_batch_get_one_launch(const evx_batch_plan* plan, int* one_launch);
 /* Times the candidate schedules on scratch_state (overwritten) and keeps
- * the fastest: the Cahn-Hilliard x-chunk (4, 8, 16, 32, 64), then the FFT
+ * the fastest: the Cahn-Hilliard x-chunk (4, 6, 8, 12, 16, 24, 32, 48, 64), then the FFT
  * group (1, 2, 4, 8) at that x-chunk, both with one launch per sweep, then
  * one cooperative launch per step against one launch per sweep; best_ms
  * (may be NULL) receives the chosen schedule's time per step.  On an error

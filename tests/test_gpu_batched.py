@@ -167,7 +167,7 @@ def test_batched_autotune(reference):
     t0 = time.perf_counter()
     sched = b.autotune(state, steps=1)
     print(f"autotune {time.perf_counter() - t0:.3f} s -> {sched}", flush=True)
-    assert sched["xchunk"] in (4, 8, 16, 32, 64) and sched["fft_group"] in (1, 2, 4, 8)
+    assert sched["xchunk"] in (4, 6, 8, 12, 16, 24, 32, 48, 64) and sched["fft_group"] in (1, 2, 4, 8)
     assert sched["launches_per_step"] == (1 if sched["one_launch"] else 6)
     assert sched["ms_per_step"] > 0
     # the autotuner ran on a copy: state is untouched
