@@ -267,7 +267,11 @@ template <> struct FftSpec<96>   { using R = Rad<3, 4, 4, 2>; static constexpr i
 // 100^3 grid's sweeps run twice the threads and blocks; the 100^3 float64
 // CH step took 0.0581 instead of 0.0709 ms (MI355X).
 template <> struct FftSpec<100>  { using R = Rad<2, 5, 2, 5>; static constexpr int L = 10; };
-template <> struct FftSpec<128>  { using R = Rad<2, 8, 8>;    static constexpr int L = 8; };
+// 128 = 2*4*4*4: four values per thread instead of eight (Rad<2, 8, 8>):
+// the 16 x 128^3 inversion gradient 0.3256 -> 0.3187 s, the 48-grid batched
+// step 0.2655 -> 0.2649 ms (MI355X).  The same change for 32 and 64
+// (Rad<4, 4, 2>, Rad<4, 4, 4>) made the batched step slower (0.2749 ms).
+template <> struct FftSpec<128>  { using R = Rad<2, 4, 4, 4>; static constexpr int L = 4; };
 template <> struct FftSpec<256>  { using R = Rad<4, 8, 8>;    static constexpr int L = 8; };
 template <> struct FftSpec<512>  { using R = Rad<8, 8, 8>;    static constexpr int L = 8; };
 template <> struct FftSpec<1024> { using R = Rad<2, 8, 8, 8>; static constexpr int L = 8; };
