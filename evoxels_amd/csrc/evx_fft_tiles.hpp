@@ -277,6 +277,17 @@ template <> struct FftSpec<512>  { using R = Rad<8, 8, 8>;    static constexpr i
 template <> struct FftSpec<1024> { using R = Rad<2, 8, 8, 8>; static constexpr int L = 8; };
 #define EVX_FAST_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(100) X(128) X(256) X(512) X(1024)
 
+// Every plan multiplies out to its size, and L is a multiple of each radix.
+template <int... R>
+constexpr int rad_product(Rad<R...>) { return (R * ... * 1); }
+template <int L, int... R>
+constexpr bool rad_divides(Rad<R...>) { return ((L % R == 0) && ...); }
+#define X(N)                                                                                     \
+  static_assert(rad_product(typename FftSpec<N>::R{}) == N && rad_divides<FftSpec<N>::L>(typename FftSpec<N>::R{}), \
+                "radix plan of size " #N);
+EVX_FAST_SIZES(X)
+#undef X
+
 // z sweeps: one line pair per TPL threads, TPL = N/L (each thread owns L values).
 template <typename T, int N>
 struct ZGeo {
