@@ -314,27 +314,41 @@ __global__ __launch_bounds__((SGeo<T, N, MODE>::THREADS)) void strided_fast(cpx<
 
 // The persistent x sweep: one resident round of blocks (86 VGPRs, one
 // 1024-thread block per CU) walking the bricks.
-template <typename T, int N, int MODE>
+// MIR = 1: mirrored x sweep (a separate instantiation, so that the
+// unmirrored sweep carries no mirror code).
+template <typename T, int N, int MODE, int MIR = 0>
 __global__ __launch_bounds__((SGeo<T, N, MODE>::THREADS), 4) void strided_x_pipe_kernel(
     cpx<T>* __restrict__ spec, int64_t nlines, LineMap lm, const cpx<T>* __restrict__ twg, Prefac pf,
     uint32_t nbricks) {
   __shared__ __align__(16) unsigned char smem[STile<T, N, MODE>::LDS_BYTES];
-  strided_x_pipe<T, N, MODE>(spec, nlines, lm, twg, pf, nbricks, blockIdx.x, gridDim.x, threadIdx.x, smem);
+  strided_x_pipe<T, N, MODE, MIR>(spec, nlines, lm, twg, pf, nbricks, blockIdx.x, gridDim.x, threadIdx.x, smem);
 }
 
-template <typename T, int N, int MODE>
+template <typename T, int N, int MODE, int MIR>
 int64_t x_pipe_resident() {
   static const int64_t resident = [] {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, strided_x_pipe_kernel<T, N, MODE>,
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, strided_x_pipe_kernel<T, N, MODE, MIR>,
                                                      SGeo<T, N, MODE>::THREADS, 0) != hipSuccess ||
         per < 1 || cus < 1)
       return (int64_t)256;
     return (int64_t)per * cus;
   }();
   return resident;
+}
+
+// Sizes with size-specialised kernels.
+bool fast_size(int n) {
+  switch (n) {
+#define X(NN) case NN:
+    EVX_FAST_SIZES(X)
+#undef X
+    return true;
+    default:
+      return false;
+  }
 }
 
 template <typename T>
@@ -386,11 +400,16 @@ bool fast_strided(int n, void* spec, int64_t nlines, const LineMap& lm, const vo
     if (!brick_offsets_fit<T>(nlines, lm, NN, G::TPL, G::TL)) return false;                                 \
     const int64_t blocks = (nlines + G::TL - 1) / G::TL;                                                    \
     if constexpr (SWave<T, NN, MODE>::ON) {                                                                 \
-      const int64_t res = x_pipe_resident<T, NN, MODE>();                                                   \
+      const int64_t res = lm.mirror ? x_pipe_resident<T, NN, MODE, 1>() : x_pipe_resident<T, NN, MODE, 0>();  \
       const int64_t grid = blocks < res ? blocks : res;                                                     \
-      hipLaunchKernelGGL((strided_x_pipe_kernel<T, NN, MODE>), dim3((unsigned)grid), dim3(G::THREADS), 0, s, \
-                         static_cast<cpx<T>*>(spec), nlines, lm, static_cast<const cpx<T>*>(tw), pf,          \
-                         (uint32_t)blocks);                                                                 \
+      if (lm.mirror)                                                                                        \
+        hipLaunchKernelGGL((strided_x_pipe_kernel<T, NN, MODE, 1>), dim3((unsigned)grid), dim3(G::THREADS), 0, \
+                           s, static_cast<cpx<T>*>(spec), nlines, lm, static_cast<const cpx<T>*>(tw), pf,    \
+                           (uint32_t)blocks);                                                               \
+      else                                                                                                  \
+        hipLaunchKernelGGL((strided_x_pipe_kernel<T, NN, MODE, 0>), dim3((unsigned)grid), dim3(G::THREADS), 0, \
+                           s, static_cast<cpx<T>*>(spec), nlines, lm, static_cast<const cpx<T>*>(tw), pf,    \
+                           (uint32_t)blocks);                                                               \
     } else {                                                                                                \
       hipLaunchKernelGGL((strided_fast<T, NN, MODE>), dim3((unsigned)blocks), dim3(G::THREADS), 0, s,       \
                          static_cast<cpx<T>*>(spec), nlines, lm, static_cast<const cpx<T>*>(tw), pf);        \
@@ -588,13 +607,15 @@ int run_z_c2r(evx_fft_plan* p, const void* spec, void* out, const void* base, do
   return check_launch("z_c2r_kernel");
 }
 
+// mirror (x sweeps of the size-specialised kernels only): see LineMap.
 template <typename T, int MODE>
 int run_strided(evx_fft_plan* p, int axis, void* spec, const Prefac& pf, int64_t pitch, hipStream_t s,
-                int64_t xa = 0, int64_t xb = -1) {
+                int64_t xa = 0, int64_t xb = -1, int mirror = 0) {
   const AxisHost& h = p->ax[axis];
   const Dims& d = p->d;
   LineMap lm;
   lm.nh = p->nh;
+  lm.mirror = mirror;
   int64_t nlines;
   if (axis == 1) {
     nlines = d.c * d.nx * p->nh;
@@ -615,6 +636,7 @@ int run_strided(evx_fft_plan* p, int axis, void* spec, const Prefac& pf, int64_t
   }
   if (h.n == 1 && MODE < 2) return EVX_OK;  // length-1 transform is the identity
   if (p->fast && fast_strided<T, MODE>(h.n, spec, nlines, lm, h.tw, pf, s)) return check_launch("strided_fast");
+  if (mirror) return fail(EVX_ERR_UNSUPPORTED, "mirrored sweeps need a size-specialised transform");
   const Geom g = strided_geom<T>(h, nlines);
   const int64_t blocks = (nlines + g.lpw - 1) / g.lpw;
   hipLaunchKernelGGL((strided_kernel<T, MODE>), dim3((unsigned)blocks), dim3(g.threads), g.lds, s,
@@ -715,10 +737,10 @@ __global__ __launch_bounds__(256) void fold_recs(const SumRec* __restrict__ rec,
 
 // The x sweep: forward x transform, spectral multiplier, inverse x transform.
 template <typename T>
-int run_x_multiplier(evx_fft_plan* p, void* spec, const Prefac& pf, int64_t pt, hipStream_t s) {
-  if (pf.etd) return run_strided<T, 4>(p, 0, spec, pf, pt, s);
-  if (pf.exact) return run_strided<T, 3>(p, 0, spec, pf, pt, s);
-  return run_strided<T, 2>(p, 0, spec, pf, pt, s);
+int run_x_multiplier(evx_fft_plan* p, void* spec, const Prefac& pf, int64_t pt, hipStream_t s, int mirror = 0) {
+  if (pf.etd) return run_strided<T, 4>(p, 0, spec, pf, pt, s, 0, -1, mirror);
+  if (pf.exact) return run_strided<T, 3>(p, 0, spec, pf, pt, s, 0, -1, mirror);
+  return run_strided<T, 2>(p, 0, spec, pf, pt, s, 0, -1, mirror);
 }
 
 // Launch trace (armed by evx_fft_plan_trace_next_step).  Kernel kinds:
@@ -1114,6 +1136,11 @@ int mirror_spectral_step(evx_fft_plan* p, int ch, const void* u, void* u_out, vo
              : evx_rd_rhs(p->dtype, u, rhs, &g, bc, prm->D, &e, s));
   EVX_TRY(trace_tick(p, TK_RHS, s));
   const int mode = staggered ? 2 : (dir ? 1 : 0);
+  // Cell-centred extensions (N = 2n) on a size-specialised x transform: the
+  // x sweep reads the upper half as the mirror image of the lower half and
+  // stores the lower half only; otherwise the half spectrum is extended first.
+  const bool xmir = !staggered && p->fast && fast_size((int)NX);
+  const int xmode = xmir ? (dir ? 2 : 1) : 0;
   const int64_t ext = staggered ? n + 2 : n;  // extension planes
   const int64_t cplane = p->d.ny * p->pitch;   // complex elements per spectrum plane
   if (ext > 65535) return fail(EVX_ERR_UNSUPPORTED, "the mirrored spectral step takes at most 65533 x planes");
@@ -1130,11 +1157,13 @@ int mirror_spectral_step(evx_fft_plan* p, int ch, const void* u, void* u_out, vo
     EVX_TRY(trace_tick(p, TK_Z_R2C, s));
     EVX_TRY((run_strided<float, 0>(p, 1, spec, pf, pt, s, 0, n)));
     EVX_TRY(trace_tick(p, TK_Y_FWD, s));
-    hipLaunchKernelGGL(mirror_spec_x_kernel<float>, mgrid, dim3(256), 0, s, static_cast<cpx<float>*>(spec), n, cplane,
-                       mode);
-    EVX_TRY(check_launch("mirror_spec_x_kernel"));
-    EVX_TRY(trace_tick(p, TK_OTHER, s));
-    EVX_TRY(run_x_multiplier<float>(p, spec, pf, pt, s));
+    if (!xmir) {
+      hipLaunchKernelGGL(mirror_spec_x_kernel<float>, mgrid, dim3(256), 0, s, static_cast<cpx<float>*>(spec), n,
+                         cplane, mode);
+      EVX_TRY(check_launch("mirror_spec_x_kernel"));
+      EVX_TRY(trace_tick(p, TK_OTHER, s));
+    }
+    EVX_TRY(run_x_multiplier<float>(p, spec, pf, pt, s, xmode));
     EVX_TRY(trace_tick(p, TK_X, s));
     EVX_TRY((run_strided<float, 1>(p, 1, spec, pf, pt, s, 0, n)));
     EVX_TRY(trace_tick(p, TK_Y_INV, s));
@@ -1144,11 +1173,13 @@ int mirror_spectral_step(evx_fft_plan* p, int ch, const void* u, void* u_out, vo
     EVX_TRY(trace_tick(p, TK_Z_R2C, s));
     EVX_TRY((run_strided<double, 0>(p, 1, spec, pf, pt, s, 0, n)));
     EVX_TRY(trace_tick(p, TK_Y_FWD, s));
-    hipLaunchKernelGGL(mirror_spec_x_kernel<double>, mgrid, dim3(256), 0, s, static_cast<cpx<double>*>(spec), n,
-                       cplane, mode);
-    EVX_TRY(check_launch("mirror_spec_x_kernel"));
-    EVX_TRY(trace_tick(p, TK_OTHER, s));
-    EVX_TRY(run_x_multiplier<double>(p, spec, pf, pt, s));
+    if (!xmir) {
+      hipLaunchKernelGGL(mirror_spec_x_kernel<double>, mgrid, dim3(256), 0, s, static_cast<cpx<double>*>(spec), n,
+                         cplane, mode);
+      EVX_TRY(check_launch("mirror_spec_x_kernel"));
+      EVX_TRY(trace_tick(p, TK_OTHER, s));
+    }
+    EVX_TRY(run_x_multiplier<double>(p, spec, pf, pt, s, xmode));
     EVX_TRY(trace_tick(p, TK_X, s));
     EVX_TRY((run_strided<double, 1>(p, 1, spec, pf, pt, s, 0, n)));
     EVX_TRY(trace_tick(p, TK_Y_INV, s));

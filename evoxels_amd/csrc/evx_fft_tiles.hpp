@@ -18,6 +18,10 @@ namespace evx {
 // x: G = ny, GM = nx*ny*nh, OM = nh (outer = c*ny + y)
 struct LineMap {
   int64_t nh, G, GM, OM, es;
+  // x sweeps of a mirror-extended axis (N = 2n): 0 none; 1 element i >= n is
+  // element N-1-i (even extension), 2 its negative (odd extension).  The
+  // sweep then reads only the n stored planes and writes only them back.
+  int32_t mirror = 0;
 };
 
 __device__ __forceinline__ int64_t line_start(const LineMap& m, int64_t L) {
@@ -577,14 +581,22 @@ struct STile {
 // Element q of a lane sits at descriptor q < H ? lo : hi, scalar offset
 // (q mod H)*TPL*es and lane offset rel + tl*es (bytes; the host checks the
 // 4 GiB bound with brick_offsets_fit).
-template <typename T, int N, int MODE>
+// MIR: -1 the mirror mode of the LineMap decides at run time; 0 never
+// mirrored (no mirror code); 1 mirrored (the sign still from the LineMap).
+template <typename T, int N, int MODE, int MIR = -1>
 struct BrickIo {
   using G = SGeo<T, N, MODE>;
   static constexpr int TPL = G::TPL, TL = G::TL, E = N / TPL, H = (E + 1) / 2;
   __amdgpu_buffer_rsrc_t rlo, rhi;
-  uint32_t voff, qstep;
+  uint32_t voff, voff_hi, qstep;
+  int32_t mirror;
   bool active;
   BrickLine bl;
+
+  __device__ __forceinline__ int32_t mir() const {
+    if constexpr (MIR == 0) return 0;
+    else return mirror;
+  }
 
   __device__ __forceinline__ void setup(cpx<T>* spec, int64_t nlines, const LineMap& lm, uint32_t tile, int t,
                                         int tl) {
@@ -593,18 +605,43 @@ struct BrickIo {
     bl = brick_line(lm, L0, active ? t : 0);
     cpx<T>* const base = uniform_ptr(spec + bl.start0);
     qstep = (uint32_t)TPL * (uint32_t)lm.es * (uint32_t)sizeof(cpx<T>);
+    mirror = lm.mirror;
     rlo = raw_rsrc(base);
-    rhi = raw_rsrc(base + (int64_t)H * TPL * lm.es);
     voff = (bl.rel + (uint32_t)tl * (uint32_t)lm.es) * (uint32_t)sizeof(cpx<T>);
+    if (mir()) {
+      // element tl + TPL*q (q >= H) mirrors element N-1-tl-TPL*q =
+      // (TPL-1-tl) + TPL*(E-1-q), which lies in the lower half
+      rhi = rlo;
+      voff_hi = (bl.rel + (uint32_t)(TPL - 1 - tl) * (uint32_t)lm.es) * (uint32_t)sizeof(cpx<T>);
+    } else {
+      rhi = raw_rsrc(base + (int64_t)H * TPL * lm.es);
+      voff_hi = voff;  // not read when MIR == 0
+    }
   }
   __device__ __forceinline__ void load(cpx<T> (&v)[E]) const {
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = buf_load<T>(q < H ? rlo : rhi, voff, (uint32_t)(q < H ? q : q - H) * qstep);
+    for (int q = 0; q < H; ++q) v[q] = buf_load<T>(rlo, voff, (uint32_t)q * qstep);
+    if (mir()) {
+      static_assert(E % 2 == 0, "mirrored sweeps need an even number of values per thread");
+#pragma unroll
+      for (int q = H; q < E; ++q) v[q] = buf_load<T>(rhi, voff_hi, (uint32_t)(E - 1 - q) * qstep);
+      if (mir() == 2) {
+#pragma unroll
+        for (int q = H; q < E; ++q) v[q] = {-v[q].x, -v[q].y};
+      }
+    } else {
+#pragma unroll
+      for (int q = H; q < E; ++q) v[q] = buf_load<T>(rhi, MIR == 0 ? voff : voff_hi, (uint32_t)(q - H) * qstep);
+    }
   }
   __device__ __forceinline__ void store(const cpx<T> (&v)[E]) const {
     if (active) {
 #pragma unroll
-      for (int q = 0; q < E; ++q) buf_store<T>(v[q], q < H ? rlo : rhi, voff, (uint32_t)(q < H ? q : q - H) * qstep);
+      for (int q = 0; q < H; ++q) buf_store<T>(v[q], rlo, voff, (uint32_t)q * qstep);
+      if (!mir()) {  // a mirrored sweep keeps only the stored (lower) half
+#pragma unroll
+        for (int q = H; q < E; ++q) buf_store<T>(v[q], rhi, MIR == 0 ? voff : voff_hi, (uint32_t)(q - H) * qstep);
+      }
     }
   }
 };
@@ -689,7 +726,8 @@ __device__ __forceinline__ void strided_tile(cpx<T>* __restrict__ spec, int64_t 
 // elements tl + 64q of line t of a brick, at exchange slot line_slot(tl +
 // 64q) = line_slot(tl) + 68q).  Wavefront w transforms line L0 + w; line
 // L0 stands in past the last line, and its results are not stored.
-template <typename T, int N, int MODE>
+// MIR (0 or 1): the brick IO's compile-time mirror mode.
+template <typename T, int N, int MODE, int MIR = 0>
 __device__ __forceinline__ void strided_x_pipe(cpx<T>* __restrict__ spec, int64_t nlines, const LineMap& lm,
                                                const cpx<T>* __restrict__ twg, const Prefac& pf, uint32_t nbricks,
                                                uint32_t first, uint32_t step, int tid, unsigned char* smem) {
@@ -708,7 +746,7 @@ __device__ __forceinline__ void strided_x_pipe(cpx<T>* __restrict__ spec, int64_
   cpx<T>* const wl = lds + w * LSX + line_slot(l);
   const LineLay<T, LSX, true> lay{lds + w * LSX};
   uint32_t tile = first;
-  BrickIo<T, N, MODE> io;
+  BrickIo<T, N, MODE, MIR> io;
   io.setup(spec, nlines, lm, tile, t, tl);
   cpx<T> v[E];
   io.load(v);
@@ -718,7 +756,7 @@ __device__ __forceinline__ void strided_x_pipe(cpx<T>* __restrict__ spec, int64_
     __syncthreads();  // the brick (and the twiddles) are in LDS
     const uint32_t next = tile + step;
     const bool more = next < nbricks;  // uniform in the block
-    BrickIo<T, N, MODE> io2;
+    BrickIo<T, N, MODE, MIR> io2;
     if (more) {
       io2.setup(spec, nlines, lm, next, t, tl);
       io2.load(v);  // in flight during the transform below
