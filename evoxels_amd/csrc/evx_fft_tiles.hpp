@@ -605,7 +605,7 @@ struct BrickIo {
     bl = brick_line(lm, L0, active ? t : 0);
     cpx<T>* const base = uniform_ptr(spec + bl.start0);
     qstep = (uint32_t)TPL * (uint32_t)lm.es * (uint32_t)sizeof(cpx<T>);
-    mirror = lm.mirror;
+    mirror = MIR == 0 ? 0 : lm.mirror;
     rlo = raw_rsrc(base);
     voff = (bl.rel + (uint32_t)tl * (uint32_t)lm.es) * (uint32_t)sizeof(cpx<T>);
     if (mir()) {
@@ -615,7 +615,7 @@ struct BrickIo {
       voff_hi = (bl.rel + (uint32_t)(TPL - 1 - tl) * (uint32_t)lm.es) * (uint32_t)sizeof(cpx<T>);
     } else {
       rhi = raw_rsrc(base + (int64_t)H * TPL * lm.es);
-      voff_hi = voff;  // not read when MIR == 0
+      voff_hi = MIR == 0 ? 0u : voff;  // the unmirrored loads and stores use voff
     }
   }
   __device__ __forceinline__ void load(cpx<T> (&v)[E]) const {
